@@ -1,0 +1,171 @@
+"""Pin the FP64 KS oracle: reference golden vectors, closed forms, ETDRK4
+coefficient identities and convergence order."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import KSOracle, etdrk4_coefficients, ks_params
+
+T = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float64)  # noqa: E731
+CASES = [f"n{n}_{s}_{d}" for n in (32, 64) for s in ("ETD1", "ETRK2") for d in ("mixed", "twopi")]
+
+
+def _solver(key, g):
+    n = int(key.split("_")[0][1:])
+    scheme = key.split("_")[1]
+    return KSOracle(ks_params(T(g[f"{key}_L"]), resolution=n, dt=0.1, time_scheme=scheme)), T(g[f"{key}_L"])
+
+
+@pytest.mark.parametrize("key", CASES)
+def test_ks_steps_match_reference_golden(golden, key):
+    g = golden("ks")
+    so, L = _solver(key, g)
+    u, s, cu = T(g[f"{key}_u"]), T(g[f"{key}_s"]), T(g[f"{key}_cu"])
+    tol = dict(rtol=1e-11, atol=1e-12)
+    so.doublestep = False
+    np.testing.assert_allclose(so.step(u, L), g[f"{key}_single"], **tol)
+    so.doublestep = True
+    np.testing.assert_allclose(so.step(u, L), g[f"{key}_double"], **tol)
+    so.correction_s = s
+    np.testing.assert_allclose(so.step(u, L), g[f"{key}_double_s"], **tol)
+    so.correction_u = cu
+    np.testing.assert_allclose(so.step(u, L), g[f"{key}_double_s_cu"], **tol)
+
+
+@pytest.mark.parametrize("key", CASES)
+def test_ks_gradient_matches_reference_autograd(golden, key):
+    g = golden("ks")
+    so, L = _solver(key, g)
+    u = T(g[f"{key}_u"])
+    ss = T(g[f"{key}_gss"]).clone().requires_grad_(True)
+    tgt = T(g[f"{key}_gtarget"])
+    u0 = u.clone().requires_grad_(True)
+    cur, traj = u0, []
+    for k in range(ss.shape[1]):
+        so.correction_s = ss[:, k]
+        cur = so.step(cur, L)
+        traj.append(cur)
+    np.testing.assert_allclose(torch.stack(traj, 1).detach(), g[f"{key}_gtraj"], rtol=1e-10, atol=1e-11)
+    ((torch.stack(traj, 1) - tgt) ** 2).mean().backward()
+    np.testing.assert_allclose(u0.grad, g[f"{key}_gu0"], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(ss.grad, g[f"{key}_gs"], rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("scheme", ["ETD1", "ETRK2", "ETDRK4"])
+def test_single_mode_linear_decay(scheme):
+    # u = eps*cos(k x): the nonlinear term is O(eps^2); the mode evolves by
+    # e^{(k^2-k^4) dt} per sub-step.
+    n, L, dt = 64, 22.0, 0.25
+    so = KSOracle(ks_params([L], resolution=n, dt=dt, time_scheme=scheme))
+    so.doublestep = False
+    x = torch.arange(n, dtype=torch.float64) * (L / n)
+    for j in (1, 3, 5):
+        k = 2 * math.pi * j / L
+        eps = 1e-9
+        u = eps * torch.cos(k * x)[None]
+        out = so.step(u, so.domain_size) / eps
+        np.testing.assert_allclose(out, math.exp((k * k - k ** 4) * dt) * u / eps, rtol=0, atol=1e-7)
+
+
+@pytest.mark.parametrize("scheme", ["ETD1", "ETRK2", "ETDRK4"])
+def test_mean_conserved_and_constant_forcing(scheme):
+    n, L, dt = 64, 32.0, 0.2
+    so = KSOracle(ks_params([L, L], resolution=n, dt=dt, time_scheme=scheme))
+    so.doublestep = False
+    g = torch.Generator().manual_seed(5)
+    u = torch.randn(2, n, generator=g, dtype=torch.float64) * 0.3
+    out = so.step(u, so.domain_size)
+    np.testing.assert_allclose(out.mean(-1), u.mean(-1), rtol=0, atol=1e-13)
+    # constant forcing s = c adds c*dt to the mean (k = 0: E = 1, phi1 = dt,
+    # and f1 + 4 f2 + f3 = dt for ETDRK4)
+    so.correction_s = torch.full((2, n), 0.7, dtype=torch.float64)
+    out = so.step(u, so.domain_size)
+    np.testing.assert_allclose(out.mean(-1), u.mean(-1) + 0.7 * dt, rtol=0, atol=1e-12)
+
+
+def test_etdrk4_coefficients_against_direct_and_taylor():
+    h = 0.25
+    big = torch.tensor([-4.0, -10.0, -100.0, -16000.0, 3.0], dtype=torch.float64) / h
+    e, e2, q, f1, f2, f3 = etdrk4_coefficients(big, h)
+    z = h * big
+    ez = torch.exp(z)
+    np.testing.assert_allclose(q, h * (torch.exp(z / 2) - 1) / z, rtol=1e-12)
+    np.testing.assert_allclose(f1, h * (-4 - z + ez * (4 - 3 * z + z * z)) / z ** 3, rtol=1e-10)
+    np.testing.assert_allclose(f2, h * (2 + z + ez * (z - 2)) / z ** 3, rtol=1e-10)
+    np.testing.assert_allclose(f3, h * (-4 - 3 * z - z * z + ez * (4 - z)) / z ** 3, rtol=1e-10)
+    # small |z|: Taylor series of phi functions
+    small = torch.tensor([0.0, 1e-8, -1e-4, 2e-3, -2.4e-3], dtype=torch.float64) / h
+    _, _, q, f1, f2, f3 = etdrk4_coefficients(small, h)
+    z = h * small
+    # series through z^3 (next terms are below 1e-11 relative here)
+    np.testing.assert_allclose(q, h * (1 / 2 + z / 8 + z ** 2 / 48 + z ** 3 / 384), rtol=1e-11)
+    np.testing.assert_allclose(f1, h * (1 / 6 + z / 6 + 3 * z ** 2 / 40 + z ** 3 / 45), rtol=1e-10)
+    np.testing.assert_allclose(f2, h * (1 / 6 + z / 12 + z ** 2 / 40 + z ** 3 / 180), rtol=1e-10)
+    np.testing.assert_allclose(f3, h * (1 / 6 - z ** 2 / 120 - z ** 3 / 360), rtol=1e-10)
+
+
+def test_etdrk4_phi_identity():
+    h = 0.25
+    lop = torch.linspace(-500, 0.3, 997, dtype=torch.float64)
+    e, _, _, f1, f2, f3 = etdrk4_coefficients(lop, h)
+    safe = torch.where(lop == 0, torch.ones_like(lop), lop)
+    want = torch.where(lop == 0, torch.full_like(lop, h), (e - 1) / safe)
+    np.testing.assert_allclose(f1 + 4 * f2 + f3, want, rtol=1e-11, atol=1e-14)
+
+
+def test_etdrk4_fourth_order_convergence():
+    # smooth KS solution over T = 1 on L = 22: error of ETDRK4 against a
+    # fine-step reference drops by ~16x per halving of dt.
+    n, L, tend = 64, 22.0, 1.0
+    x = torch.arange(n, dtype=torch.float64) * (L / n)
+    u0 = (torch.cos(2 * math.pi * x / L) * (1 + torch.sin(2 * math.pi * x / L)))[None] * 0.8
+
+    def run(dt):
+        so = KSOracle(ks_params([L], resolution=n, dt=dt, time_scheme="ETDRK4", dealias=False))
+        so.doublestep = False
+        u = u0
+        for _ in range(int(round(tend / dt))):
+            u = so.step(u, so.domain_size)
+        return u
+
+    ref = run(1 / 1024)
+    errs = [float((run(dt) - ref).norm()) for dt in (1 / 16, 1 / 32, 1 / 64)]
+    ratios = [errs[i] / errs[i + 1] for i in range(2)]
+    # order 4 means a ratio of 16 asymptotically; order 3 would give 8
+    assert ratios[0] > 11 and ratios[1] > 13, (errs, ratios)
+
+
+def test_ks_fft_is_textbook_dft():
+    # the oracle's torch.fft agrees with a brute-force O(N^2) DFT
+    n = 32
+    g = torch.Generator().manual_seed(9)
+    u = torch.randn(n, generator=g, dtype=torch.float64)
+    j = torch.arange(n, dtype=torch.float64)
+    w = torch.exp(-2j * math.pi * torch.outer(j, j) / n)
+    np.testing.assert_allclose(torch.fft.fft(u), w @ u.to(torch.complex128), rtol=0, atol=1e-12)
+
+
+def test_ks_gradient_central_difference_etdrk4():
+    n, L = 32, 22.0
+    so = KSOracle(ks_params([L, 30.0], resolution=n, dt=0.25, time_scheme="ETDRK4"))
+    g = torch.Generator().manual_seed(4)
+    u = torch.randn(2, n, generator=g, dtype=torch.float64) * 0.5
+    s = torch.randn(2, n, generator=g, dtype=torch.float64) * 0.1
+    w = torch.randn(2, n, generator=g, dtype=torch.float64)
+
+    def f(uu, sv):
+        so.correction_s = sv
+        return (so.step(uu, so.domain_size) * w).sum()
+
+    u0 = u.clone().requires_grad_(True)
+    s0 = s.clone().requires_grad_(True)
+    f(u0, s0).backward()
+    for _ in range(4):
+        du = torch.randn(2, n, generator=g, dtype=torch.float64)
+        ds = torch.randn(2, n, generator=g, dtype=torch.float64)
+        h = 1e-6
+        fd = (f(u + h * du, s + h * ds) - f(u - h * du, s - h * ds)) / (2 * h)
+        an = (u0.grad * du).sum() + (s0.grad * ds).sum()
+        np.testing.assert_allclose(fd.item(), an.item(), rtol=1e-7)
