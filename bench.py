@@ -1,0 +1,179 @@
+"""Benchmark: KS ETDRK4 forward+adjoint rollouts on one MI355X.
+
+Workload (plain KS, fp32 state, ETDRK4 with 2/3 dealiasing): B = 256
+independent trajectories, N = 256, L = 64, dt = 0.25, 1000 steps, seeded
+u0 = sum_{k=1..10} b_k cos(2 pi k x / L + psi_k), b_k ~ N(0, 0.5^2).
+One bench step = one forward rollout of 1000 steps (every step's state
+written, as simulate_rollout returns them) plus the adjoint rollout of a
+trajectory cotangent back to u0, through the drop-in API
+(``simulate_rollout`` + ``torch.autograd.backward``).
+
+Prints one JSON line.  value = trajectory-steps per second, forward+adjoint
+(B * 1000 per bench step).  roofline: the dominant kernel's algorithmic HBM
+bytes (adjoint: read u^n, read lambda^{n+1}, write lambda^n = 3*4*N bytes
+per trajectory-step; forward: 2*4*N) over its duration measured with HIP
+events on the launch stream, against 8 TB/s.  cpu_baseline: the FP64 oracle
+(PyTorch CPU restatement of the same scheme) on a bounded sample, fwd+adj.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+B, N, L, DT, H = 256, 256, 64.0, 0.25, 1000
+HBM_PEAK_GBS = 8000.0
+
+
+def initial_state(b, n, L, seed, kmax=10):
+    import torch
+    gen = torch.Generator().manual_seed(seed)
+    x = torch.arange(n, dtype=torch.float64) * (L / n)
+    u = torch.zeros(b, n, dtype=torch.float64)
+    for k in range(1, kmax + 1):
+        amp = torch.randn(b, 1, generator=gen, dtype=torch.float64) * 0.5
+        ph = torch.rand(b, 1, generator=gen, dtype=torch.float64) * 2 * math.pi
+        u += amp * torch.cos(2 * math.pi * k * x / L + ph)
+    return u
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", "ks_adj_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(budget_s=12.0):
+    import torch
+    from oracle import KSOracle, ks_params
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    so = KSOracle(ks_params([L] * B, resolution=N, dt=DT, time_scheme="ETDRK4"))
+    so.doublestep = False
+    u0 = initial_state(B, N, L, 1001)
+
+    def run(h):
+        u = u0.clone().requires_grad_(True)
+        cur, outs = u, []
+        for _ in range(h):
+            cur = so.step(cur, so.domain_size)
+            outs.append(cur)
+        torch.autograd.backward(torch.stack(outs, 1), torch.ones(B, h, N, dtype=torch.float64))
+        return u.grad
+
+    t0 = time.perf_counter()
+    run(2)
+    t1 = time.perf_counter()
+    h = max(2, min(200, int(budget_s / max(t1 - t0, 1e-3) * 2)))
+    t0 = time.perf_counter()
+    run(h)
+    dt = time.perf_counter() - t0
+    return {"value": B * h / dt, "unit": "trajectory-steps/s (fwd+adj)", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle (torch CPU) ETDRK4, B={B}, N={N}, {h} steps fwd+adj, {dt:.1f} s"}
+
+
+def plain_path(solver, u0, Ld, ntraj=8, h=50):
+    """One launch per trajectory per step (no batching), fwd + adjoint."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b in range(ntraj):
+        u = u0[b:b + 1].clone().requires_grad_(True)
+        cur, outs = u, []
+        for _ in range(h):
+            cur = solver.step(cur, Ld)
+            outs.append(cur)
+        torch.autograd.backward(torch.stack(outs, 1), torch.ones(1, h, N, device=u0.device))
+    torch.cuda.synchronize()
+    return ntraj * h / (time.perf_counter() - t0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-trajectory plain path")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: trajectories are independent, a multi-GPU run would shard the "
+              "batch with no exchange on this path; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import inc_amd
+    dev = torch.device("cuda:0")
+    solver = inc_amd.KSSolverTorch(inc_amd.KSSimParams([L], resolution=N, dt=DT, time_scheme="ETDRK4",
+                                                       device="cuda"))
+    solver.doublestep = False
+    Ld = torch.tensor([L], device=dev)
+    u0 = initial_state(B, N, L, 1001).float().to(dev)
+    gen = torch.Generator().manual_seed(7)
+    cot = (torch.randn(B, H, N, generator=gen) * 1e-3).to(dev)
+
+    def one_step(ev=None):
+        u = u0.clone().requires_grad_(True)
+        if ev is not None:
+            ev[0].record()
+        traj, _ = inc_amd.simulate_rollout(None, solver, u, H, None, L=Ld)
+        if ev is not None:
+            ev[1].record()
+        torch.autograd.backward(traj, cot)
+        if ev is not None:
+            ev[2].record()
+        return u
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        u = one_step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_fwd = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps * 1e-3
+    t_adj = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps * 1e-3
+    assert torch.isfinite(u.grad).all()
+
+    ts = B * H
+    value = ts * args.steps / wall
+    fwd_bytes, adj_bytes = 2 * 4 * N * ts, 3 * 4 * N * ts
+    dom_is_adj = t_adj >= t_fwd
+    dom_t, dom_bytes = (t_adj, adj_bytes) if dom_is_adj else (t_fwd, fwd_bytes)
+    achieved = dom_bytes / dom_t / 1e9
+    out = {
+        "metric": "fwd+adjoint trajectory-steps/s, batched vs plain, 1 MI355X; % HBM roofline",
+        "value": value, "unit": "trajectory-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded low-mode cosine initial states)",
+        "config": {"workload": "KS ETDRK4 plain rollout fwd+adj: B=256, N=256, L=64, dt=0.25, 1000 steps, "
+                               "2/3 dealias, fp32", "batch": B, "n": N, "horizon": H},
+        "roofline": {"bound": "hbm", "kernel": "ks_adj_kernel" if dom_is_adj else "ks_fwd_kernel",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic()},
+        "forward_trajectory_steps_per_s": ts / t_fwd,
+        "kernel_ms": {"fwd": t_fwd * 1e3, "adj": t_adj * 1e3},
+    }
+    if not args.no_plain:
+        plain = plain_path(solver, u0, Ld)
+        out["plain_trajectory_steps_per_s"] = plain
+        out["batched_over_plain"] = value / plain
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

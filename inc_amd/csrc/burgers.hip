@@ -1,0 +1,896 @@
+// Burgers WENO5 / FDM step kernels for gfx950 (forward and adjoint).
+//
+// Replaces the eager op sequence of BurgersSolverTorch (reference
+// solvers/solver_1d.py:44-195) and weno_reconstruction (:4-42).
+//
+// Layout: a trajectory is a row of n points in HBM; a team of `team`
+// threads (a power of two <= 256) owns one trajectory and each thread owns
+// PPT consecutive points, kept in registers across stages and steps.  The
+// stage state is staged through LDS so neighbours can read the periodic
+// 6-point WENO window; the Lax-Friedrichs alpha = max|u| (and its argmax,
+// for the adjoint) is a wavefront shuffle reduction plus an LDS combine
+// across the team's wavefronts.
+#include "common.hpp"
+
+namespace inc {
+namespace {
+
+// ---- WENO5 tables ----------------------------------------------------------
+// Candidate k of direction d: q (value), a and b (linear forms with
+// beta = 13/12 a^2 + 1/4 b^2) over 5 stencil taps.  Taps are offsets
+// -2..2 for 'left' (solver_1d.py:14-22) and -1..3 for 'right'
+// (solver_1d.py:24-33; the reference's right reconstruction is the left one
+// shifted by a cell, with beta_2's b form mirrored).
+__device__ constexpr double kQ[2][3][5] = {
+    {{1. / 3, -7. / 6, 11. / 6, 0, 0}, {0, -1. / 6, 5. / 6, 1. / 3, 0}, {0, 0, 1. / 3, 5. / 6, -1. / 6}},
+    {{1. / 3, -7. / 6, 11. / 6, 0, 0}, {0, -1. / 6, 5. / 6, 1. / 3, 0}, {0, 0, 1. / 3, 5. / 6, -1. / 6}}};
+__device__ constexpr double kA[2][3][5] = {
+    {{1, -2, 1, 0, 0}, {0, 1, -2, 1, 0}, {0, 0, 1, -2, 1}},
+    {{1, -2, 1, 0, 0}, {0, 1, -2, 1, 0}, {0, 0, 1, -2, 1}}};
+__device__ constexpr double kB[2][3][5] = {
+    {{1, -4, 3, 0, 0}, {0, 1, 0, -1, 0}, {0, 0, 3, -4, 1}},
+    {{1, -4, 3, 0, 0}, {0, -1, 0, 1, 0}, {0, 0, 1, -4, 3}}};
+__device__ constexpr double kGamma[3] = {0.1, 0.6, 0.3};
+constexpr double kEps = 1e-6;  // solver_1d.py:5
+
+template <int D, int K, typename T>
+__device__ __forceinline__ T lin_q(const T* f) {
+  T acc = T(0);
+#pragma unroll
+  for (int m = 0; m < 5; ++m)
+    if (kQ[D][K][m] != 0.0) acc += T(kQ[D][K][m]) * f[m];
+  return acc;
+}
+template <int D, int K, typename T>
+__device__ __forceinline__ T lin_a(const T* f) {
+  T acc = T(0);
+#pragma unroll
+  for (int m = 0; m < 5; ++m)
+    if (kA[D][K][m] != 0.0) acc += T(kA[D][K][m]) * f[m];
+  return acc;
+}
+template <int D, int K, typename T>
+__device__ __forceinline__ T lin_b(const T* f) {
+  T acc = T(0);
+#pragma unroll
+  for (int m = 0; m < 5; ++m)
+    if (kB[D][K][m] != 0.0) acc += T(kB[D][K][m]) * f[m];
+  return acc;
+}
+
+template <typename T>
+struct WenoParts {
+  T q[3], a[3], b[3], d[3], w[3], F;
+};
+
+// Nonlinear weights and reconstructed value for one 5-tap stencil.
+template <int D, typename T>
+__device__ __forceinline__ void weno_parts(const T* f, WenoParts<T>& o) {
+  o.q[0] = lin_q<D, 0>(f); o.q[1] = lin_q<D, 1>(f); o.q[2] = lin_q<D, 2>(f);
+  o.a[0] = lin_a<D, 0>(f); o.a[1] = lin_a<D, 1>(f); o.a[2] = lin_a<D, 2>(f);
+  o.b[0] = lin_b<D, 0>(f); o.b[1] = lin_b<D, 1>(f); o.b[2] = lin_b<D, 2>(f);
+  T al[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    T beta = T(13.0 / 12.0) * o.a[k] * o.a[k] + T(0.25) * o.b[k] * o.b[k];
+    o.d[k] = T(kEps) + beta;
+    al[k] = T(kGamma[k]) * rcp(o.d[k] * o.d[k]);
+  }
+  T inv = rcp(al[0] + al[1] + al[2]);
+  o.w[0] = al[0] * inv; o.w[1] = al[1] * inv; o.w[2] = al[2] * inv;
+  o.F = o.w[0] * o.q[0] + o.w[1] * o.q[1] + o.w[2] * o.q[2];
+}
+
+template <int D, typename T>
+__device__ __forceinline__ T weno_value(const T* f) {
+  WenoParts<T> o;
+  weno_parts<D>(f, o);
+  return o.F;
+}
+
+// df[m] = gbar * dF/df_m.  With w_k = alpha_k / sum alpha,
+// alpha_k = gamma_k / d_k^2, d_k = eps + beta_k:
+//   dF/df_m = sum_k w_k dq_k/df_m + sum_k c_k dbeta_k/df_m,
+//   c_k = -2 w_k (q_k - F) / d_k,
+//   dbeta_k/df_m = 13/6 a_k da_k/df_m + 1/2 b_k db_k/df_m.
+template <int D, typename T>
+__device__ __forceinline__ void weno_vjp(const T* f, T gbar, T* df) {
+  WenoParts<T> o;
+  weno_parts<D>(f, o);
+  T pw[3], pa[3], pb[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    T ck = T(-2) * o.w[k] * (o.q[k] - o.F) * rcp(o.d[k]);
+    pw[k] = gbar * o.w[k];
+    pa[k] = gbar * ck * T(13.0 / 6.0) * o.a[k];
+    pb[k] = gbar * ck * T(0.5) * o.b[k];
+  }
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    T acc = T(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (kQ[D][k][m] != 0.0) acc += T(kQ[D][k][m]) * pw[k];
+      if (kA[D][k][m] != 0.0) acc += T(kA[D][k][m]) * pa[k];
+      if (kB[D][k][m] != 0.0) acc += T(kB[D][k][m]) * pb[k];
+    }
+    df[m] = acc;
+  }
+}
+
+__device__ __forceinline__ int wrap(int j, int n) {
+  return j < 0 ? j + n : (j >= n ? j - n : j);
+}
+
+// ---- standalone WENO reconstruction (weno_reconstruction) ------------------
+template <typename T, int D>
+__global__ void __launch_bounds__(256) weno_fwd_kernel(const T* __restrict__ f, T* __restrict__ out,
+                                                       int64_t rows, int n) {
+  int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= rows * n) return;
+  int64_t r = gid / n;
+  int i = int(gid - r * n);
+  const T* row = f + r * n;
+  T taps[5];
+  const int o0 = D == 0 ? -2 : -1;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) taps[m] = row[wrap(i + o0 + m, n)];
+  out[gid] = weno_value<D>(taps);
+}
+
+// g_f[j] = sum over reconstructions i whose stencil covers j.
+template <typename T, int D>
+__global__ void __launch_bounds__(256) weno_adj_kernel(const T* __restrict__ f, const T* __restrict__ g,
+                                                       T* __restrict__ gf, int64_t rows, int n) {
+  int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= rows * n) return;
+  int64_t r = gid / n;
+  int j = int(gid - r * n);
+  const T* row = f + r * n;
+  const T* grow = g + r * n;
+  const int o0 = D == 0 ? -2 : -1;
+  T acc = T(0);
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    // reconstruction i = j - (o0 + m) has tap m at point j
+    int i = wrap(j - (o0 + m), n);
+    T taps[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) taps[q] = row[wrap(i + o0 + q, n)];
+    T df[5];
+    weno_vjp<D>(taps, grow[i], df);
+    acc += df[m];
+  }
+  gf[gid] = acc;
+}
+
+// ---- Burgers step kernels ---------------------------------------------------
+template <typename T>
+struct BArgs {
+  int batch, n, J, team, steps, scheme, source, parts;
+  T dt, inv_dx, inv_dx2, half_inv_dx, nu, beta, t0, two_pi_over_L;
+  const T* nu_ptr;
+  const T* beta_ptr;
+  const T* t_ptr;
+  const T* x;
+  const T* A;
+  const T* omega;
+  const T* phi;
+  const T* l;
+  const T* u;
+  const T* s;
+  T* out;
+  T* traj;
+  T* stages;
+  const T* g_traj;
+  T* g_s;
+};
+
+constexpr int kBlock = 256;
+constexpr int kMaxWaves = kBlock / kWave;
+
+template <typename T>
+__device__ __forceinline__ bool better(T v, int i, T ov, int oi) {
+  // first maximal element; NaN wins (torch.max propagates NaN)
+  bool vn = v != v, on = ov != ov;
+  if (vn || on) return on && (!vn || oi < i);
+  return ov > v || (ov == v && oi < i);
+}
+
+template <typename T>
+struct TeamCtx {
+  int lt, team, wave_in_block, first_wave, i0, n, b;
+  bool active;
+};
+
+// Team-wide argmax of |u| over the team's points.  Writes per-wave partials
+// to LDS before the barrier the caller issues; finish after that barrier.
+template <typename T, int PPT>
+__device__ __forceinline__ void argmax_begin(const TeamCtx<T>& c, const T* u, T& v, int& idx,
+                                             T* red_v, int* red_i) {
+  v = T(-1);
+  idx = 0x7fffffff;
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    int i = c.i0 + p;
+    if (i < c.n) {
+      T a = fabs(u[p]);
+      if (better(v, idx, a, i)) { v = a; idx = i; }
+    }
+  }
+  const int width = c.team < kWave ? c.team : kWave;
+  for (int off = 1; off < width; off <<= 1) {
+    T ov = shfl_xor(v, off);
+    int oi = __shfl_xor(idx, off, kWave);
+    if (better(v, idx, ov, oi)) { v = ov; idx = oi; }
+  }
+  if (c.team > kWave && (threadIdx.x & (kWave - 1)) == 0) {
+    red_v[c.wave_in_block] = v;
+    red_i[c.wave_in_block] = idx;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void argmax_end(const TeamCtx<T>& c, T& v, int& idx, const T* red_v,
+                                           const int* red_i) {
+  if (c.team > kWave) {
+    v = red_v[c.first_wave];
+    idx = red_i[c.first_wave];
+    for (int w = 1; w < c.team / kWave; ++w) {
+      T ov = red_v[c.first_wave + w];
+      int oi = red_i[c.first_wave + w];
+      if (better(v, idx, ov, oi)) { v = ov; idx = oi; }
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T source_at(const BArgs<T>& a, int b, int i, T u, T t, T xi) {
+  if (a.source == INC_SRC_LINEAR) {
+    T be = a.beta_ptr ? a.beta_ptr[b] : a.beta;
+    return be * u;
+  }
+  if (a.source == INC_SRC_LEARNING) {
+    // solver_1d.py:82-84
+    T acc = T(0);
+    for (int j = 0; j < a.J; ++j) {
+      int o = b * a.J + j;
+      T arg = a.omega[o] * t + (a.two_pi_over_L * a.l[o]) * xi + a.phi[o];
+      acc += a.A[o] * sin(arg);
+    }
+    return acc;
+  }
+  return T(0);
+}
+
+// One spatial right-hand side evaluation for the team's own points:
+// R = convection + diffusion + source (+ s), restricted to `parts`.
+// Contains two block barriers; the caller issues the third before LDS reuse.
+template <typename T, int PPT, int CONV>
+__device__ __forceinline__ void rhs_eval(const BArgs<T>& a, const TeamCtx<T>& c, const T* u,
+                                         const T* sv, const T* xv, T t, T nu, int parts, T* us,
+                                         T* Fs, T* red_v, int* red_i, T* R) {
+  constexpr int W = PPT + 5;
+#pragma unroll
+  for (int p = 0; p < PPT; ++p)
+    if (c.i0 + p < c.n) us[c.i0 + p] = u[p];
+  T alpha = T(0);
+  int amax = 0;
+  if (CONV == INC_WENO) argmax_begin<T, PPT>(c, u, alpha, amax, red_v, red_i);
+  __syncthreads();
+  if (CONV == INC_WENO) argmax_end(c, alpha, amax, red_v, red_i);
+
+  T win[W];
+  if (c.active && c.i0 < c.n) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) win[k] = us[wrap(c.i0 - 2 + k, c.n)];
+  }
+  T F[PPT];
+  if (CONV == INC_WENO && (parts & INC_PART_CONVECTION) && c.active && c.i0 < c.n) {
+    T fp[W], fm[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      T f = T(0.5) * win[k] * win[k];
+      fp[k] = T(0.5) * (f + alpha * win[k]);
+      fm[k] = T(0.5) * (f - alpha * win[k]);
+    }
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      F[p] = weno_value<0>(fp + p) + weno_value<1>(fm + p + 1);
+      if (c.i0 + p < c.n) Fs[c.i0 + p] = F[p];
+    }
+  }
+  __syncthreads();
+  if (!(c.active && c.i0 < c.n)) return;
+  T Fm1 = T(0);
+  if (CONV == INC_WENO && (parts & INC_PART_CONVECTION)) Fm1 = Fs[wrap(c.i0 - 1, c.n)];
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    T r = T(0);
+    if (parts & INC_PART_CONVECTION) {
+      if (CONV == INC_WENO) {
+        r = -(F[p] - (p == 0 ? Fm1 : F[p - 1])) * a.inv_dx;
+      } else {
+        // -u * (D1 u), central difference (solver_1d.py:87-93,125-127)
+        r = -win[p + 2] * ((win[p + 3] - win[p + 1]) * a.half_inv_dx);
+      }
+    }
+    if (parts & INC_PART_DIFFUSION) r += nu * ((win[p + 1] - T(2) * win[p + 2] + win[p + 3]) * a.inv_dx2);
+    if (parts & INC_PART_SOURCE) {
+      r += source_at(a, c.b, c.i0 + p, win[p + 2], t, xv[p]);
+      if (sv) r += sv[p];
+    }
+    R[p] = r;
+  }
+}
+
+// cotangent of the right-hand side: ub = (dR/du)^T cb for the team's points.
+template <typename T, int PPT, int CONV>
+__device__ __forceinline__ void rhs_vjp(const BArgs<T>& a, const TeamCtx<T>& c, const T* u,
+                                        const T* cb, T nu, int parts, T* us, T* cs, T* C,
+                                        T* red_v, int* red_i, T* ub) {
+  constexpr int W = PPT + 5;
+  const int n = c.n;
+  const bool own = c.active && c.i0 < n;
+#pragma unroll
+  for (int p = 0; p < PPT; ++p)
+    if (c.i0 + p < n) { us[c.i0 + p] = u[p]; cs[c.i0 + p] = cb[p]; }
+  T alpha = T(0);
+  int amax = 0;
+  const bool conv = parts & INC_PART_CONVECTION;
+  if (CONV == INC_WENO && conv) argmax_begin<T, PPT>(c, u, alpha, amax, red_v, red_i);
+  __syncthreads();
+  if (CONV == INC_WENO && conv) argmax_end(c, alpha, amax, red_v, red_i);
+
+  T uw[W], cw[PPT + 2];
+  T abar = T(0);
+  if (own) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) uw[k] = us[wrap(c.i0 - 2 + k, n)];
+#pragma unroll
+    for (int k = 0; k < PPT + 2; ++k) cw[k] = cs[wrap(c.i0 - 1 + k, n)];
+  }
+  if (own && conv) {
+    if (CONV == INC_WENO) {
+      T fp[W], fm[W];
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        T f = T(0.5) * uw[k] * uw[k];
+        fp[k] = T(0.5) * (f + alpha * uw[k]);
+        fm[k] = T(0.5) * (f - alpha * uw[k]);
+      }
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        const int i = c.i0 + p;
+        // N_i = -(F_i - F_{i-1})/dx  =>  Fbar_i = (c_{i+1} - c_i)/dx
+        T Fbar = (cw[p + 2] - cw[p + 1]) * a.inv_dx;
+        T dfp[5], dfm[5];
+        weno_vjp<0>(fp + p, Fbar, dfp);      // taps i-2..i+2
+        weno_vjp<1>(fm + p + 1, Fbar, dfm);  // taps i-1..i+3
+        // d f+/du = (u + alpha)/2, d f-/du = (u - alpha)/2,
+        // d f+/dalpha = u/2, d f-/dalpha = -u/2
+#pragma unroll
+        for (int m = -2; m <= 3; ++m) {
+          T gp = (m <= 2) ? dfp[m + 2] : T(0);
+          T gm = (m >= -1) ? dfm[m + 1] : T(0);
+          T uu = uw[p + 2 + m];
+          T contrib = T(0.5) * (gp * (uu + alpha) + gm * (uu - alpha));
+          abar += T(0.5) * uu * (gp - gm);
+          if (i < n) C[(m + 2) * n + i] = contrib;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        const int i = c.i0 + p;
+        if (i < n) {
+          T ci = cw[p + 1];
+          T ui = uw[p + 2];
+          C[0 * n + i] = ci * ui * a.half_inv_dx;                       // to u_{i-1}
+          C[1 * n + i] = -ci * (uw[p + 3] - uw[p + 1]) * a.half_inv_dx;  // to u_i
+          C[2 * n + i] = -ci * ui * a.half_inv_dx;                      // to u_{i+1}
+        }
+      }
+    }
+  }
+  if (CONV == INC_WENO && conv) {
+    const int width = c.team < kWave ? c.team : kWave;
+    abar = wave_sum(abar, width);
+    if (c.team > kWave && (threadIdx.x & (kWave - 1)) == 0) red_v[kMaxWaves + c.wave_in_block] = abar;
+  }
+  __syncthreads();
+  if (CONV == INC_WENO && conv && c.team > kWave) {
+    abar = red_v[kMaxWaves + c.first_wave];
+    for (int w = 1; w < c.team / kWave; ++w) abar += red_v[kMaxWaves + c.first_wave + w];
+  }
+  if (!own) return;
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    const int i = c.i0 + p;
+    T g = T(0);
+    if (conv) {
+      if (CONV == INC_WENO) {
+#pragma unroll
+        for (int m = -2; m <= 3; ++m) g += C[(m + 2) * n + wrap(i - m, n)];
+        if (i == amax) {
+          T ui = uw[p + 2];
+          T sg = ui > T(0) ? T(1) : (ui < T(0) ? T(-1) : T(0));
+          g += abar * sg;
+        }
+      } else {
+        g += C[0 * n + wrap(i + 1, n)] + C[1 * n + i] + C[2 * n + wrap(i - 1, n)];
+      }
+    }
+    if (parts & INC_PART_DIFFUSION) g += nu * ((cw[p] - T(2) * cw[p + 1] + cw[p + 2]) * a.inv_dx2);
+    if ((parts & INC_PART_SOURCE) && a.source == INC_SRC_LINEAR) {
+      T be = a.beta_ptr ? a.beta_ptr[c.b] : a.beta;
+      g += be * cw[p + 1];
+    }
+    ub[p] = g;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ TeamCtx<T> make_ctx(int team, int n, int batch, int ppt) {
+  TeamCtx<T> c;
+  c.team = team;
+  int tpb = kBlock / team;
+  int tid = threadIdx.x;
+  int t = tid / team;
+  c.lt = tid - t * team;
+  c.b = blockIdx.x * tpb + t;
+  c.active = c.b < batch;
+  c.wave_in_block = tid / kWave;
+  c.first_wave = (t * team) / kWave;
+  c.i0 = c.lt * ppt;
+  c.n = n;
+  return c;
+}
+
+// Shared-memory layout per team (nb = number of n-sized arrays).
+template <typename T>
+__device__ __forceinline__ T* team_lds(T* base, int team, int n, int nb) {
+  int t = threadIdx.x / team;
+  return base + size_t(t) * nb * n;
+}
+
+template <typename T, int PPT, int CONV>
+__global__ void __launch_bounds__(kBlock) burgers_fwd_kernel(BArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ T red_v[2 * kMaxWaves];
+  __shared__ int red_i[kMaxWaves];
+  TeamCtx<T> c = make_ctx<T>(a.team, a.n, a.batch, PPT);
+  T* us = team_lds(reinterpret_cast<T*>(smem_raw), a.team, a.n, 2);
+  T* Fs = us + a.n;
+  const int n = a.n;
+  const bool own = c.active && c.i0 < n;
+  const int ns = a.scheme == INC_SSPRK3 ? 3 : 1;
+  const T nu = a.nu_ptr ? (c.active ? a.nu_ptr[c.b] : T(0)) : a.nu;
+  const T t0 = a.t_ptr ? (c.active ? a.t_ptr[c.b] : T(0)) : a.t0;
+  T u0[PPT], uc[PPT], sv[PPT], xv[PPT], R[PPT];
+  const size_t row = size_t(c.b) * n;
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    int i = c.i0 + p;
+    bool ok = own && i < n;
+    u0[p] = ok ? a.u[row + i] : T(0);
+    sv[p] = (ok && a.s) ? a.s[row + i] : T(0);
+    xv[p] = (ok && a.source == INC_SRC_LEARNING) ? a.x[i] : T(0);
+  }
+  for (int step = 0; step < a.steps; ++step) {
+    const T t = t0 + T(step) * a.dt;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) uc[p] = u0[p];
+    for (int st = 0; st < ns; ++st) {
+      if (a.stages && own) {
+        T* dst = a.stages + ((size_t(c.b) * a.steps + step) * ns + st) * n;
+#pragma unroll
+        for (int p = 0; p < PPT; ++p)
+          if (c.i0 + p < n) dst[c.i0 + p] = uc[p];
+      }
+      const T ts = st == 0 ? t : (st == 1 ? t + a.dt : t + T(0.5) * a.dt);
+      rhs_eval<T, PPT, CONV>(a, c, uc, a.s ? sv : nullptr, xv, ts, nu, a.parts, us, Fs, red_v,
+                             red_i, R);
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        T v = uc[p] + a.dt * R[p];  // forward-Euler stage u + dt*R(u)
+        if (ns == 1 || st == 0) uc[p] = v;
+        else if (st == 1) uc[p] = T(0.75) * u0[p] + T(0.25) * v;
+        else uc[p] = T(1.0 / 3.0) * u0[p] + T(2.0 / 3.0) * v;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) u0[p] = uc[p];
+    if (a.traj && own) {
+      T* dst = a.traj + (size_t(c.b) * a.steps + step) * n;
+#pragma unroll
+      for (int p = 0; p < PPT; ++p)
+        if (c.i0 + p < n) dst[c.i0 + p] = u0[p];
+    }
+  }
+  if (own) {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p)
+      if (c.i0 + p < n) a.out[row + c.i0 + p] = u0[p];
+  }
+}
+
+// Right-hand side parts of one state (calc_nonlinear / calc_linear / source).
+template <typename T, int PPT, int CONV>
+__global__ void __launch_bounds__(kBlock) burgers_rhs_kernel(BArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ T red_v[2 * kMaxWaves];
+  __shared__ int red_i[kMaxWaves];
+  TeamCtx<T> c = make_ctx<T>(a.team, a.n, a.batch, PPT);
+  T* us = team_lds(reinterpret_cast<T*>(smem_raw), a.team, a.n, 2);
+  T* Fs = us + a.n;
+  const int n = a.n;
+  const bool own = c.active && c.i0 < n;
+  const T nu = a.nu_ptr ? (c.active ? a.nu_ptr[c.b] : T(0)) : a.nu;
+  const T t0 = a.t_ptr ? (c.active ? a.t_ptr[c.b] : T(0)) : a.t0;
+  T u[PPT], sv[PPT], xv[PPT], R[PPT];
+  const size_t row = size_t(c.b) * n;
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    int i = c.i0 + p;
+    bool ok = own && i < n;
+    u[p] = ok ? a.u[row + i] : T(0);
+    sv[p] = (ok && a.s) ? a.s[row + i] : T(0);
+    xv[p] = (ok && a.source == INC_SRC_LEARNING) ? a.x[i] : T(0);
+  }
+  rhs_eval<T, PPT, CONV>(a, c, u, a.s ? sv : nullptr, xv, t0, nu, a.parts, us, Fs, red_v, red_i, R);
+  if (own) {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p)
+      if (c.i0 + p < n) a.out[row + c.i0 + p] = R[p];
+  }
+}
+
+template <typename T, int PPT, int CONV>
+__global__ void __launch_bounds__(kBlock) burgers_rhs_adj_kernel(BArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ T red_v[2 * kMaxWaves];
+  __shared__ int red_i[kMaxWaves];
+  TeamCtx<T> c = make_ctx<T>(a.team, a.n, a.batch, PPT);
+  T* us = team_lds(reinterpret_cast<T*>(smem_raw), a.team, a.n, 8);
+  T* cs = us + a.n;
+  T* C = cs + a.n;
+  const int n = a.n;
+  const bool own = c.active && c.i0 < n;
+  const T nu = a.nu_ptr ? (c.active ? a.nu_ptr[c.b] : T(0)) : a.nu;
+  T u[PPT], g[PPT], ub[PPT];
+  const size_t row = size_t(c.b) * n;
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    int i = c.i0 + p;
+    bool ok = own && i < n;
+    u[p] = ok ? a.u[row + i] : T(0);
+    g[p] = ok ? a.g_traj[row + i] : T(0);
+  }
+  rhs_vjp<T, PPT, CONV>(a, c, u, g, nu, a.parts, us, cs, C, red_v, red_i, ub);
+  if (own) {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p)
+      if (c.i0 + p < n) a.out[row + c.i0 + p] = ub[p];
+  }
+}
+
+// Reverse sweep over `steps` steps from the saved stage states.
+template <typename T, int PPT, int CONV>
+__global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ T red_v[2 * kMaxWaves];
+  __shared__ int red_i[kMaxWaves];
+  TeamCtx<T> c = make_ctx<T>(a.team, a.n, a.batch, PPT);
+  T* us = team_lds(reinterpret_cast<T*>(smem_raw), a.team, a.n, 8);
+  T* cs = us + a.n;
+  T* C = cs + a.n;
+  const int n = a.n;
+  const bool own = c.active && c.i0 < n;
+  const int ns = a.scheme == INC_SSPRK3 ? 3 : 1;
+  const T nu = a.nu_ptr ? (c.active ? a.nu_ptr[c.b] : T(0)) : a.nu;
+  const T dt = a.dt;
+  const int parts = INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE;
+  T lam[PPT], gs[PPT], st[3][PPT], cb[PPT], r[PPT], b1[PPT], b2[PPT];
+  auto gload = [&](int step, T* dst) {
+    const T* src = a.g_traj + (size_t(c.b) * a.steps + step) * n;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      int i = c.i0 + p;
+      dst[p] = (own && i < n) ? src[i] : T(0);
+    }
+  };
+  gload(a.steps - 1, lam);
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) gs[p] = T(0);
+  for (int step = a.steps - 1; step >= 0; --step) {
+    for (int k = 0; k < ns; ++k) {
+      const T* src = a.stages + ((size_t(c.b) * a.steps + step) * ns + k) * n;
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        int i = c.i0 + p;
+        st[k][p] = (own && i < n) ? src[i] : T(0);
+      }
+    }
+    if (ns == 1) {
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) cb[p] = dt * lam[p];
+      rhs_vjp<T, PPT, CONV>(a, c, st[0], cb, nu, parts, us, cs, C, red_v, red_i, r);
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) { gs[p] += cb[p]; lam[p] = lam[p] + r[p]; }
+    } else {
+      // u' = u0/3 + 2/3 (u2 + dt R(u2))
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) cb[p] = T(2.0 / 3.0) * dt * lam[p];
+      rhs_vjp<T, PPT, CONV>(a, c, st[2], cb, nu, parts, us, cs, C, red_v, red_i, r);
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        gs[p] += cb[p];
+        b2[p] = T(2.0 / 3.0) * lam[p] + r[p];
+        lam[p] = T(1.0 / 3.0) * lam[p];  // running cotangent of u0
+      }
+      // u2 = 3/4 u0 + 1/4 (u1 + dt R(u1))
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) cb[p] = T(0.25) * dt * b2[p];
+      rhs_vjp<T, PPT, CONV>(a, c, st[1], cb, nu, parts, us, cs, C, red_v, red_i, r);
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        gs[p] += cb[p];
+        b1[p] = T(0.25) * b2[p] + r[p];
+        lam[p] += T(0.75) * b2[p];
+      }
+      // u1 = u0 + dt R(u0)
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) cb[p] = dt * b1[p];
+      rhs_vjp<T, PPT, CONV>(a, c, st[0], cb, nu, parts, us, cs, C, red_v, red_i, r);
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        gs[p] += cb[p];
+        lam[p] += b1[p] + r[p];
+      }
+    }
+    if (step > 0) {
+      gload(step - 1, cb);
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) lam[p] += cb[p];
+    }
+  }
+  if (own) {
+    const size_t row = size_t(c.b) * n;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      int i = c.i0 + p;
+      if (i < n) {
+        a.out[row + i] = lam[p];
+        if (a.g_s) a.g_s[row + i] = gs[p];
+      }
+    }
+  }
+}
+
+// ---- host side ---------------------------------------------------------------
+int pick_ppt(int n, int team_req, int* team_out) {
+  // Default: 4 points per thread and team = next power of two of n/4; above
+  // 1024 points the team stays at 256 threads and each owns 8 or 16 points.
+  static const int ppts[] = {1, 2, 4, 8, 16};
+  if (team_req > 0) {
+    if (team_req > kBlock || (team_req & (team_req - 1)) != 0) return -1;
+    for (int ppt : ppts) {
+      if (ppt * team_req >= n) {
+        *team_out = team_req;
+        return ppt;
+      }
+    }
+    return -1;
+  }
+  for (int ppt : ppts) {
+    if (ppt < 4) continue;
+    int need = (n + ppt - 1) / ppt;
+    if (need > kBlock) continue;
+    int team = 1;
+    while (team < need) team <<= 1;
+    *team_out = team;
+    return ppt;
+  }
+  return -1;
+}
+
+template <typename T>
+int fill_args(const inc_burgers_desc* d, BArgs<T>& a) {
+  INC_REQUIRE(d->n >= 8, INC_EINVAL, "resolution must be >= 8 (got %d)", d->n);
+  INC_REQUIRE(d->batch >= 0, INC_EINVAL, "batch must be >= 0");
+  INC_REQUIRE(d->scheme == INC_EULER || d->scheme == INC_SSPRK3, INC_EINVAL,
+              "unknown time scheme %d", d->scheme);
+  INC_REQUIRE(d->convection == INC_WENO || d->convection == INC_FDM, INC_EINVAL,
+              "Convection scheme not recognized");
+  INC_REQUIRE(d->source >= INC_SRC_NONE && d->source <= INC_SRC_LEARNING, INC_EINVAL,
+              "Unknown equation type.");
+  INC_REQUIRE(d->source != INC_SRC_LEARNING || (d->x && d->A && d->omega && d->phi && d->l) || d->J == 0,
+              INC_EINVAL, "'learning' source needs x, A, omega, phi, l");
+  INC_REQUIRE(d->dx > 0, INC_EINVAL, "dx must be positive");
+  a = BArgs<T>{};
+  a.batch = d->batch;
+  a.n = d->n;
+  a.J = d->J;
+  a.scheme = d->scheme;
+  a.source = d->source;
+  a.parts = INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE;
+  a.dt = T(d->dt);
+  a.inv_dx = T(1.0 / d->dx);
+  a.inv_dx2 = T(1.0 / (d->dx * d->dx));
+  a.half_inv_dx = T(0.5 / d->dx);
+  a.nu = T(d->nu);
+  a.beta = T(d->beta);
+  a.t0 = T(d->t0);
+  a.two_pi_over_L = T(2.0 * 3.14159265358979323846 / d->L);
+  a.nu_ptr = static_cast<const T*>(d->nu_ptr);
+  a.beta_ptr = static_cast<const T*>(d->beta_ptr);
+  a.t_ptr = static_cast<const T*>(d->t_ptr);
+  a.x = static_cast<const T*>(d->x);
+  a.A = static_cast<const T*>(d->A);
+  a.omega = static_cast<const T*>(d->omega);
+  a.phi = static_cast<const T*>(d->phi);
+  a.l = static_cast<const T*>(d->l);
+  return INC_OK;
+}
+
+enum class Kind { Fwd, Rhs, RhsAdj, Adj };
+
+template <typename T, int PPT, int CONV>
+void* kernel_ptr(Kind k) {
+  switch (k) {
+    case Kind::Fwd: return reinterpret_cast<void*>(&burgers_fwd_kernel<T, PPT, CONV>);
+    case Kind::Rhs: return reinterpret_cast<void*>(&burgers_rhs_kernel<T, PPT, CONV>);
+    case Kind::RhsAdj: return reinterpret_cast<void*>(&burgers_rhs_adj_kernel<T, PPT, CONV>);
+    default: return reinterpret_cast<void*>(&burgers_adj_kernel<T, PPT, CONV>);
+  }
+}
+
+template <typename T, int CONV>
+void* kernel_for_ppt(Kind k, int ppt) {
+  switch (ppt) {
+    case 1: return kernel_ptr<T, 1, CONV>(k);
+    case 2: return kernel_ptr<T, 2, CONV>(k);
+    case 4: return kernel_ptr<T, 4, CONV>(k);
+    case 8: return kernel_ptr<T, 8, CONV>(k);
+    default: return kernel_ptr<T, 16, CONV>(k);
+  }
+}
+
+template <typename T>
+int launch(Kind kind, const inc_burgers_desc* d, BArgs<T>& a, void* stream) {
+  int team = 0;
+  int ppt = pick_ppt(d->n, d->team, &team);
+  INC_REQUIRE(ppt > 0 && team >= 1 && team <= kBlock && (team & (team - 1)) == 0, INC_EUNSUPPORTED,
+              "no launch shape for n=%d team=%d (team must be a power of two <= 256, n <= 16*team)",
+              d->n, d->team);
+  a.team = team;
+  if (a.batch == 0) return INC_OK;
+  const int tpb = kBlock / team;
+  const int nb = (kind == Kind::Fwd || kind == Kind::Rhs) ? 2 : 8;
+  size_t lds = size_t(tpb) * nb * d->n * sizeof(T);
+  INC_REQUIRE(lds <= 160 * 1024 - 1024, INC_EUNSUPPORTED,
+              "n=%d needs %zu bytes of LDS per block (team=%d); use a larger team", d->n, lds, team);
+  void* fn = d->convection == INC_WENO ? kernel_for_ppt<T, INC_WENO>(kind, ppt)
+                                       : kernel_for_ppt<T, INC_FDM>(kind, ppt);
+  dim3 grid((a.batch + tpb - 1) / tpb), block(kBlock);
+  void* args[] = {&a};
+  INC_HIP(hipLaunchKernel(fn, grid, block, args, lds, static_cast<hipStream_t>(stream)));
+  return INC_OK;
+}
+
+template <typename T>
+int weno_launch(int direction, const void* f, const void* g, void* out, int64_t rows, int n,
+                void* stream, bool adj) {
+  INC_REQUIRE(direction == 0 || direction == 1, INC_EINVAL, "Direction must be 'left' or 'right'.");
+  INC_REQUIRE(n >= 5, INC_EINVAL, "weno_reconstruction needs at least 5 points (got %d)", n);
+  int64_t total = rows * int64_t(n);
+  if (total == 0) return INC_OK;
+  dim3 grid(unsigned((total + 255) / 256)), block(256);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const T* fp = static_cast<const T*>(f);
+  T* op = static_cast<T*>(out);
+  if (!adj) {
+    if (direction == 0) hipLaunchKernelGGL((weno_fwd_kernel<T, 0>), grid, block, 0, s, fp, op, rows, n);
+    else hipLaunchKernelGGL((weno_fwd_kernel<T, 1>), grid, block, 0, s, fp, op, rows, n);
+  } else {
+    const T* gp = static_cast<const T*>(g);
+    if (direction == 0) hipLaunchKernelGGL((weno_adj_kernel<T, 0>), grid, block, 0, s, fp, gp, op, rows, n);
+    else hipLaunchKernelGGL((weno_adj_kernel<T, 1>), grid, block, 0, s, fp, gp, op, rows, n);
+  }
+  return check_launch("weno");
+}
+
+template <typename T>
+int burgers_call(Kind kind, const inc_burgers_desc* d, int parts, const void* u, const void* s,
+                 void* out, int steps, void* traj, void* stages, const void* g_traj, void* g_s,
+                 void* stream) {
+  BArgs<T> a;
+  int rc = fill_args<T>(d, a);
+  if (rc) return rc;
+  a.parts = parts;
+  a.steps = steps;
+  a.u = static_cast<const T*>(u);
+  a.s = static_cast<const T*>(s);
+  a.out = static_cast<T*>(out);
+  a.traj = static_cast<T*>(traj);
+  a.stages = static_cast<T*>(stages);
+  a.g_traj = static_cast<const T*>(g_traj);
+  a.g_s = static_cast<T*>(g_s);
+  return launch<T>(kind, d, a, stream);
+}
+
+int dispatch(Kind kind, const inc_burgers_desc* d, int parts, const void* u, const void* s, void* out,
+             int steps, void* traj, void* stages, const void* g_traj, void* g_s, void* stream) {
+  clear_error();
+  INC_REQUIRE(d != nullptr, INC_EINVAL, "null descriptor");
+  if (d->dtype == INC_F32)
+    return burgers_call<float>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream);
+  if (d->dtype == INC_F64)
+    return burgers_call<double>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream);
+  set_error("unsupported dtype %d", d->dtype);
+  return INC_EINVAL;
+}
+
+}  // namespace
+}  // namespace inc
+
+using inc::Kind;
+
+extern "C" {
+
+int inc_weno_fwd(int dtype, int direction, const void* f, void* out, int64_t rows, int n,
+                 void* stream) {
+  inc::clear_error();
+  if (dtype == INC_F32) return inc::weno_launch<float>(direction, f, nullptr, out, rows, n, stream, false);
+  if (dtype == INC_F64) return inc::weno_launch<double>(direction, f, nullptr, out, rows, n, stream, false);
+  inc::set_error("unsupported dtype %d", dtype);
+  return INC_EINVAL;
+}
+
+int inc_weno_adj(int dtype, int direction, const void* f, const void* g_out, void* g_f, int64_t rows,
+                 int n, void* stream) {
+  inc::clear_error();
+  if (dtype == INC_F32) return inc::weno_launch<float>(direction, f, g_out, g_f, rows, n, stream, true);
+  if (dtype == INC_F64) return inc::weno_launch<double>(direction, f, g_out, g_f, rows, n, stream, true);
+  inc::set_error("unsupported dtype %d", dtype);
+  return INC_EINVAL;
+}
+
+int inc_burgers_rhs(const inc_burgers_desc* d, int parts, const void* u, const void* s, void* out,
+                    void* stream) {
+  return inc::dispatch(Kind::Rhs, d, parts, u, s, out, 1, nullptr, nullptr, nullptr, nullptr, stream);
+}
+
+int inc_burgers_rhs_adj(const inc_burgers_desc* d, int parts, const void* u, const void* g, void* g_u,
+                        void* stream) {
+  return inc::dispatch(Kind::RhsAdj, d, parts, u, nullptr, g_u, 1, nullptr, nullptr, g, nullptr, stream);
+}
+
+int inc_burgers_fwd(const inc_burgers_desc* d, const void* u, const void* s, void* u_out, int steps,
+                    void* traj, void* stages, void* stream) {
+  if (steps < 1) {
+    inc::set_error("steps must be >= 1 (got %d)", steps);
+    return INC_EINVAL;
+  }
+  return inc::dispatch(Kind::Fwd, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE, u, s,
+                       u_out, steps, traj, stages, nullptr, nullptr, stream);
+}
+
+int inc_burgers_adj(const inc_burgers_desc* d, const void* stages, const void* g_traj, void* g_u,
+                    void* g_s, int steps, void* stream) {
+  if (steps < 1) {
+    inc::set_error("steps must be >= 1 (got %d)", steps);
+    return INC_EINVAL;
+  }
+  return inc::dispatch(Kind::Adj, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE,
+                       nullptr, nullptr, g_u, steps, nullptr, const_cast<void*>(stages), g_traj, g_s,
+                       stream);
+}
+
+}  // extern "C"

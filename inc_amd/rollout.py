@@ -1,0 +1,90 @@
+"""Rollout driver with the reference's signature (trainer/trainer_1d.py:154-183).
+
+``simulate_rollout(model, solver, u_initial, mstep, correction_term, L)``
+returns (results [B, mstep, D], corrections [B, mstep, D] or None) exactly
+as the reference does.  Two MI355X additions:
+
+* with ``correction_term=None`` and an inc_amd solver on a fixed time step,
+  the whole rollout is ONE kernel launch (state resident in registers/LDS,
+  every step's output written once) and its gradient ONE adjoint launch;
+* ``checkpoint_every=k`` recomputes the INC rollout in segments of k steps
+  during the backward pass (segment-start states are the only activations
+  kept), for long unrolled horizons.
+"""
+import torch
+from torch.utils.checkpoint import checkpoint
+
+from .solver_1d import BurgersSolverTorch, KSSolverTorch
+
+
+def _clone_t(t):
+    return t.clone() if isinstance(t, torch.Tensor) else t
+
+
+def _fusable(solver, correction_term):
+    if correction_term is not None:
+        return False
+    if isinstance(solver, BurgersSolverTorch):
+        return solver.adaptive_CFL is None
+    if isinstance(solver, KSSolverTorch):
+        return solver.correction_u is None
+    return False
+
+
+def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, checkpoint_every=None):
+    """Unroll ``mstep`` solver steps (trainer_1d.py:154-183)."""
+    b, d = u_initial.shape
+    if correction_term not in (None, "INC"):
+        raise ValueError(f"Invalid correction term: {correction_term}")
+    solver.correction_s = None
+    if mstep <= 0:
+        empty = u_initial.new_zeros(b, 0, d)
+        return empty, (empty.clone() if correction_term else None)
+    if _fusable(solver, correction_term):
+        if isinstance(solver, KSSolverTorch):
+            return solver.rollout(u_initial, mstep, L), None
+        return solver.rollout(u_initial, mstep), None
+
+    model_fn = (lambda u: model(u)) if L is None else (lambda u: model(u, L))
+    step_fn = (lambda u: solver.step(u)) if L is None else (lambda u: solver.step(u, L))
+
+    def run(u, nsteps):
+        outs, corr = [], []
+        for _ in range(nsteps):
+            if correction_term == "INC":
+                s = model_fn(u)
+                solver.correction_s = s
+                corr.append(s)
+            u = step_fn(u)
+            outs.append(u)
+        res = torch.stack(outs, 1)
+        return res, (torch.stack(corr, 1) if corr else res.new_zeros(b, 0, d))
+
+    k = checkpoint_every
+    if not k or k >= mstep or not torch.is_grad_enabled():
+        res, corr = run(u_initial, mstep)
+        return res, (corr if correction_term == "INC" else None)
+
+    results, corrections = [], []
+    u = u_initial
+    for k0 in range(0, mstep, k):
+        n_seg = min(k, mstep - k0)
+        t_start = _clone_t(getattr(solver, "t", None))
+        state = {"first": True}
+
+        def seg(u_in, n_seg=n_seg, t_start=t_start, state=state):
+            t_after = getattr(solver, "t", None)
+            if t_start is not None:
+                solver.t = _clone_t(t_start)
+            out = run(u_in, n_seg)
+            if not state["first"] and t_start is not None:
+                solver.t = t_after  # recomputation must not move the solver clock
+            state["first"] = False
+            return out
+
+        res, corr = checkpoint(seg, u, use_reentrant=False)
+        results.append(res)
+        corrections.append(corr)
+        u = res[:, -1]
+    res = torch.cat(results, 1)
+    return res, (torch.cat(corrections, 1) if correction_term == "INC" else None)

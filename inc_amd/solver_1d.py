@@ -1,0 +1,540 @@
+"""Drop-in MI355X solvers: BurgersSolverTorch, KSSolverTorch,
+weno_reconstruction.
+
+Same constructors, attributes and method signatures as the reference's
+``solvers/solver_1d.py`` (weno_reconstruction :4-42, BurgersSolverTorch
+:44-195, KSSolverTorch :198-288).  Every step runs in libinc_amd (HIP,
+gfx950) through the C ABI of include/inc_amd.h; gradients come from the
+library's hand-written adjoint kernels behind ``torch.autograd.Function``s.
+Tensors must live on the GPU; nothing falls back to the CPU.
+"""
+import ctypes
+import math
+
+import torch
+
+from . import _lib as L
+
+__all__ = ["BurgersSolverTorch", "KSSolverTorch", "weno_reconstruction"]
+
+
+# ---------------------------------------------------------------------------
+# weno_reconstruction (solver_1d.py:4-42)
+# ---------------------------------------------------------------------------
+class _Weno(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, f, direction):
+        fc = f.contiguous()
+        out = torch.empty_like(fc)
+        n = fc.shape[-1]
+        rows = fc.numel() // n if n else 0
+        L.check(L.load().inc_weno_fwd(L.dtype_code(fc.dtype), direction, L.ptr(fc), L.ptr(out), rows, n,
+                                      L.stream()), "weno_reconstruction")
+        ctx.save_for_backward(fc)
+        ctx.direction = direction
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (f,) = ctx.saved_tensors
+        g = g.contiguous()
+        gf = torch.empty_like(f)
+        n = f.shape[-1]
+        rows = f.numel() // n if n else 0
+        L.check(L.load().inc_weno_adj(L.dtype_code(f.dtype), ctx.direction, L.ptr(f), L.ptr(g), L.ptr(gf),
+                                      rows, n, L.stream()), "weno_reconstruction backward")
+        return gf, None
+
+
+def weno_reconstruction(f, direction="left"):
+    """WENO5 reconstruction along the last axis with the reference's stencils
+    ('left': value at x_{i+1/2}; 'right': the reference's shifted stencil)."""
+    if direction not in ("left", "right"):
+        raise ValueError("Direction must be 'left' or 'right'.")
+    L.require_gpu(f)
+    return _Weno.apply(f, 0 if direction == "left" else 1)
+
+
+# ---------------------------------------------------------------------------
+# Burgers
+# ---------------------------------------------------------------------------
+_SOURCES = {"homogeneous": L.INC_SRC_NONE, "viscosity": L.INC_SRC_NONE,
+            "linear": L.INC_SRC_LINEAR, "learning": L.INC_SRC_LEARNING}
+
+
+class _BurgersAdvance(torch.autograd.Function):
+    """`steps` steps of the Burgers map; returns the final state, or the
+    [B, steps, N] trajectory when ``want_traj``."""
+
+    @staticmethod
+    def forward(ctx, u, s, solver, dt, steps, t, want_traj, need_stages):
+        u = u.contiguous()
+        s = s.contiguous() if s is not None else None
+        b, n = u.shape
+        desc, keep = solver._desc(u, dt, t)
+        out = torch.empty_like(u)
+        traj = u.new_empty(b, steps, n) if want_traj else None
+        ns = 3 if solver.time_scheme == "SSPRK3" else 1
+        stages = u.new_empty(b, steps, ns, n) if need_stages else None
+        rc = L.load().inc_burgers_fwd(ctypes.byref(desc), L.ptr(u), L.ptr(s), L.ptr(out), steps, L.ptr(traj),
+                                      L.ptr(stages), L.stream())
+        L.check(rc, "BurgersSolverTorch.step")
+        ctx.save_for_backward(stages)
+        ctx.desc, ctx.keep, ctx.steps, ctx.want_traj = desc, keep, steps, want_traj
+        ctx.shape = (b, n)
+        return traj if want_traj else out
+
+    @staticmethod
+    def backward(ctx, g):
+        (stages,) = ctx.saved_tensors
+        if stages is None:
+            raise RuntimeError("Burgers step was run without saved stages; enable grad before the step")
+        b, n = ctx.shape
+        if ctx.want_traj:
+            g_traj = g.contiguous()
+        elif ctx.steps == 1:
+            g_traj = g.contiguous().view(b, 1, n)
+        else:
+            g_traj = g.new_zeros(b, ctx.steps, n)
+            g_traj[:, -1] = g
+        g_u = torch.empty(b, n, dtype=g.dtype, device=g.device)
+        g_s = torch.empty_like(g_u) if ctx.needs_input_grad[1] else None
+        rc = L.load().inc_burgers_adj(ctypes.byref(ctx.desc), L.ptr(stages), L.ptr(g_traj), L.ptr(g_u),
+                                      L.ptr(g_s), ctx.steps, L.stream())
+        L.check(rc, "BurgersSolverTorch.step backward")
+        return g_u, g_s, None, None, None, None, None, None
+
+
+class _BurgersRHS(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u, solver, parts, t):
+        u = u.contiguous()
+        desc, keep = solver._desc(u, solver.dt, t)
+        out = torch.empty_like(u)
+        rc = L.load().inc_burgers_rhs(ctypes.byref(desc), parts, L.ptr(u), None, L.ptr(out), L.stream())
+        L.check(rc, "BurgersSolverTorch right-hand side")
+        ctx.save_for_backward(u)
+        ctx.desc, ctx.keep, ctx.parts = desc, keep, parts
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (u,) = ctx.saved_tensors
+        g = g.contiguous()
+        g_u = torch.empty_like(u)
+        rc = L.load().inc_burgers_rhs_adj(ctypes.byref(ctx.desc), ctx.parts, L.ptr(u), L.ptr(g), L.ptr(g_u),
+                                          L.stream())
+        L.check(rc, "BurgersSolverTorch right-hand side backward")
+        return g_u, None, None, None
+
+
+class BurgersSolverTorch:
+    """Burgers u_t + (u^2/2)_x = nu u_xx + source + s on a periodic grid
+    (solver_1d.py:44-195).
+
+    Differences from the reference, deliberate:
+      * the correction s (``correction_s``) enters the right-hand side as the
+        reference's callers intend (its ``_apply_corrections`` has the wrong
+        arity, solver_1d.py:116 vs :166,:188);
+      * ``sim_params.time_scheme`` may select 'SSPRK3' (default 'Euler', the
+        reference's update); the forcing is then evaluated at the stage times;
+      * the diffusion is the 3-point stencil, not a dense N x N product.
+    """
+
+    def __init__(self, sim_params, LOG=None):
+        self.param = sim_params
+        self.resolution = sim_params.resolution
+        self.dt = sim_params.dt
+        self.x = sim_params.x
+        self.t = 0.0
+        self.adaptive_CFL = sim_params.adaptive_CFL
+        self.correction_s = None
+        self.L = sim_params.L
+        self.nu = sim_params.nu
+        self.device = sim_params.device
+        self.dx = sim_params.L / sim_params.resolution
+        self.LOG = LOG
+        self.time_scheme = getattr(sim_params, "time_scheme", "Euler")
+        if self.time_scheme not in ("Euler", "SSPRK3"):
+            raise ValueError("time_scheme must be 'Euler' or 'SSPRK3'")
+        self.team = int(getattr(sim_params, "team", 0) or 0)
+        if sim_params.convection_term == "FDM":
+            self.calc_nonlinear = self.calc_nonlinear_FDM
+            self._conv = L.INC_FDM
+        elif sim_params.convection_term == "WENO":
+            self.calc_nonlinear = self.calc_nonlinear_weno
+            self._conv = L.INC_WENO
+        else:
+            raise ValueError("Convection scheme not recognized")
+        self._cache = {}
+
+    # -- descriptor ------------------------------------------------------------
+    def _per_traj(self, name, v, b, dtype, device):
+        """float -> (value, None); tensor with 1 or b entries -> (0, [b] tensor)."""
+        if not isinstance(v, torch.Tensor):
+            return float(v), None
+        key = (name, id(v), v._version, dtype, b)
+        hit = self._cache.get(name)
+        if hit is not None and hit[0] == key:
+            return 0.0, hit[1]
+        flat = v.detach().reshape(-1).to(device=device, dtype=dtype)
+        if flat.numel() == 1:
+            flat = flat.expand(b)
+        if flat.numel() != b:
+            raise ValueError(f"{name} must be a scalar or have one value per trajectory ([batch_size,1])")
+        flat = flat.contiguous()
+        self._cache[name] = (key, flat)
+        return 0.0, flat
+
+    def _forcing(self, b, dtype, device):
+        p = self.param
+        key = ("forcing", id(p.A), id(p.omega), id(p.phi), id(p.l), dtype, b)
+        hit = self._cache.get("forcing")
+        if hit is not None and hit[0] == key:
+            return hit[1]
+        J = p.J if not callable(getattr(p, "J", None)) else p.J()
+        arrs = []
+        for v in (p.A, p.omega, p.phi, p.l):
+            a = torch.as_tensor(v, device=device).to(dtype).reshape(b, -1)[:, :J].contiguous()
+            arrs.append(a)
+        self._cache["forcing"] = (key, (J, arrs))
+        return J, arrs
+
+    def _desc(self, u, dt, t, x=None):
+        b, n = u.shape
+        dtype, dev = u.dtype, u.device
+        d = L.BurgersDesc()
+        d.dtype = L.dtype_code(dtype)
+        d.scheme = L.INC_SSPRK3 if self.time_scheme == "SSPRK3" else L.INC_EULER
+        d.convection = self._conv
+        kind = self.param.equation_type
+        if kind not in _SOURCES:
+            raise ValueError("Unknown equation type.")
+        d.source = _SOURCES[kind]
+        d.batch, d.n, d.team = b, n, self.team
+        d.dt, d.dx, d.L = float(dt), float(self.dx), float(self.L)
+        keep = []
+        d.nu, nu_t = self._per_traj("nu", self.nu, b, dtype, dev)
+        if nu_t is not None:
+            if isinstance(self.nu, torch.Tensor) and self.nu.requires_grad:
+                raise ValueError("gradients with respect to nu are not supported by the HIP adjoint")
+            keep.append(nu_t)
+            d.nu_ptr = nu_t.data_ptr()
+        if d.source == L.INC_SRC_LINEAR:
+            d.beta, be_t = self._per_traj("beta", self.param.beta, b, dtype, dev)
+            if be_t is not None:
+                keep.append(be_t)
+                d.beta_ptr = be_t.data_ptr()
+        if isinstance(t, torch.Tensor):
+            if tuple(t.shape) not in ((b, 1), (b,)):
+                raise ValueError("t tensor must have shape [batch_size,1]")
+            tt = t.detach().reshape(-1).to(device=dev, dtype=dtype).contiguous()
+            keep.append(tt)
+            d.t_ptr = tt.data_ptr()
+        else:
+            d.t0 = float(t)
+        if d.source == L.INC_SRC_LEARNING:
+            xg = self.x if x is None else x
+            xg = torch.as_tensor(xg, device=dev).to(dtype).reshape(-1).contiguous()
+            if xg.numel() != n:
+                raise ValueError("x must have one coordinate per grid point")
+            J, arrs = self._forcing(b, dtype, dev)
+            keep += [xg] + arrs
+            d.J = J
+            d.x = xg.data_ptr()
+            d.A, d.omega, d.phi, d.l = (a.data_ptr() for a in arrs)
+        return d, keep
+
+    # -- reference API -----------------------------------------------------------
+    def initial_conditions(self, x=None, t=None):
+        """Compute the forcing field from the parameters (solver_1d.py:68-72)."""
+        return self._compute_delta(self.x if x is None else x, self.t if t is None else t)
+
+    def _compute_delta(self, x, t):
+        p = self.param
+        x = torch.as_tensor(x)
+        dev = x.device if x.is_cuda else torch.device(self.device)
+        dtype = x.dtype if x.dtype in (torch.float32, torch.float64) else torch.float32
+        if isinstance(t, torch.Tensor) and tuple(t.shape) != (p.batch_size, 1):
+            raise ValueError("t tensor must have shape [batch_size,1]")
+        z = torch.zeros(p.batch_size, x.numel(), device=dev, dtype=dtype)
+        L.require_gpu(z)
+        saved = p.equation_type
+        try:
+            p.equation_type = "learning"
+            desc, keep = self._desc(z, self.dt, t, x=x.to(dev))
+        finally:
+            p.equation_type = saved
+        out = torch.empty_like(z)
+        L.check(L.load().inc_burgers_rhs(ctypes.byref(desc), L.INC_PART_SOURCE, L.ptr(z), None, L.ptr(out),
+                                         L.stream()), "_compute_delta")
+        return out
+
+    def _compute_source(self, u):
+        kind = self.param.equation_type
+        if kind in ("homogeneous", "viscosity"):
+            return 0.0
+        if kind not in _SOURCES:
+            raise ValueError("Unknown equation type.")
+        L.require_gpu(u)
+        return _BurgersRHS.apply(u, self, L.INC_PART_SOURCE, self.t)
+
+    def _apply_corrections(self, source):
+        return source + self.correction_s if self.correction_s is not None else source
+
+    def calc_nonlinear_weno(self, u):
+        L.require_gpu(u)
+        if self._conv != L.INC_WENO:
+            raise ValueError("solver was built with convection_term='FDM'")
+        return _BurgersRHS.apply(u, self, L.INC_PART_CONVECTION, self.t)
+
+    def calc_nonlinear_FDM(self, u):
+        L.require_gpu(u)
+        if self._conv != L.INC_FDM:
+            raise ValueError("solver was built with convection_term='WENO'")
+        return _BurgersRHS.apply(u, self, L.INC_PART_CONVECTION, self.t)
+
+    def calc_linear(self, u):
+        L.require_gpu(u)
+        return _BurgersRHS.apply(u, self, L.INC_PART_DIFFUSION, self.t)
+
+    def _advance(self, u, dt, steps=1, want_traj=False):
+        L.require_gpu(u, self.correction_s)
+        s = self.correction_s
+        if s is not None:
+            s = s.to(dtype=u.dtype)
+        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
+        return _BurgersAdvance.apply(u, s, self, dt, steps, self.t, want_traj, need)
+
+    def step_adaptive(self, u):
+        """Sub-step to CFL * dx / max|u| over one dt (solver_1d.py:148-179):
+        a whole-batch max with a host sync per sub-step, capped at 1000."""
+        target = self.dt
+        n_sub = 0
+        max_u = None
+        while target > 0:
+            max_u = torch.max(torch.abs(u))
+            mu = max_u.item()
+            allowed = self.adaptive_CFL * self.dx / mu if mu > 0 else target
+            dt_loc = min(target, allowed)
+            u = self._advance(u, dt_loc)
+            self.t += dt_loc
+            target -= dt_loc
+            if int(math.ceil(target / dt_loc)) > 1000:
+                if self.LOG is not None:
+                    self.LOG.warning(f"Substeps {int(math.ceil(target / dt_loc))} exceeded max substeps (1000)")
+                break
+            n_sub += 1
+        if self.LOG is not None:
+            self.LOG.info(f"CFL: {self.adaptive_CFL}, using substeps: {n_sub}, time_step: {self.dt}, max_u: {max_u}")
+        return u
+
+    def step(self, u):
+        """Advance by one dt (solver_1d.py:181-195)."""
+        if self.adaptive_CFL is not None:
+            return self.step_adaptive(u)
+        u_new = self._advance(u, self.dt)
+        self.t += self.dt
+        if self.LOG is not None:
+            self.LOG.info(f"CFL = {torch.max(torch.abs(u)) * self.dt / self.dx:.2f}, time_step: {self.dt}")
+        return u_new
+
+    def rollout(self, u, mstep):
+        """``mstep`` steps in one launch with the correction held fixed;
+        returns the [B, mstep, N] trajectory (the fused path of
+        simulate_rollout)."""
+        if self.adaptive_CFL is not None:
+            raise ValueError("fused rollout needs a fixed time step (adaptive_CFL is set)")
+        traj = self._advance(u, self.dt, steps=mstep, want_traj=True)
+        for _ in range(mstep):
+            self.t += self.dt
+        return traj
+
+
+# ---------------------------------------------------------------------------
+# Kuramoto-Sivashinsky
+# ---------------------------------------------------------------------------
+_KS_SCHEMES = {"ETD1": L.INC_ETD1, "ETRK2": L.INC_ETRK2, "ETDRK4": L.INC_ETDRK4}
+_KS_NS = {L.INC_ETD1: 1, L.INC_ETRK2: 2, L.INC_ETDRK4: 4}
+
+
+class _KSAdvance(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u, s, solver, scheme, substeps, steps, domain_size, want_traj, need_stages):
+        u = u.contiguous()
+        s = s.contiguous() if s is not None else None
+        b, n = u.shape
+        desc, keep = solver._desc(u, scheme, substeps, domain_size)
+        out = torch.empty_like(u)
+        traj = u.new_empty(b, steps, n) if want_traj else None
+        nst = _KS_NS[scheme] * substeps
+        stages = u.new_empty(b, steps, nst, n) if need_stages else None
+        rc = L.load().inc_ks_fwd(ctypes.byref(desc), L.ptr(u), L.ptr(s), L.ptr(out), steps, L.ptr(traj),
+                                 L.ptr(stages), L.stream())
+        L.check(rc, "KSSolverTorch.step")
+        ctx.save_for_backward(stages)
+        ctx.desc, ctx.keep, ctx.steps, ctx.want_traj, ctx.shape = desc, keep, steps, want_traj, (b, n)
+        return traj if want_traj else out
+
+    @staticmethod
+    def backward(ctx, g):
+        (stages,) = ctx.saved_tensors
+        if stages is None:
+            raise RuntimeError("KS step was run without saved stages; enable grad before the step")
+        b, n = ctx.shape
+        if ctx.want_traj:
+            g_traj = g.contiguous()
+        elif ctx.steps == 1:
+            g_traj = g.contiguous().view(b, 1, n)
+        else:
+            g_traj = g.new_zeros(b, ctx.steps, n)
+            g_traj[:, -1] = g
+        g_u = torch.empty(b, n, dtype=g.dtype, device=g.device)
+        g_s = torch.empty_like(g_u) if ctx.needs_input_grad[1] else None
+        rc = L.load().inc_ks_adj(ctypes.byref(ctx.desc), L.ptr(stages), L.ptr(g_traj), L.ptr(g_u), L.ptr(g_s),
+                                 ctx.steps, L.stream())
+        L.check(rc, "KSSolverTorch.step backward")
+        return g_u, g_s, None, None, None, None, None, None, None
+
+
+class KSSolverTorch:
+    """Kuramoto-Sivashinsky u_t + u u_x + u_xx + u_xxxx = s on [0, L)
+    (solver_1d.py:198-288), pseudo-spectral exponential integrators.
+
+    ``time_scheme`` 'ETD1' / 'ETRK2' reproduce the reference (including the
+    double step, ``doublestep``); 'ETDRK4' adds Cox-Matthews ETDRK4 with
+    contour-integral coefficients and, when ``sim_params.dealias`` (default
+    True), the 2/3-rule mask on the nonlinear term.  ``resolution`` must be a
+    power of two in [16, 1024].
+    """
+
+    def __init__(self, sim_params):
+        self.resolution = sim_params.resolution
+        self.dt = sim_params.dt
+        self.doublestep = True
+        self.device = sim_params.device
+        self.domain_size = sim_params.domain_size
+        self.alpha = sim_params.alpha
+        self.correction_s = None
+        self.correction_u = None
+        self.dealias = bool(getattr(sim_params, "dealias", True))
+        self.radix = int(getattr(sim_params, "radix", 0) or 0)
+        self.fft_indices = torch.fft.fftfreq(self.resolution, d=1.0, device=self.device) * self.resolution
+        time_scheme_map = {"ETRK2": self.etrk2, "ETD1": self.etd1, "ETDRK4": self.etdrk4}
+        self.step = time_scheme_map[sim_params.time_scheme]
+        self.time_scheme = sim_params.time_scheme
+        self._tab = {}
+
+    def initial_conditions(self):
+        """cos(2 pi x) + alpha sin(2 pi x) + eps/||eps|| (solver_1d.py:214-223)."""
+        domain_size = self.domain_size.unsqueeze(1)
+        alpha = self.alpha.unsqueeze(1)
+        if len(alpha) != len(domain_size):
+            raise ValueError("Length of alpha and domain_size must be the same")
+        x = torch.linspace(0, 1, self.resolution, device=self.device)
+        u = torch.cos(2 * torch.pi * x) + alpha * torch.sin(2 * torch.pi * x)
+        eps = torch.randn_like(u)
+        return u + eps / torch.linalg.norm(eps)
+
+    def calc_linear(self, domain_size):
+        """(Lop, ik) spectra on [B, N] (solver_1d.py:235-240); a helper of the
+        reference API, not used by the HIP step."""
+        domain_size = torch.as_tensor(domain_size, device=self.device).unsqueeze(-1)
+        wavenumbers = (2j * torch.pi * self.fft_indices) / domain_size
+        return -wavenumbers ** 2 - wavenumbers ** 4, wavenumbers
+
+    def calc_nonlinear(self, u, wavenumbers):
+        """-ik/2 * fft(u^2) (solver_1d.py:225-233); a helper of the reference
+        API, not used by the HIP step."""
+        return -0.5 * wavenumbers * torch.fft.fft(u ** 2)
+
+    # -- tables and descriptor -----------------------------------------------------
+    def _tables(self, domain_size, b, dtype, dev, scheme, dt):
+        ds = domain_size if isinstance(domain_size, torch.Tensor) else torch.as_tensor(domain_size)
+        key = (ds.data_ptr() if ds.numel() else 0, ds._version, tuple(ds.shape), str(ds.device), dtype, scheme,
+               float(dt), self.dealias, b)
+        hit = self._tab.get(key)
+        if hit is not None:
+            return hit
+        Lf = ds.detach().reshape(-1).to(device=dev, dtype=torch.float64)
+        if Lf.numel() not in (1, b):
+            raise ValueError("domain_size must have one value or one per trajectory")
+        uniq, inv = torch.unique(Lf, return_inverse=True)
+        count = int(uniq.numel())
+        n = self.resolution
+        rows = n // 2 + 1
+        table = torch.empty(count * rows * L.INC_KS_TABLE_STRIDE + 2 * n, device=dev, dtype=dtype)
+        uniq = uniq.contiguous()
+        rc = L.load().inc_ks_tables(L.dtype_code(dtype), scheme, n, float(dt), int(self.dealias), L.ptr(uniq),
+                                    count, L.ptr(table), L.stream())
+        L.check(rc, "KSSolverTorch coefficient tables")
+        if count == 1:
+            idx = None
+        else:
+            idx = inv.to(torch.int32).expand(b).contiguous() if inv.numel() == 1 else inv.to(torch.int32).contiguous()
+        entry = (table, idx, uniq)
+        if len(self._tab) > 8:
+            self._tab.clear()
+        self._tab[key] = entry
+        return entry
+
+    def _desc(self, u, scheme, substeps, domain_size):
+        b, n = u.shape
+        if n != self.resolution:
+            raise ValueError(f"state has {n} points, solver resolution is {self.resolution}")
+        table, idx, uniq = self._tables(domain_size, b, u.dtype, u.device, scheme, self.dt)
+        d = L.KSDesc()
+        d.dtype = L.dtype_code(u.dtype)
+        d.scheme = scheme
+        d.batch, d.n = b, n
+        d.substeps = substeps
+        d.dealias = int(self.dealias)
+        d.radix = self.radix
+        d.dt = float(self.dt)
+        d.table = table.data_ptr()
+        d.twiddle = table.data_ptr() + (table.numel() - 2 * n) * table.element_size()
+        d.table_idx = idx.data_ptr() if idx is not None else None
+        return d, [table, idx, uniq]
+
+    def _advance(self, u, domain_size, scheme, substeps, steps=1, want_traj=False):
+        L.require_gpu(u, self.correction_s)
+        s = self.correction_s
+        if s is not None:
+            s = s.to(dtype=u.dtype)
+        if domain_size is None:
+            domain_size = self.domain_size
+        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
+        return _KSAdvance.apply(u, s, self, scheme, substeps, steps, domain_size, want_traj, need)
+
+    def _run(self, u, domain_size, scheme):
+        if self.correction_u is not None:
+            u = u + self.correction_u
+        return self._advance(u, domain_size, scheme, 2 if self.doublestep else 1)
+
+    # single sub-steps (solver_1d.py:242-278)
+    def etd1_step(self, u, domain_size):
+        return self._advance(u, domain_size, L.INC_ETD1, 1)
+
+    def etrk2_step(self, u, domain_size):
+        return self._advance(u, domain_size, L.INC_ETRK2, 1)
+
+    def etdrk4_step(self, u, domain_size):
+        return self._advance(u, domain_size, L.INC_ETDRK4, 1)
+
+    # step() bindings with correction_u and the double step (solver_1d.py:280-288)
+    def etd1(self, u, domain_size):
+        return self._run(u, domain_size, L.INC_ETD1)
+
+    def etrk2(self, u, domain_size):
+        return self._run(u, domain_size, L.INC_ETRK2)
+
+    def etdrk4(self, u, domain_size):
+        return self._run(u, domain_size, L.INC_ETDRK4)
+
+    def rollout(self, u, mstep, domain_size=None):
+        """``mstep`` step() calls in one launch with the correction held
+        fixed; returns the [B, mstep, N] trajectory."""
+        if self.correction_u is not None:
+            raise ValueError("fused rollout does not apply correction_u; use step()")
+        scheme = _KS_SCHEMES[self.time_scheme]
+        return self._advance(u, domain_size, scheme, 2 if self.doublestep else 1, steps=mstep, want_traj=True)

@@ -1,0 +1,173 @@
+/*
+ * inc_amd.h -- C ABI of the MI355X INC 1D solver-step library (libinc_amd.so).
+ *
+ * The reference exposes this path as a Python class API with no FFI
+ * (solvers/solver_1d.py: weno_reconstruction :4-42, BurgersSolverTorch
+ * :44-195, KSSolverTorch :198-288).  Each entry point below replaces the
+ * eager-PyTorch op sequence of one of those methods; the Python package
+ * inc_amd binds them with ctypes behind classes of the same names.
+ *
+ * Conventions
+ *  - All array arguments are device pointers (HBM), row-major, contiguous
+ *    along x: a batch of trajectories is [batch, n].  `dtype` selects
+ *    float32 (INC_F32) or float64 (INC_F64) for every floating array of a
+ *    call, except the KS domain sizes, which are always float64.
+ *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = the
+ *    null stream), allocates nothing and is safe under hipGraph capture.
+ *  - Return value: INC_OK (0) or an error code; inc_last_error() returns a
+ *    thread-local message for the last failing call of the calling thread.
+ *  - Calls that overlap on one stream are ordered; calls on different
+ *    streams that write the same buffers race (the caller orders them).
+ */
+#ifndef INC_AMD_H
+#define INC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { INC_OK = 0, INC_EINVAL = 1, INC_EHIP = 2, INC_EUNSUPPORTED = 3 };
+enum { INC_F32 = 0, INC_F64 = 1 };
+
+/* Burgers: time schemes, convection schemes, source kinds, RHS parts. */
+enum { INC_EULER = 0, INC_SSPRK3 = 1 };
+enum { INC_WENO = 0, INC_FDM = 1 };
+enum { INC_SRC_NONE = 0, INC_SRC_LINEAR = 1, INC_SRC_LEARNING = 2 };
+enum { INC_PART_CONVECTION = 1, INC_PART_DIFFUSION = 2, INC_PART_SOURCE = 4 };
+
+/* KS time schemes. */
+enum { INC_ETD1 = 0, INC_ETRK2 = 1, INC_ETDRK4 = 2 };
+
+/* Library version (major*10000 + minor*100 + patch). */
+int inc_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* inc_last_error(void);
+
+/*
+ * WENO5 reconstruction with the reference's stencils.
+ * Replaces weno_reconstruction(f, direction), solver_1d.py:4-42.
+ * direction: 0 = 'left' (value at x_{i+1/2}), 1 = 'right' (the reference's
+ * shifted stencil, value at x_{i+3/2}).  f, out: [rows, n], periodic in n.
+ */
+int inc_weno_fwd(int dtype, int direction, const void* f, void* out,
+                 int64_t rows, int n, void* stream);
+/* Vector-Jacobian product: g_f = (dF/df)^T g_out. */
+int inc_weno_adj(int dtype, int direction, const void* f, const void* g_out,
+                 void* g_f, int64_t rows, int n, void* stream);
+
+/*
+ * Description of a Burgers problem (constructor parameters of
+ * BurgersSolverTorch, solver_1d.py:45-66, plus the forcing parameters read
+ * by _compute_delta, solver_1d.py:74-85).
+ */
+typedef struct inc_burgers_desc {
+  int32_t dtype;      /* INC_F32 / INC_F64 */
+  int32_t scheme;     /* INC_EULER (reference) / INC_SSPRK3 */
+  int32_t convection; /* INC_WENO / INC_FDM */
+  int32_t source;     /* INC_SRC_* (equation_type, solver_1d.py:104-114) */
+  int32_t batch;      /* trajectories */
+  int32_t n;          /* grid points (resolution) */
+  int32_t J;          /* forcing modes ('learning') */
+  int32_t team;       /* threads per trajectory (0 = library default) */
+  double dt, dx, L;
+  double nu;          /* viscosity, used when nu_ptr == NULL */
+  double beta;        /* 'linear' source coefficient, when beta_ptr == NULL */
+  double t0;          /* time of the first step, when t_ptr == NULL */
+  const void* nu_ptr;   /* [batch] per-trajectory viscosity or NULL */
+  const void* beta_ptr; /* [batch] or NULL */
+  const void* t_ptr;    /* [batch] start time per trajectory or NULL */
+  const void* x;        /* [n] grid coordinates ('learning' only) */
+  const void* A;        /* [batch, J] forcing amplitudes */
+  const void* omega;    /* [batch, J] */
+  const void* phi;      /* [batch, J] */
+  const void* l;        /* [batch, J] */
+} inc_burgers_desc;
+
+/*
+ * Spatial right-hand side parts (bitmask of INC_PART_*):
+ * convection = calc_nonlinear_weno / calc_nonlinear_FDM (solver_1d.py:125-142),
+ * diffusion = calc_linear (:144-146), source = _compute_source (+ s if non-NULL).
+ * u, s, out: [batch, n]; the source is evaluated at time t0 / t_ptr.
+ */
+int inc_burgers_rhs(const inc_burgers_desc* d, int parts, const void* u,
+                    const void* s, void* out, void* stream);
+/* g_u = (d rhs_parts / du)^T g. */
+int inc_burgers_rhs_adj(const inc_burgers_desc* d, int parts, const void* u,
+                        const void* g, void* g_u, void* stream);
+
+/*
+ * Advance `steps` steps of size dt from u (BurgersSolverTorch.step,
+ * solver_1d.py:181-195; Euler as in the reference or SSP-RK3), with the
+ * correction s ([batch, n] or NULL) held fixed over the steps and the
+ * forcing clock starting at t0 / t_ptr.
+ *   u_out  [batch, n]               final state
+ *   traj   [batch, steps, n]        every step's output, or NULL
+ *   stages [batch, steps, ns, n]    stage states for the adjoint, or NULL
+ *          (ns = 1 for Euler, 3 for SSP-RK3)
+ */
+int inc_burgers_fwd(const inc_burgers_desc* d, const void* u, const void* s,
+                    void* u_out, int steps, void* traj, void* stages,
+                    void* stream);
+/*
+ * Adjoint of inc_burgers_fwd (the exact transpose of the reference's autograd
+ * through the same step): from the saved stages and the cotangents g_traj
+ * [batch, steps, n] of every step's output, returns g_u [batch, n] (cotangent
+ * of the input state) and g_s [batch, n] (of the fixed correction s; NULL to
+ * skip).
+ */
+int inc_burgers_adj(const inc_burgers_desc* d, const void* stages,
+                    const void* g_traj, void* g_u, void* g_s, int steps,
+                    void* stream);
+
+/*
+ * Description of a KS problem (KSSolverTorch, solver_1d.py:198-212).
+ * n must be a power of two, 16 <= n <= 1024.
+ */
+typedef struct inc_ks_desc {
+  int32_t dtype;      /* INC_F32 / INC_F64 */
+  int32_t scheme;     /* INC_ETD1 / INC_ETRK2 (reference) / INC_ETDRK4 */
+  int32_t batch;
+  int32_t n;
+  int32_t substeps;   /* sub-steps per step() call: 2 = reference double step */
+  int32_t dealias;    /* 2/3-rule mask on the nonlinear term (ETDRK4) */
+  int32_t radix;      /* points per lane of the FFT (0 = library default) */
+  int32_t reserved;
+  double dt;
+  const void* table;        /* coefficient tables from inc_ks_tables */
+  const void* twiddle;      /* FFT twiddles written by inc_ks_tables */
+  const int32_t* table_idx; /* [batch] table per trajectory, or NULL = table 0
+                               for all (then trajectories are FFT-packed in pairs) */
+} inc_ks_desc;
+
+/* Floats per wavenumber in a KS coefficient table. */
+#define INC_KS_TABLE_STRIDE 8
+/*
+ * Build coefficient tables, one per domain size in L ([count] float64,
+ * device): table_out holds count*(n/2+1)*INC_KS_TABLE_STRIDE elements of
+ * dtype (the tables, row f = |wavenumber index|) followed by 2n elements of
+ * FFT twiddles exp(-2 pi i k/n), whose address goes into inc_ks_desc.twiddle.
+ * ETD1/ETRK2: E, phi1, phi2 with the reference's branches
+ * (solver_1d.py:245-246, :259-267); ETDRK4: E, E2, Q, f1, f2, f3 by the
+ * contour integral; all evaluated in float64 on the device; the nonlinear
+ * factor k/2 (times the dealias mask) completes each row.
+ */
+int inc_ks_tables(int dtype, int scheme, int n, double dt, int dealias,
+                  const double* L, int count, void* table_out, void* stream);
+/*
+ * Advance `steps` step() calls (each `substeps` sub-steps) from u with the
+ * fixed correction s (or NULL).  Same output buffers as inc_burgers_fwd;
+ * stages: [batch, steps, substeps*ns, n] with ns = 1 (ETD1), 2 (ETRK2),
+ * 4 (ETDRK4) physical stage states.
+ */
+int inc_ks_fwd(const inc_ks_desc* d, const void* u, const void* s, void* u_out,
+               int steps, void* traj, void* stages, void* stream);
+/* Adjoint of inc_ks_fwd; same contract as inc_burgers_adj. */
+int inc_ks_adj(const inc_ks_desc* d, const void* stages, const void* g_traj,
+               void* g_u, void* g_s, int steps, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* INC_AMD_H */

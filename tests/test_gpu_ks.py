@@ -1,0 +1,259 @@
+"""GPU parity of the KS HIP kernels against the FP64 oracle and the reference
+golden vectors.
+
+Tolerances: float64 kernels agree with the oracle to rounding (1e-11
+relative L2 per step); float32 forward states within 1e-5 relative L2 per
+step (teacher-forced: each step starts from the oracle state) and float32
+gradients within 1e-4 relative L2 of the FP64 oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from inc_amd import KSSimParams, KSSolverTorch, simulate_rollout
+from inc_amd.correction import CorrectionCNN
+from oracle import KSOracle, ks_params
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+GOLD = [f"n{n}_{s}_{d}" for n in (32, 64) for s in ("ETD1", "ETRK2") for d in ("mixed", "twopi")]
+
+
+def modes(b, n, L, gen, kmax=10, sd=0.5):
+    x = torch.arange(n, dtype=torch.float64) * (L / n)
+    u = torch.zeros(b, n, dtype=torch.float64)
+    for k in range(1, kmax + 1):
+        a = torch.randn(b, 1, generator=gen, dtype=torch.float64) * sd
+        ps = torch.rand(b, 1, generator=gen, dtype=torch.float64) * 2 * math.pi
+        u += a * torch.cos(2 * math.pi * k * x / L + ps)
+    return u
+
+
+def gpu_solver(L, n, dt, scheme, dtype=torch.float64, radix=0, dealias=True):
+    p = KSSimParams(torch.as_tensor(L, dtype=torch.float64).reshape(-1).to("cuda"), resolution=n, dt=dt,
+                    time_scheme=scheme, device="cuda", dealias=dealias)
+    p.radix = radix
+    return KSSolverTorch(p)
+
+
+@pytest.mark.parametrize("key", GOLD)
+def test_ks_golden_steps_fp64(cuda, golden, key):
+    g = golden("ks")
+    n = int(key.split("_")[0][1:])
+    scheme = key.split("_")[1]
+    Lv = torch.as_tensor(g[f"{key}_L"])
+    sv = gpu_solver(Lv, n, 0.1, scheme)
+    u, s, cu = (torch.as_tensor(g[f"{key}_{k}"]).to(cuda) for k in ("u", "s", "cu"))
+    Ld = Lv.to(cuda)
+    tol = dict(rtol=1e-10, atol=1e-11)
+    sv.doublestep = False
+    np.testing.assert_allclose(sv.step(u, Ld).cpu(), g[f"{key}_single"], **tol)
+    sv.doublestep = True
+    np.testing.assert_allclose(sv.step(u, Ld).cpu(), g[f"{key}_double"], **tol)
+    sv.correction_s = s
+    np.testing.assert_allclose(sv.step(u, Ld).cpu(), g[f"{key}_double_s"], **tol)
+    sv.correction_u = cu
+    np.testing.assert_allclose(sv.step(u, Ld).cpu(), g[f"{key}_double_s_cu"], **tol)
+
+
+@pytest.mark.parametrize("key", GOLD)
+def test_ks_golden_gradients_fp64(cuda, golden, key):
+    g = golden("ks")
+    n = int(key.split("_")[0][1:])
+    scheme = key.split("_")[1]
+    Lv = torch.as_tensor(g[f"{key}_L"])
+    sv = gpu_solver(Lv, n, 0.1, scheme)
+    Ld = Lv.to(cuda)
+    u0 = torch.as_tensor(g[f"{key}_u"]).to(cuda).requires_grad_(True)
+    ss = torch.as_tensor(g[f"{key}_gss"]).to(cuda).requires_grad_(True)
+    cur, traj = u0, []
+    for k in range(ss.shape[1]):
+        sv.correction_s = ss[:, k]
+        cur = sv.step(cur, Ld)
+        traj.append(cur)
+    traj = torch.stack(traj, 1)
+    np.testing.assert_allclose(traj.detach().cpu(), g[f"{key}_gtraj"], rtol=1e-9, atol=1e-10)
+    ((traj - torch.as_tensor(g[f"{key}_gtarget"]).to(cuda)) ** 2).mean().backward()
+    np.testing.assert_allclose(u0.grad.cpu(), g[f"{key}_gu0"], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(ss.grad.cpu(), g[f"{key}_gs"], rtol=1e-9, atol=1e-12)
+
+
+RADIX_CASES = [(16, 4), (16, 16), (32, 8), (64, 4), (64, 16), (128, 4), (128, 8), (128, 16), (256, 4), (256, 8),
+               (256, 16), (512, 8), (512, 16), (1024, 16)]
+
+
+@pytest.mark.parametrize("n,radix", RADIX_CASES)
+@pytest.mark.parametrize("scheme", ["ETRK2", "ETDRK4"])
+def test_ks_step_every_fft_shape(cuda, n, radix, scheme):
+    # fp32 kernel for every (n, radix); odd batch exercises an unpaired
+    # trajectory; uniform L keeps the packed-pair path
+    b, L, dt = 5, 22.0 * n / 64, 0.1
+    gen = torch.Generator().manual_seed(n * 31 + radix)
+    u = modes(b, n, L, gen, kmax=min(10, n // 4))
+    s = 0.1 * torch.randn(b, n, generator=gen, dtype=torch.float64)
+    so = KSOracle(ks_params([L] * b, resolution=n, dt=dt, time_scheme=scheme))
+    so.doublestep = False
+    so.correction_s = s
+    ref = so.step(u, so.domain_size)
+    sv = gpu_solver([L], n, dt, scheme, dtype=torch.float32, radix=radix)
+    sv.doublestep = False
+    sv.correction_s = s.float().to(cuda)
+    out = sv.step(u.float().to(cuda), torch.tensor([L], device=cuda))
+    assert rel_l2(out, ref) < 1e-5
+
+
+@pytest.mark.parametrize("scheme", ["ETD1", "ETRK2", "ETDRK4"])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_ks_step_and_gradient_vs_oracle(cuda, scheme, mixed):
+    n, b, dt = 128, 6, 0.25
+    Ls = [22.0, 35.0, 64.0, 100.0, 22.0, 50.0] if mixed else [64.0] * b
+    gen = torch.Generator().manual_seed(17)
+    u = modes(b, n, 64.0, gen)
+    s = 0.1 * torch.randn(b, n, generator=gen, dtype=torch.float64)
+    w = torch.randn(b, n, generator=gen, dtype=torch.float64)
+    so = KSOracle(ks_params(Ls, resolution=n, dt=dt, time_scheme=scheme))
+    uo = u.clone().requires_grad_(True)
+    sso = s.clone().requires_grad_(True)
+    so.correction_s = sso
+    ref = so.step(uo, so.domain_size)
+    (ref * w).sum().backward()
+    for dtype in (torch.float64, torch.float32):
+        sv = gpu_solver(Ls, n, dt, scheme, dtype=dtype)
+        ug = u.to(cuda, dtype).requires_grad_(True)
+        sg = s.to(cuda, dtype).requires_grad_(True)
+        sv.correction_s = sg
+        out = sv.step(ug, torch.tensor(Ls, device=cuda))
+        (out * w.to(cuda, dtype)).sum().backward()
+        ftol, gtol = (1e-11, 1e-10) if dtype == torch.float64 else (1e-5, 1e-4)
+        assert rel_l2(out, ref.detach()) < ftol, ("forward", dtype)
+        assert rel_l2(ug.grad, uo.grad) < gtol, ("grad u", dtype)
+        assert rel_l2(sg.grad, sso.grad) < gtol, ("grad s", dtype)
+
+
+def test_ks_dealias_off_matches_oracle(cuda):
+    n, b, L, dt = 64, 4, 22.0, 0.25
+    gen = torch.Generator().manual_seed(3)
+    u = modes(b, n, L, gen)
+    so = KSOracle(ks_params([L] * b, resolution=n, dt=dt, time_scheme="ETDRK4", dealias=False))
+    so.doublestep = False
+    sv = gpu_solver([L], n, dt, "ETDRK4", dealias=False)
+    sv.doublestep = False
+    assert rel_l2(sv.step(u.to(cuda), torch.tensor([L], device=cuda)), so.step(u, so.domain_size)) < 1e-11
+
+
+def test_ks_fused_rollout_fp32_teacher_forced_and_gradient(cuda):
+    # config-1 shape, shortened: B=256, N=256, L=64, dt=0.25, ETDRK4, fp32
+    n, b, L, dt, H = 256, 256, 64.0, 0.25, 40
+    gen = torch.Generator().manual_seed(1001)
+    u0 = modes(b, n, L, gen)
+    sv = gpu_solver([L], n, dt, "ETDRK4", dtype=torch.float32)
+    sv.doublestep = False
+    ug = u0.float().to(cuda).requires_grad_(True)
+    traj, corr = simulate_rollout(None, sv, ug, H, None, L=torch.tensor([L], device=cuda))
+    assert corr is None and traj.shape == (b, H, n)
+    # teacher-forced one-step error against the FP64 oracle on a subset
+    so = KSOracle(ks_params([L] * 8, resolution=n, dt=dt, time_scheme="ETDRK4"))
+    so.doublestep = False
+    prev = u0[:8]
+    t64 = traj[:8].detach().double().cpu()
+    for k in range(H):
+        ref = so.step(prev, so.domain_size)
+        if k in (0, H // 2, H - 1):
+            # one fp32 step from the fp32 state vs one fp64 step from the same state
+            one = sv.step(prev.float().to(cuda), torch.tensor([L], device=cuda))
+            assert rel_l2(one, ref) < 1e-5, k
+        prev = t64[:, k]
+    # mean conserved over the whole rollout (size-independent property)
+    means = traj.detach().double().mean(-1)
+    assert (means - u0.to(cuda).mean(-1, keepdim=True)).abs().max() < 1e-5
+    # gradient of a trajectory loss: fused adjoint vs the step-by-step path
+    w = torch.randn(b, H, n, generator=gen).to(cuda)
+    (traj * w).sum().backward()
+    g_fused = ug.grad.clone()
+    ug2 = u0.float().to(cuda).requires_grad_(True)
+    cur, outs = ug2, []
+    for _ in range(H):
+        cur = sv.step(cur, torch.tensor([L], device=cuda))
+        outs.append(cur)
+    (torch.stack(outs, 1) * w).sum().backward()
+    assert rel_l2(g_fused, ug2.grad) < 1e-5
+
+
+def test_ks_rollout_gradient_matches_fp64_oracle(cuda):
+    n, b, L, dt, H = 64, 4, 22.0, 0.25, 12
+    gen = torch.Generator().manual_seed(44)
+    u0 = modes(b, n, L, gen, kmax=6)
+    s = 0.05 * torch.randn(b, n, generator=gen, dtype=torch.float64)
+    w = torch.randn(b, H, n, generator=gen, dtype=torch.float64)
+    so = KSOracle(ks_params([L] * b, resolution=n, dt=dt, time_scheme="ETDRK4"))
+    so.doublestep = False
+    uo = u0.clone().requires_grad_(True)
+    sso = s.clone().requires_grad_(True)
+    so.correction_s = sso
+    cur, outs = uo, []
+    for _ in range(H):
+        cur = so.step(cur, so.domain_size)
+        outs.append(cur)
+    (torch.stack(outs, 1) * w).sum().backward()
+    for dtype, tol in ((torch.float64, 1e-10), (torch.float32, 1e-4)):
+        sv = gpu_solver([L], n, dt, "ETDRK4", dtype=dtype)
+        sv.doublestep = False
+        ug = u0.to(cuda, dtype).requires_grad_(True)
+        sg = s.to(cuda, dtype).requires_grad_(True)
+        sv.correction_s = sg
+        traj = sv.rollout(ug, H, torch.tensor([L], device=cuda))
+        (traj * w.to(cuda, dtype)).sum().backward()
+        assert rel_l2(ug.grad, uo.grad) < tol, dtype
+        assert rel_l2(sg.grad, sso.grad) < tol, dtype
+
+
+def test_ks_inc_rollout_checkpointing_matches_full_graph(cuda):
+    # config-3 pattern, small: CNN correction, checkpoint every 4 steps
+    n, b, L, dt, H = 128, 8, 100.0, 0.25, 12
+    gen = torch.Generator().manual_seed(12)
+    u0 = modes(b, n, L, gen, kmax=16).float().to(cuda)
+    tgt = torch.randn(b, H, n, generator=gen).to(cuda) * 0.3
+    cnn = CorrectionCNN(depth=3, width=16, k=7, generator=torch.Generator().manual_seed(0)).to(cuda)
+    grads = []
+    for ck in (None, 4):
+        sv = gpu_solver([L], n, dt, "ETDRK4", dtype=torch.float32)
+        sv.doublestep = False
+        cnn.zero_grad()
+        u = u0.clone().requires_grad_(True)
+        res, corr = simulate_rollout(cnn, sv, u, H, "INC", L=torch.tensor([L], device=cuda), checkpoint_every=ck)
+        assert corr.shape == (b, H, n)
+        ((res - tgt) ** 2).mean().backward()
+        grads.append((res.detach(), u.grad.clone(), [p.grad.clone() for p in cnn.parameters()]))
+    (r0, g0, p0), (r1, g1, p1) = grads
+    assert rel_l2(r1, r0) < 1e-6
+    assert rel_l2(g1, g0) < 1e-5
+    for a, c in zip(p1, p0):
+        assert rel_l2(a, c) < 1e-5
+
+
+def test_ks_fd_gradient_fp64_32_directions(cuda):
+    # FP64 central differences on 32 random directions (config-3 check, small)
+    n, b, L, dt, H = 64, 2, 100.0, 0.25, 6
+    gen = torch.Generator().manual_seed(77)
+    u0 = modes(b, n, L, gen, kmax=16).to(cuda)
+    w = torch.randn(b, H, n, generator=gen, dtype=torch.float64).to(cuda)
+    sv = gpu_solver([L], n, dt, "ETDRK4")
+    sv.doublestep = False
+    Ld = torch.tensor([L], device=cuda)
+
+    def f(u):
+        return (sv.rollout(u, H, Ld) * w).sum()
+
+    ug = u0.clone().requires_grad_(True)
+    f(ug).backward()
+    worst = 0.0
+    with torch.no_grad():
+        for _ in range(32):
+            d = torch.randn(b, n, generator=gen, dtype=torch.float64).to(cuda)
+            h = 1e-6
+            fd = (f(u0 + h * d) - f(u0 - h * d)) / (2 * h)
+            an = (ug.grad * d).sum()
+            worst = max(worst, abs(float(fd - an)) / max(abs(float(an)), 1e-12))
+    assert worst < 1e-6

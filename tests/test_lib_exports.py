@@ -1,0 +1,72 @@
+"""CPU checks of the C ABI library: it loads, exports every symbol the header
+declares with the ctypes signature the package binds, and maps argument errors
+(no kernel launches, no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import inc_amd._lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "inc_amd.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(inc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_what_the_binding_binds():
+    assert header_functions() == sorted(L.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    for name in header_functions():
+        assert hasattr(lib, name), name
+    assert lib.inc_version() >= 100
+
+
+def test_struct_sizes_match_header_layout():
+    # int32 x8 + double x6 + pointer x8 = 32 + 48 + 64
+    assert ctypes.sizeof(L.BurgersDesc) == 144
+    # int32 x8 + double + pointer x3 = 32 + 8 + 24
+    assert ctypes.sizeof(L.KSDesc) == 64
+
+
+def test_argument_errors_map_to_codes_without_launch():
+    lib = L.load()
+    # bad WENO direction: rejected before any launch
+    rc = lib.inc_weno_fwd(L.INC_F32, 7, None, None, 1, 16, None)
+    assert rc == L.INC_EINVAL and b"left" in lib.inc_last_error()
+    with pytest.raises(ValueError):
+        L.check(rc, "weno")
+    # Burgers grid too small
+    d = L.BurgersDesc(dtype=L.INC_F32, batch=1, n=4, dx=0.1)
+    rc = lib.inc_burgers_fwd(ctypes.byref(d), None, None, None, 1, None, None, None)
+    assert rc == L.INC_EINVAL and b"resolution" in lib.inc_last_error()
+    # unknown equation type
+    d = L.BurgersDesc(dtype=L.INC_F32, batch=1, n=64, dx=0.1, source=9)
+    assert lib.inc_burgers_fwd(ctypes.byref(d), None, None, None, 1, None, None, None) == L.INC_EINVAL
+    # KS resolution not a power of two
+    k = L.KSDesc(dtype=L.INC_F32, scheme=L.INC_ETDRK4, batch=1, n=48, substeps=1)
+    rc = lib.inc_ks_fwd(ctypes.byref(k), None, None, None, 1, None, None, None)
+    assert rc == L.INC_EUNSUPPORTED and b"power of two" in lib.inc_last_error()
+    # adjoint without saved stages
+    k = L.KSDesc(dtype=L.INC_F32, scheme=L.INC_ETD1, batch=1, n=64, substeps=1, table=1)
+    assert lib.inc_ks_adj(ctypes.byref(k), None, None, None, None, 1, None) == L.INC_EINVAL
+    # successful-path state is reset by the next call
+    assert lib.inc_weno_fwd(L.INC_F32, 0, None, None, 0, 16, None) == L.INC_OK
+    assert lib.inc_last_error() == b""
+
+
+def test_product_refuses_cpu_tensors():
+    import torch
+    import inc_amd
+    with pytest.raises(ValueError, match="GPU"):
+        inc_amd.weno_reconstruction(torch.zeros(2, 16), "left")
+    with pytest.raises(ValueError):
+        inc_amd.weno_reconstruction(torch.zeros(2, 16), "up")
