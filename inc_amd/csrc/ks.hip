@@ -31,7 +31,7 @@ namespace inc {
 namespace {
 
 template <typename T>
-struct cpx {
+struct __attribute__((aligned(2 * sizeof(T)))) cpx {
   T x, y;
 };
 template <typename T>
@@ -176,6 +176,12 @@ struct Plan {
   static constexpr int NTW = plan_tw_off(N, R, NP) > 0 ? plan_tw_off(N, R, NP) : 1;
 };
 
+// LDS slot of complex element p: one padding slot per 32 elements keeps the
+// strided Stockham writes off a single bank
+__device__ __forceinline__ int lds_slot(int p) { return p + (p >> 5); }
+template <int N>
+constexpr int lds_team_elems() { return 2 * (N + N / 32); }
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -204,7 +210,7 @@ __device__ __forceinline__ void load_twiddles(const T* tab, int j, cpx<T>* tw) {
 }
 
 template <typename T, int N, int R, int SIGN, int S>
-__device__ __forceinline__ void fft_pass(cpx<T>* v, const cpx<T>* tw, T* lds, int j) {
+__device__ __forceinline__ void fft_pass(cpx<T>* v, const cpx<T>* tw, cpx<T>* lds, int j) {
   constexpr int TL = N / R;
   constexpr int NP = plan_passes(N, R);
   constexpr int Ns = plan_ns(N, R, S);
@@ -212,14 +218,13 @@ __device__ __forceinline__ void fft_pass(cpx<T>* v, const cpx<T>* tw, T* lds, in
   constexpr int Q = R / P;
   constexpr bool last = (S == NP - 1);
   constexpr int off = plan_tw_off(N, R, S);
+  constexpr int half = lds_team_elems<N>() / 2;
   if constexpr (S > 0) {
-    const T* re = lds + ((S - 1) & 1) * 2 * N;
-    const T* im = re + N;
+    const cpx<T>* rd = lds + ((S - 1) & 1) * half;
 #pragma unroll
-    for (int m = 0; m < R; ++m) v[m] = cpx<T>{re[j + m * TL], im[j + m * TL]};
+    for (int m = 0; m < R; ++m) v[m] = rd[lds_slot(j + m * TL)];
   }
-  T* wre = lds + (S & 1) * 2 * N;
-  T* wim = wre + N;
+  cpx<T>* wr = lds + (S & 1) * half;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     cpx<T> x[P];
@@ -240,17 +245,14 @@ __device__ __forceinline__ void fft_pass(cpx<T>* v, const cpx<T>* tw, T* lds, in
       const int jb = j + q * TL;
       const int base = (jb / Ns) * Ns * P + (jb % Ns);
 #pragma unroll
-      for (int r = 0; r < P; ++r) {
-        wre[base + r * Ns] = x[r].x;
-        wim[base + r * Ns] = x[r].y;
-      }
+      for (int r = 0; r < P; ++r) wr[lds_slot(base + r * Ns)] = x[r];
     }
   }
   if constexpr (!last) wave_sync();
 }
 
 template <typename T, int N, int R, int SIGN, int S>
-__device__ __forceinline__ void fft_passes(cpx<T>* v, const cpx<T>* tw, T* lds, int j) {
+__device__ __forceinline__ void fft_passes(cpx<T>* v, const cpx<T>* tw, cpx<T>* lds, int j) {
   if constexpr (S < plan_passes(N, R)) {
     fft_pass<T, N, R, SIGN, S>(v, tw, lds, j);
     fft_passes<T, N, R, SIGN, S + 1>(v, tw, lds, j);
@@ -260,7 +262,7 @@ __device__ __forceinline__ void fft_passes(cpx<T>* v, const cpx<T>* tw, T* lds, 
 // Unnormalised DFT (SIGN = -1) or inverse DFT without 1/N (SIGN = +1) of the
 // team's N values, strided layout in and out.
 template <typename T, int N, int R, int SIGN>
-__device__ __forceinline__ void fft(cpx<T>* v, const cpx<T>* tw, T* lds, int j) {
+__device__ __forceinline__ void fft(cpx<T>* v, const cpx<T>* tw, cpx<T>* lds, int j) {
   wave_sync();
   fft_passes<T, N, R, SIGN, 0>(v, tw, lds, j);
 }
@@ -339,7 +341,7 @@ __device__ __forceinline__ void store_pair(T* base, size_t stride_b, int b0, int
 // nonlinear term in the scaled convention: G~ * F(z^2) with G~ = -i g/N
 template <typename T, int N, int R>
 __device__ __forceinline__ void square_fft_g(cpx<T>* z, const Coef<T, N, R>& co, const cpx<T>* tw,
-                                             T* lds, int j) {
+                                             cpx<T>* lds, int j) {
 #pragma unroll
   for (int m = 0; m < R; ++m) z[m] = cpx<T>{z[m].x * z[m].x, z[m].y * z[m].y};
   fft<T, N, R, -1>(z, tw, lds, j);
@@ -353,7 +355,7 @@ __device__ __forceinline__ void square_fft_g(cpx<T>* z, const Coef<T, N, R>& co,
 // w-bar = Re F^H(conj(G~) X): multiply by +i g/N, inverse transform
 template <typename T, int N, int R>
 __device__ __forceinline__ void adj_g_ifft(const cpx<T>* X, cpx<T>* y, const Coef<T, N, R>& co,
-                                           const cpx<T>* tw, T* lds, int j) {
+                                           const cpx<T>* tw, cpx<T>* lds, int j) {
 #pragma unroll
   for (int m = 0; m < R; ++m) {
     T g = co.c[6][m];
@@ -379,11 +381,11 @@ __global__ void __launch_bounds__(64) ks_fwd_kernel(KArgs<T> a) {
   using PL = Plan<T, N, R>;
   constexpr int TL = PL::TL;
   constexpr int TPB = 64 / TL;
-  __shared__ T lds_all[TPB * 4 * N];
+  __shared__ cpx<T> lds_all[TPB * lds_team_elems<N>()];
   const int team = threadIdx.x / TL, j = threadIdx.x % TL;
   const int p = blockIdx.x * TPB + team;
   if (p >= a.npairs) return;
-  T* lds = lds_all + team * 4 * N;
+  cpx<T>* lds = lds_all + team * lds_team_elems<N>();
   int b0, b1;
   const int tab = pair_rows(a, p, b0, b1);
   cpx<T> tw[PL::NTW];
@@ -491,11 +493,11 @@ __global__ void __launch_bounds__(64) ks_adj_kernel(KArgs<T> a) {
   using PL = Plan<T, N, R>;
   constexpr int TL = PL::TL;
   constexpr int TPB = 64 / TL;
-  __shared__ T lds_all[TPB * 4 * N];
+  __shared__ cpx<T> lds_all[TPB * lds_team_elems<N>()];
   const int team = threadIdx.x / TL, j = threadIdx.x % TL;
   const int p = blockIdx.x * TPB + team;
   if (p >= a.npairs) return;
-  T* lds = lds_all + team * 4 * N;
+  cpx<T>* lds = lds_all + team * lds_team_elems<N>();
   int b0, b1;
   const int tab = pair_rows(a, p, b0, b1);
   cpx<T> tw[PL::NTW];
@@ -508,105 +510,127 @@ __global__ void __launch_bounds__(64) ks_adj_kernel(KArgs<T> a) {
   const size_t gstride = size_t(a.steps) * N;
   const size_t sstride = size_t(a.steps) * nst * N;
 
-  cpx<T> Vb[R], Sb[R], y[R], zs[R];
+  cpx<T> Vb[R], Sb[R], y[R];
 #pragma unroll
   for (int m = 0; m < R; ++m) Sb[m] = cpx<T>{T(0), T(0)};
   // cotangent of the last output u^H = F^H(V^H): Vbar = F(gbar)
   load_pair<T, N, R>(a.g_traj + size_t(a.steps - 1) * N, gstride, b0, b1, j, Vb);
+
+  // Stage states and loss cotangent of sub-step `idx` (reverse order),
+  // prefetched into registers one sub-step ahead so the HBM latency hides
+  // behind the current sub-step's transforms.
+  cpx<T> cur[4][R], nxt[4][R], lcur[R], lnxt[R];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int m = 0; m < R; ++m) cur[k][m] = nxt[k][m] = cpx<T>{T(0), T(0)};
+#pragma unroll
+  for (int m = 0; m < R; ++m) lcur[m] = lnxt[m] = cpx<T>{T(0), T(0)};
+  auto fetch = [&](int idx, cpx<T> (*dst)[R], cpx<T>* ldst) {
+    const int step = idx / a.substeps, sub = idx - step * a.substeps;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < ns) load_pair<T, N, R>(a.stages + (size_t(step) * nst + sub * ns + k) * N, sstride, b0, b1, j, dst[k]);
+    if (sub == 0 && step > 0) load_pair<T, N, R>(a.g_traj + size_t(step - 1) * N, gstride, b0, b1, j, ldst);
+  };
+  const int total = a.steps * a.substeps;
+  fetch(total - 1, cur, lcur);
   fft<T, N, R, -1>(Vb, tw, lds, j);
 
   // z-bar = 2 z * Re F^H(conj(G~) X) (+ loss cotangent), then F(z-bar)
-  auto back_nl = [&](const cpx<T>* X, int step, int st, bool with_loss, cpx<T>* out) {
+  auto back_nl = [&](const cpx<T>* X, const cpx<T>* zs, const cpx<T>* loss, cpx<T>* out) {
     adj_g_ifft<T, N, R>(X, y, co, tw, lds, j);
-    load_pair<T, N, R>(a.stages + (size_t(step) * nst + st) * N, sstride, b0, b1, j, zs);
 #pragma unroll
     for (int m = 0; m < R; ++m) out[m] = cpx<T>{T(2) * zs[m].x * y[m].x, T(2) * zs[m].y * y[m].y};
-    if (with_loss) {
-      load_pair<T, N, R>(a.g_traj + size_t(step - 1) * N, gstride, b0, b1, j, zs);
+    if (loss) {
 #pragma unroll
-      for (int m = 0; m < R; ++m) out[m] = out[m] + zs[m];
+      for (int m = 0; m < R; ++m) out[m] = out[m] + loss[m];
     }
     fft<T, N, R, -1>(out, tw, lds, j);
   };
 
-  for (int step = a.steps - 1; step >= 0; --step) {
-    for (int sub = a.substeps - 1; sub >= 0; --sub) {
-      const bool loss = (sub == 0 && step > 0);
-      const int st0 = sub * ns;
-      cpx<T> t[R];
-      if (a.scheme == INC_ETD1) {
-        cpx<T> Nb[R];
+  for (int idx = total - 1; idx >= 0; --idx) {
+    const int step = idx / a.substeps, sub = idx - step * a.substeps;
+    if (idx > 0) fetch(idx - 1, nxt, lnxt);
+    const cpx<T>* loss = (sub == 0 && step > 0) ? lcur : nullptr;
+    cpx<T> t[R];
+    if (a.scheme == INC_ETD1) {
+      cpx<T> Nb[R];
 #pragma unroll
-        for (int m = 0; m < R; ++m) {
-          Nb[m] = co.c[1][m] * Vb[m];
-          Sb[m] = Sb[m] + Nb[m];
-          Vb[m] = co.c[0][m] * Vb[m];
-        }
-        back_nl(Nb, step, st0, loss, t);
-#pragma unroll
-        for (int m = 0; m < R; ++m) Vb[m] = Vb[m] + t[m];
-      } else if (a.scheme == INC_ETRK2) {
-        cpx<T> Mb[R], N1b[R], N2b[R];
-#pragma unroll
-        for (int m = 0; m < R; ++m) {
-          Mb[m] = Vb[m];
-          N1b[m] = (-co.c[2][m]) * Vb[m];
-          N2b[m] = co.c[2][m] * Vb[m];
-        }
-        back_nl(N2b, step, st0 + 1, false, t);
-#pragma unroll
-        for (int m = 0; m < R; ++m) {
-          Mb[m] = Mb[m] + t[m];
-          Vb[m] = co.c[0][m] * Mb[m];
-          N1b[m] = N1b[m] + co.c[1][m] * Mb[m];
-          Sb[m] = Sb[m] + co.c[1][m] * Mb[m];
-        }
-        back_nl(N1b, step, st0, loss, t);
-#pragma unroll
-        for (int m = 0; m < R; ++m) Vb[m] = Vb[m] + t[m];
-      } else {
-        cpx<T> Nvb[R], Nab[R], Nbb[R], Ab[R];
-#pragma unroll
-        for (int m = 0; m < R; ++m) {
-          cpx<T> vp = Vb[m];
-          Vb[m] = co.c[0][m] * vp;
-          Nvb[m] = co.c[3][m] * vp;
-          Nab[m] = (T(2) * co.c[4][m]) * vp;
-          Nbb[m] = Nab[m];
-          t[m] = co.c[5][m] * vp;  // Nc bar
-          Sb[m] = Sb[m] + t[m];
-        }
-        cpx<T> ncb[R];
-#pragma unroll
-        for (int m = 0; m < R; ++m) ncb[m] = t[m];
-        back_nl(ncb, step, st0 + 3, false, t);  // c bar
-#pragma unroll
-        for (int m = 0; m < R; ++m) {
-          Ab[m] = co.c[1][m] * t[m];
-          Nbb[m] = Nbb[m] + (T(2) * co.c[2][m]) * t[m];
-          Nvb[m] = Nvb[m] - co.c[2][m] * t[m];
-        }
-        back_nl(Nbb, step, st0 + 2, false, t);  // b bar
-#pragma unroll
-        for (int m = 0; m < R; ++m) {
-          Vb[m] = Vb[m] + co.c[1][m] * t[m];
-          Nab[m] = Nab[m] + co.c[2][m] * t[m];
-        }
-        back_nl(Nab, step, st0 + 1, false, t);  // a bar (part)
-#pragma unroll
-        for (int m = 0; m < R; ++m) {
-          Ab[m] = Ab[m] + t[m];
-          Vb[m] = Vb[m] + co.c[1][m] * Ab[m];
-          Nvb[m] = Nvb[m] + co.c[2][m] * Ab[m];
-          Sb[m] = Sb[m] + Nab[m] + Nbb[m];
-        }
-        back_nl(Nvb, step, st0, loss, t);
-#pragma unroll
-        for (int m = 0; m < R; ++m) {
-          Vb[m] = Vb[m] + t[m];
-          Sb[m] = Sb[m] + Nvb[m];
-        }
+      for (int m = 0; m < R; ++m) {
+        Nb[m] = co.c[1][m] * Vb[m];
+        Sb[m] = Sb[m] + Nb[m];
+        Vb[m] = co.c[0][m] * Vb[m];
       }
+      back_nl(Nb, cur[0], loss, t);
+#pragma unroll
+      for (int m = 0; m < R; ++m) Vb[m] = Vb[m] + t[m];
+    } else if (a.scheme == INC_ETRK2) {
+      cpx<T> Mb[R], N1b[R], N2b[R];
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        Mb[m] = Vb[m];
+        N1b[m] = (-co.c[2][m]) * Vb[m];
+        N2b[m] = co.c[2][m] * Vb[m];
+      }
+      back_nl(N2b, cur[1], nullptr, t);
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        Mb[m] = Mb[m] + t[m];
+        Vb[m] = co.c[0][m] * Mb[m];
+        N1b[m] = N1b[m] + co.c[1][m] * Mb[m];
+        Sb[m] = Sb[m] + co.c[1][m] * Mb[m];
+      }
+      back_nl(N1b, cur[0], loss, t);
+#pragma unroll
+      for (int m = 0; m < R; ++m) Vb[m] = Vb[m] + t[m];
+    } else {
+      cpx<T> Nvb[R], Nab[R], Nbb[R], Ab[R], Ncb[R];
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        cpx<T> vp = Vb[m];
+        Vb[m] = co.c[0][m] * vp;
+        Nvb[m] = co.c[3][m] * vp;
+        Nab[m] = (T(2) * co.c[4][m]) * vp;
+        Nbb[m] = Nab[m];
+        Ncb[m] = co.c[5][m] * vp;
+        Sb[m] = Sb[m] + Ncb[m];
+      }
+      back_nl(Ncb, cur[3], nullptr, t);  // c bar
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        Ab[m] = co.c[1][m] * t[m];
+        Nbb[m] = Nbb[m] + (T(2) * co.c[2][m]) * t[m];
+        Nvb[m] = Nvb[m] - co.c[2][m] * t[m];
+      }
+      back_nl(Nbb, cur[2], nullptr, t);  // b bar
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        Vb[m] = Vb[m] + co.c[1][m] * t[m];
+        Nab[m] = Nab[m] + co.c[2][m] * t[m];
+      }
+      back_nl(Nab, cur[1], nullptr, t);  // a bar (second part)
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        Ab[m] = Ab[m] + t[m];
+        Vb[m] = Vb[m] + co.c[1][m] * Ab[m];
+        Nvb[m] = Nvb[m] + co.c[2][m] * Ab[m];
+        Sb[m] = Sb[m] + Nab[m] + Nbb[m];
+      }
+      back_nl(Nvb, cur[0], loss, t);
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        Vb[m] = Vb[m] + t[m];
+        Sb[m] = Sb[m] + Nvb[m];
+      }
+    }
+    if (idx > 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int m = 0; m < R; ++m) cur[k][m] = nxt[k][m];
+#pragma unroll
+      for (int m = 0; m < R; ++m) lcur[m] = lnxt[m];
     }
   }
   // u^0 = F^H-side of V^0 = F(u^0)/N:  u-bar = Re F^H(V-bar) / N
@@ -683,7 +707,7 @@ __global__ void ks_table_kernel(int scheme, int n, double dt, int dealias, const
       c[4] = dt * f2 / M;
       c[5] = dt * f3 / M;
     }
-    double mask = (dealias && 3 * f >= n) ? 0.0 : 1.0;
+    double mask = (dealias && scheme == INC_ETDRK4 && 3 * f >= n) ? 0.0 : 1.0;
     double g = (2 * f == n) ? 0.0 : 0.5 * kw * mask;
     T* row = out + size_t(gid) * INC_KS_TABLE_STRIDE;
     for (int k = 0; k < 6; ++k) row[k] = T(c[k]);

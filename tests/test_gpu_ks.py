@@ -222,11 +222,14 @@ def test_ks_inc_rollout_checkpointing_matches_full_graph(cuda):
         sv.doublestep = False
         cnn.zero_grad()
         u = u0.clone().requires_grad_(True)
-        res, corr = simulate_rollout(cnn, sv, u, H, "INC", L=torch.tensor([L], device=cuda), checkpoint_every=ck)
+        def model(v, Lv):
+            return 0.05 * cnn(v, Lv)  # keep the random network's correction small
+        res, corr = simulate_rollout(model, sv, u, H, "INC", L=torch.tensor([L], device=cuda), checkpoint_every=ck)
         assert corr.shape == (b, H, n)
         ((res - tgt) ** 2).mean().backward()
         grads.append((res.detach(), u.grad.clone(), [p.grad.clone() for p in cnn.parameters()]))
     (r0, g0, p0), (r1, g1, p1) = grads
+    assert torch.isfinite(r0).all()
     assert rel_l2(r1, r0) < 1e-6
     assert rel_l2(g1, g0) < 1e-5
     for a, c in zip(p1, p0):
