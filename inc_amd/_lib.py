@@ -50,6 +50,16 @@ class KSDesc(ctypes.Structure):
     ]
 
 
+class RaggedDesc(ctypes.Structure):
+    """inc_ragged_desc (include/inc_amd.h)."""
+    _fields_ = [
+        ("dtype", ctypes.c_int32), ("scheme", ctypes.c_int32), ("convection", ctypes.c_int32),
+        ("count", ctypes.c_int32), ("total", ctypes.c_int64),
+        ("nmin", ctypes.c_int32), ("nmax", ctypes.c_int32), ("team", ctypes.c_int32), ("step0", ctypes.c_int32),
+        ("n", _vp), ("offset", _vp), ("order", _vp), ("horizon", _vp), ("nu", _vp), ("dt", _vp), ("dx", _vp),
+    ]
+
+
 #: every symbol include/inc_amd.h declares, with its ctypes signature
 SIGNATURES = {
     "inc_version": (ctypes.c_int, []),
@@ -60,6 +70,9 @@ SIGNATURES = {
     "inc_burgers_rhs_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp]),
     "inc_burgers_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "inc_burgers_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "inc_burgers_ragged_fwd": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp,
+                                              _vp]),
+    "inc_burgers_ragged_adj": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "inc_ks_tables": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                      _vp, ctypes.c_int, _vp, _vp]),
     "inc_ks_fwd": (ctypes.c_int, [ctypes.POINTER(KSDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),

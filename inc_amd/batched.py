@@ -1,0 +1,185 @@
+"""Uneven batches: many Burgers trajectories of different resolution,
+viscosity, time step and horizon advanced by ONE kernel launch per step.
+
+The reference trains on batches drawn from one dataset resolution and steps
+them with one ``BurgersSolverTorch`` per batch (trainer/trainer_1d.py:338-358,
+solvers/solver_1d.py:181-195).  ``RaggedBurgers`` runs a heterogeneous set in a
+single launch: trajectories are packed end to end, grouped by resolution (so
+a correction network can run on each group as a contiguous [count, n] view)
+and issued longest-first (n * horizon), so the in-order block dispatcher acts
+as the work queue.  A trajectory whose horizon is reached keeps its state for
+the remaining steps of the rollout.
+"""
+import ctypes
+import math
+
+import torch
+
+from . import _lib
+from . import autotune
+
+
+RaggedDesc = _lib.RaggedDesc
+
+
+def stable_dt(n, nu, L=2 * math.pi, cfl=0.4, umax=1.0, viscous=0.628):
+    """min(CFL * dx / max|u|, viscous * dx^2 / nu): SSP-RK3's advective limit
+    and its real-axis limit 2.5127 over the Laplacian's spectral radius
+    4 nu / dx^2 (0.628 = 2.5127 / 4)."""
+    dx = L / n
+    return min(cfl * dx / umax, viscous * dx * dx / nu)
+
+
+class _RaggedAdvance(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u, s, rb, steps, step0, want_traj, need_stages):
+        u = u.contiguous()
+        s = s.contiguous() if s is not None else None
+        d = rb._desc(step0)
+        out = torch.empty_like(u)
+        traj = u.new_empty(steps, rb.total) if want_traj else None
+        ns = 3 if rb.scheme == _lib.INC_SSPRK3 else 1
+        stages = u.new_empty(steps, ns, rb.total) if need_stages else None
+        rc = _lib.load().inc_burgers_ragged_fwd(ctypes.byref(d), _lib.ptr(u), _lib.ptr(s), _lib.ptr(out), steps,
+                                                _lib.ptr(traj), _lib.ptr(stages), _lib.stream())
+        _lib.check(rc, "RaggedBurgers.step")
+        ctx.save_for_backward(stages)
+        ctx.rb, ctx.steps, ctx.step0, ctx.want_traj = rb, steps, step0, want_traj
+        return traj if want_traj else out
+
+    @staticmethod
+    def backward(ctx, g):
+        (stages,) = ctx.saved_tensors
+        if stages is None:
+            raise RuntimeError("ragged step was run without saved stages; enable grad before the step")
+        rb = ctx.rb
+        if ctx.want_traj:
+            g_traj = g.contiguous()
+        elif ctx.steps == 1:
+            g_traj = g.contiguous().view(1, rb.total)
+        else:
+            g_traj = g.new_zeros(ctx.steps, rb.total)
+            g_traj[-1] = g
+        d = rb._desc(ctx.step0)
+        g_u = torch.empty(rb.total, dtype=g.dtype, device=g.device)
+        g_s = torch.empty_like(g_u) if ctx.needs_input_grad[1] else None
+        rc = _lib.load().inc_burgers_ragged_adj(ctypes.byref(d), _lib.ptr(stages), _lib.ptr(g_traj), _lib.ptr(g_u),
+                                                _lib.ptr(g_s), ctx.steps, _lib.stream())
+        _lib.check(rc, "RaggedBurgers.step backward")
+        return g_u, g_s, None, None, None, None, None
+
+
+class RaggedBurgers:
+    """A heterogeneous batch of periodic Burgers problems on [0, L).
+
+    ``n``, ``nu``, ``dt``, ``horizon``: one entry per trajectory (user order).
+    Packed tensors are [total] in the internal order (grouped by n);
+    ``pack``/``unpack`` convert from/to per-trajectory lists.
+    """
+
+    def __init__(self, n, nu, dt, horizon=None, L=2 * math.pi, scheme="SSPRK3", convection="WENO",
+                 device="cuda", dtype=torch.float32, team=0):
+        n = [int(v) for v in n]
+        self.count = len(n)
+        if not (len(nu) == len(dt) == self.count) or (horizon is not None and len(horizon) != self.count):
+            raise ValueError("n, nu, dt (and horizon) need one entry per trajectory")
+        if self.count and min(n) < 8:
+            raise ValueError("every trajectory needs at least 8 points")
+        self.scheme = {"Euler": _lib.INC_EULER, "SSPRK3": _lib.INC_SSPRK3}[scheme]
+        self.convection = {"WENO": _lib.INC_WENO, "FDM": _lib.INC_FDM}[convection]
+        self.dtype, self.device, self.L = dtype, torch.device(device), float(L)
+        self._team_auto = team == "auto" or (team == 0 and autotune.enabled())
+        self.team = 0 if team == "auto" else int(team)
+        # internal order: grouped by n (stable), so each resolution is contiguous
+        self.perm = sorted(range(self.count), key=lambda b: n[b])
+        self.n = [n[b] for b in self.perm]
+        self.horizon = None if horizon is None else [int(horizon[b]) for b in self.perm]
+        offs, o = [], 0
+        for v in self.n:
+            offs.append(o)
+            o += v
+        self.offsets, self.total = offs, o
+        # resolution groups: (n, first trajectory, count, first point)
+        self.groups = []
+        for i, v in enumerate(self.n):
+            if self.groups and self.groups[-1][0] == v:
+                g = self.groups[-1]
+                self.groups[-1] = (v, g[1], g[2] + 1, g[3])
+            else:
+                self.groups.append((v, i, 1, offs[i]))
+        cost = [self.n[i] * (self.horizon[i] if self.horizon else 1) for i in range(self.count)]
+        order = sorted(range(self.count), key=lambda i: -cost[i])
+        dev = self.device
+        self.n_d = torch.tensor(self.n, dtype=torch.int32, device=dev)
+        self.off_d = torch.tensor(offs, dtype=torch.int64, device=dev)
+        self.order_d = torch.tensor(order, dtype=torch.int32, device=dev)
+        self.horizon_d = (torch.tensor(self.horizon, dtype=torch.int32, device=dev) if self.horizon else None)
+        self.nu_d = torch.tensor([float(nu[b]) for b in self.perm], dtype=dtype, device=dev)
+        self.dt_d = torch.tensor([float(dt[b]) for b in self.perm], dtype=dtype, device=dev)
+        self.dx_d = torch.tensor([self.L / v for v in self.n], dtype=dtype, device=dev)
+        self.nmin = min(self.n) if self.n else 8
+        self.nmax = max(self.n) if self.n else 8
+
+    def _desc(self, step0=0):
+        d = RaggedDesc()
+        d.dtype = _lib.dtype_code(self.dtype)
+        d.scheme, d.convection = self.scheme, self.convection
+        d.count, d.total = self.count, self.total
+        d.nmin, d.nmax, d.team, d.step0 = self.nmin, self.nmax, self.team, int(step0)
+        d.n, d.offset, d.order = self.n_d.data_ptr(), self.off_d.data_ptr(), self.order_d.data_ptr()
+        d.horizon = self.horizon_d.data_ptr() if self.horizon_d is not None else None
+        d.nu, d.dt, d.dx = self.nu_d.data_ptr(), self.dt_d.data_ptr(), self.dx_d.data_ptr()
+        return d
+
+    # -- packing --------------------------------------------------------------------
+    def pack(self, fields):
+        """List of per-trajectory 1D tensors (user order) -> packed [total]."""
+        if len(fields) != self.count:
+            raise ValueError("need one field per trajectory")
+        return torch.cat([fields[b].reshape(-1).to(self.device, self.dtype) for b in self.perm])
+
+    def unpack(self, packed):
+        """Packed [..., total] -> list of [..., n_b] views in user order."""
+        out = [None] * self.count
+        for i, b in enumerate(self.perm):
+            out[b] = packed[..., self.offsets[i]:self.offsets[i] + self.n[i]]
+        return out
+
+    def group_views(self, packed):
+        """[count_k, n_k] views of each resolution group (internal order)."""
+        return [packed[o:o + c * v].view(c, v) for (v, _, c, o) in self.groups]
+
+    def valid_mask(self, steps):
+        """[steps, total] 1.0 where the trajectory is still inside its horizon."""
+        m = torch.ones(steps, self.total, device=self.device, dtype=self.dtype)
+        if self.horizon:
+            for i, h in enumerate(self.horizon):
+                if h < steps:
+                    m[h:, self.offsets[i]:self.offsets[i] + self.n[i]] = 0
+        return m
+
+    # -- stepping -------------------------------------------------------------------
+    def advance(self, u, s=None, steps=1, step0=0, want_traj=False):
+        """``steps`` steps from rollout step ``step0`` with a fixed correction."""
+        _lib.require_gpu(u, s)
+        if self._team_auto:
+            self._team_auto = False
+            self.team = autotune.tune_ragged(self)
+        if u.numel() != self.total:
+            raise ValueError(f"packed state has {u.numel()} values, batch has {self.total}")
+        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
+        return _RaggedAdvance.apply(u, s, self, steps, step0, want_traj, need)
+
+    def rollout(self, u0, steps, model=None):
+        """[steps, total] trajectory.  With ``model`` (a correction network
+        taking [count, n] -> [count, n]) the correction is re-evaluated on
+        every resolution group before every step (INC); without it the whole
+        rollout is one launch."""
+        if model is None:
+            return self.advance(u0, None, steps=steps, step0=0, want_traj=True)
+        outs, u = [], u0
+        for k in range(steps):
+            s = torch.cat([model(v).reshape(-1) for v in self.group_views(u)])
+            u = self.advance(u, s, steps=1, step0=k)
+            outs.append(u)
+        return torch.stack(outs, 0)

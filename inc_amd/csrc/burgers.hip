@@ -673,6 +673,315 @@ __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
   }
 }
 
+// ---- ragged batches ------------------------------------------------------------
+// Trajectories of different length n_b, viscosity, dt and horizon packed end to
+// end (trajectory b occupies [off_b, off_b + n_b) of every packed array).  One
+// block of `team` threads owns one trajectory; the per-thread point count is
+// chosen per trajectory (uniform within the block).  Blocks are issued in the
+// order of `order` (longest first), so the hardware dispatcher acts as the work
+// queue.  Packed per-step buffers (trajectory, stages) have stride `total`.
+template <typename T>
+struct RArgs {
+  int count, scheme, steps, step0, team;
+  int64_t total;
+  const int* n;
+  const int64_t* off;
+  const int* order;
+  const int* horizon;
+  const T* nu;
+  const T* dt;
+  const T* dx;
+  const T* u;
+  const T* s;
+  T* out;
+  T* traj;
+  T* stages;
+  const T* g_traj;
+  T* g_s;
+};
+
+template <typename T>
+__device__ __forceinline__ BArgs<T> ragged_bargs(const RArgs<T>& r, int b, int n) {
+  BArgs<T> a{};
+  a.batch = 1;
+  a.n = n;
+  a.team = r.team;
+  a.scheme = r.scheme;
+  a.source = INC_SRC_NONE;
+  a.parts = INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE;
+  a.dt = r.dt[b];
+  T dx = r.dx[b];
+  a.inv_dx = T(1) / dx;
+  a.inv_dx2 = T(1) / (dx * dx);
+  a.half_inv_dx = T(0.5) / dx;
+  a.nu = r.nu[b];
+  return a;
+}
+
+template <typename T>
+__device__ __forceinline__ TeamCtx<T> ragged_ctx(int n, int ppt) {
+  TeamCtx<T> c;
+  c.team = blockDim.x;
+  c.lt = threadIdx.x;
+  c.b = 0;
+  c.active = true;
+  c.wave_in_block = threadIdx.x / kWave;
+  c.first_wave = 0;
+  c.i0 = c.lt * ppt;
+  c.n = n;
+  return c;
+}
+
+template <typename T, int PPT, int CONV>
+__device__ void ragged_fwd_body(const RArgs<T>& r, int b, T* us, T* Fs, T* red_v, int* red_i) {
+  const int n = r.n[b];
+  const int64_t off = r.off[b];
+  const int hz = r.horizon ? r.horizon[b] : (r.step0 + r.steps);
+  BArgs<T> a = ragged_bargs(r, b, n);
+  TeamCtx<T> c = ragged_ctx<T>(n, PPT);
+  const bool own = c.i0 < n;
+  const int ns = r.scheme == INC_SSPRK3 ? 3 : 1;
+  T u0[PPT], uc[PPT], sv[PPT], xv[PPT], R[PPT];
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    int i = c.i0 + p;
+    bool ok = own && i < n;
+    u0[p] = ok ? r.u[off + i] : T(0);
+    sv[p] = (ok && r.s) ? r.s[off + i] : T(0);
+    xv[p] = T(0);
+  }
+  for (int step = 0; step < r.steps; ++step) {
+    if (r.step0 + step < hz) {
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) uc[p] = u0[p];
+      for (int st = 0; st < ns; ++st) {
+        if (r.stages && own) {
+          T* dst = r.stages + (int64_t(step) * ns + st) * r.total + off;
+#pragma unroll
+          for (int p = 0; p < PPT; ++p)
+            if (c.i0 + p < n) dst[c.i0 + p] = uc[p];
+        }
+        rhs_eval<T, PPT, CONV>(a, c, uc, r.s ? sv : nullptr, xv, T(0), a.nu, a.parts, us, Fs, red_v, red_i, R);
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+          T v = uc[p] + a.dt * R[p];
+          if (ns == 1 || st == 0) uc[p] = v;
+          else if (st == 1) uc[p] = T(0.75) * u0[p] + T(0.25) * v;
+          else uc[p] = T(1.0 / 3.0) * u0[p] + T(2.0 / 3.0) * v;
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) u0[p] = uc[p];
+    }
+    if (r.traj && own) {
+      T* dst = r.traj + int64_t(step) * r.total + off;
+#pragma unroll
+      for (int p = 0; p < PPT; ++p)
+        if (c.i0 + p < n) dst[c.i0 + p] = u0[p];
+    }
+  }
+  if (own) {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p)
+      if (c.i0 + p < n) r.out[off + c.i0 + p] = u0[p];
+  }
+}
+
+template <typename T, int PPT, int CONV>
+__device__ void ragged_adj_body(const RArgs<T>& r, int b, T* us, T* cs, T* C, T* red_v, int* red_i) {
+  const int n = r.n[b];
+  const int64_t off = r.off[b];
+  const int hz = r.horizon ? r.horizon[b] : (r.step0 + r.steps);
+  BArgs<T> a = ragged_bargs(r, b, n);
+  TeamCtx<T> c = ragged_ctx<T>(n, PPT);
+  const bool own = c.i0 < n;
+  const int ns = r.scheme == INC_SSPRK3 ? 3 : 1;
+  const T dt = a.dt, nu = a.nu;
+  T lam[PPT], gs[PPT], st[3][PPT], cb[PPT], rr[PPT], b1[PPT], b2[PPT];
+  auto gload = [&](int step, T* dst) {
+    const T* src = r.g_traj + int64_t(step) * r.total + off;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      int i = c.i0 + p;
+      dst[p] = (own && i < n) ? src[i] : T(0);
+    }
+  };
+  gload(r.steps - 1, lam);
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) gs[p] = T(0);
+  for (int step = r.steps - 1; step >= 0; --step) {
+    if (r.step0 + step < hz) {
+      for (int k = 0; k < ns; ++k) {
+        const T* src = r.stages + (int64_t(step) * ns + k) * r.total + off;
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+          int i = c.i0 + p;
+          st[k][p] = (own && i < n) ? src[i] : T(0);
+        }
+      }
+      if (ns == 1) {
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) cb[p] = dt * lam[p];
+        rhs_vjp<T, PPT, CONV>(a, c, st[0], cb, nu, a.parts, us, cs, C, red_v, red_i, rr);
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) { gs[p] += cb[p]; lam[p] += rr[p]; }
+      } else {
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) cb[p] = T(2.0 / 3.0) * dt * lam[p];
+        rhs_vjp<T, PPT, CONV>(a, c, st[2], cb, nu, a.parts, us, cs, C, red_v, red_i, rr);
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+          gs[p] += cb[p];
+          b2[p] = T(2.0 / 3.0) * lam[p] + rr[p];
+          lam[p] = T(1.0 / 3.0) * lam[p];
+        }
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) cb[p] = T(0.25) * dt * b2[p];
+        rhs_vjp<T, PPT, CONV>(a, c, st[1], cb, nu, a.parts, us, cs, C, red_v, red_i, rr);
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+          gs[p] += cb[p];
+          b1[p] = T(0.25) * b2[p] + rr[p];
+          lam[p] += T(0.75) * b2[p];
+        }
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) cb[p] = dt * b1[p];
+        rhs_vjp<T, PPT, CONV>(a, c, st[0], cb, nu, a.parts, us, cs, C, red_v, red_i, rr);
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+          gs[p] += cb[p];
+          lam[p] += b1[p] + rr[p];
+        }
+      }
+    }
+    if (step > 0) {
+      gload(step - 1, cb);
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) lam[p] += cb[p];
+    }
+  }
+  if (own) {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      int i = c.i0 + p;
+      if (i < n) {
+        r.out[off + i] = lam[p];
+        if (r.g_s) r.g_s[off + i] = gs[p];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int ragged_ppt(int n, int team) {
+  int need = (n + team - 1) / team;
+  int ppt = 1;
+  while (ppt < need) ppt <<= 1;
+  return ppt;
+}
+
+template <typename T, int CONV>
+__global__ void __launch_bounds__(kBlock) burgers_ragged_fwd_kernel(RArgs<T> r) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ T red_v[2 * kMaxWaves];
+  __shared__ int red_i[kMaxWaves];
+  const int b = r.order ? r.order[blockIdx.x] : int(blockIdx.x);
+  const int n = r.n[b];
+  T* us = reinterpret_cast<T*>(smem_raw);
+  T* Fs = us + n;
+  switch (ragged_ppt(n, blockDim.x)) {
+    case 1: ragged_fwd_body<T, 1, CONV>(r, b, us, Fs, red_v, red_i); break;
+    case 2: ragged_fwd_body<T, 2, CONV>(r, b, us, Fs, red_v, red_i); break;
+    case 4: ragged_fwd_body<T, 4, CONV>(r, b, us, Fs, red_v, red_i); break;
+    case 8: ragged_fwd_body<T, 8, CONV>(r, b, us, Fs, red_v, red_i); break;
+    default: ragged_fwd_body<T, 16, CONV>(r, b, us, Fs, red_v, red_i); break;
+  }
+}
+
+template <typename T, int CONV>
+__global__ void __launch_bounds__(kBlock) burgers_ragged_adj_kernel(RArgs<T> r) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ T red_v[2 * kMaxWaves];
+  __shared__ int red_i[kMaxWaves];
+  const int b = r.order ? r.order[blockIdx.x] : int(blockIdx.x);
+  const int n = r.n[b];
+  T* us = reinterpret_cast<T*>(smem_raw);
+  T* cs = us + n;
+  T* C = cs + n;
+  switch (ragged_ppt(n, blockDim.x)) {
+    case 1: ragged_adj_body<T, 1, CONV>(r, b, us, cs, C, red_v, red_i); break;
+    case 2: ragged_adj_body<T, 2, CONV>(r, b, us, cs, C, red_v, red_i); break;
+    case 4: ragged_adj_body<T, 4, CONV>(r, b, us, cs, C, red_v, red_i); break;
+    case 8: ragged_adj_body<T, 8, CONV>(r, b, us, cs, C, red_v, red_i); break;
+    default: ragged_adj_body<T, 16, CONV>(r, b, us, cs, C, red_v, red_i); break;
+  }
+}
+
+template <typename T>
+int ragged_launch(const inc_ragged_desc* d, const void* u, const void* s, void* out, int steps, void* traj,
+                  void* stages, const void* g_traj, void* g_s, bool adj, void* stream) {
+  INC_REQUIRE(d->scheme == INC_EULER || d->scheme == INC_SSPRK3, INC_EINVAL, "unknown time scheme %d", d->scheme);
+  INC_REQUIRE(d->convection == INC_WENO || d->convection == INC_FDM, INC_EINVAL, "Convection scheme not recognized");
+  INC_REQUIRE(d->count >= 0 && d->total >= 0, INC_EINVAL, "negative sizes");
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(d->nmin >= 8, INC_EINVAL, "every trajectory needs >= 8 points (smallest is %d)", d->nmin);
+  const int team = d->team > 0 ? d->team : 64;
+  INC_REQUIRE(team >= kWave && team <= kBlock && (team & (team - 1)) == 0, INC_EUNSUPPORTED,
+              "ragged team must be 64, 128 or 256 threads (got %d)", team);
+  INC_REQUIRE(d->nmax <= 16 * team, INC_EUNSUPPORTED, "n=%d exceeds 16 points per thread of a %d-thread team",
+              d->nmax, team);
+  INC_REQUIRE(d->n && d->offset && d->nu && d->dt && d->dx, INC_EINVAL, "ragged descriptor arrays missing");
+  INC_REQUIRE(!adj || stages, INC_EINVAL, "adjoint needs the saved stages");
+  if (d->count == 0) return INC_OK;
+  size_t lds = size_t(adj ? 8 : 2) * d->nmax * sizeof(T);
+  INC_REQUIRE(lds <= 160 * 1024 - 1024, INC_EUNSUPPORTED, "n=%d needs %zu bytes of LDS", d->nmax, lds);
+  RArgs<T> r{};
+  r.count = d->count;
+  r.scheme = d->scheme;
+  r.steps = steps;
+  r.step0 = d->step0;
+  r.team = team;
+  r.total = d->total;
+  r.n = d->n;
+  r.off = d->offset;
+  r.order = d->order;
+  r.horizon = d->horizon;
+  r.nu = static_cast<const T*>(d->nu);
+  r.dt = static_cast<const T*>(d->dt);
+  r.dx = static_cast<const T*>(d->dx);
+  r.u = static_cast<const T*>(u);
+  r.s = static_cast<const T*>(s);
+  r.out = static_cast<T*>(out);
+  r.traj = static_cast<T*>(traj);
+  r.stages = static_cast<T*>(stages);
+  r.g_traj = static_cast<const T*>(g_traj);
+  r.g_s = static_cast<T*>(g_s);
+  void* fn;
+  if (adj)
+    fn = d->convection == INC_WENO ? reinterpret_cast<void*>(&burgers_ragged_adj_kernel<T, INC_WENO>)
+                                   : reinterpret_cast<void*>(&burgers_ragged_adj_kernel<T, INC_FDM>);
+  else
+    fn = d->convection == INC_WENO ? reinterpret_cast<void*>(&burgers_ragged_fwd_kernel<T, INC_WENO>)
+                                   : reinterpret_cast<void*>(&burgers_ragged_fwd_kernel<T, INC_FDM>);
+  void* args[] = {&r};
+  INC_HIP(hipLaunchKernel(fn, dim3(d->count), dim3(team), args, lds, static_cast<hipStream_t>(stream)));
+  return INC_OK;
+}
+
+int ragged_dispatch(const inc_ragged_desc* d, const void* u, const void* s, void* out, int steps, void* traj,
+                    void* stages, const void* g_traj, void* g_s, bool adj, void* stream) {
+  clear_error();
+  INC_REQUIRE(d != nullptr, INC_EINVAL, "null descriptor");
+  if (d->dtype == INC_F32) return ragged_launch<float>(d, u, s, out, steps, traj, stages, g_traj, g_s, adj, stream);
+  if (d->dtype == INC_F64) return ragged_launch<double>(d, u, s, out, steps, traj, stages, g_traj, g_s, adj, stream);
+  set_error("unsupported dtype %d", d->dtype);
+  return INC_EINVAL;
+}
+
 // ---- host side ---------------------------------------------------------------
 int pick_ppt(int n, int team_req, int* team_out) {
   // Default: 4 points per thread and team = next power of two of n/4; above
@@ -880,6 +1189,17 @@ int inc_burgers_fwd(const inc_burgers_desc* d, const void* u, const void* s, voi
   }
   return inc::dispatch(Kind::Fwd, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE, u, s,
                        u_out, steps, traj, stages, nullptr, nullptr, stream);
+}
+
+int inc_burgers_ragged_fwd(const inc_ragged_desc* d, const void* u, const void* s, void* u_out, int steps,
+                           void* traj, void* stages, void* stream) {
+  return inc::ragged_dispatch(d, u, s, u_out, steps, traj, stages, nullptr, nullptr, false, stream);
+}
+
+int inc_burgers_ragged_adj(const inc_ragged_desc* d, const void* stages, const void* g_traj, void* g_u, void* g_s,
+                           int steps, void* stream) {
+  return inc::ragged_dispatch(d, nullptr, nullptr, g_u, steps, nullptr, const_cast<void*>(stages), g_traj, g_s,
+                              true, stream);
 }
 
 int inc_burgers_adj(const inc_burgers_desc* d, const void* stages, const void* g_traj, void* g_u,

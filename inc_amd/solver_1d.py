@@ -14,6 +14,7 @@ import math
 import torch
 
 from . import _lib as L
+from . import autotune
 
 __all__ = ["BurgersSolverTorch", "KSSolverTorch", "weno_reconstruction"]
 
@@ -157,7 +158,9 @@ class BurgersSolverTorch:
         self.time_scheme = getattr(sim_params, "time_scheme", "Euler")
         if self.time_scheme not in ("Euler", "SSPRK3"):
             raise ValueError("time_scheme must be 'Euler' or 'SSPRK3'")
-        self.team = int(getattr(sim_params, "team", 0) or 0)
+        team = getattr(sim_params, "team", 0) or 0
+        self._team_auto = team == "auto" or (team == 0 and autotune.enabled())
+        self.team = 0 if team == "auto" else int(team)
         if sim_params.convection_term == "FDM":
             self.calc_nonlinear = self.calc_nonlinear_FDM
             self._conv = L.INC_FDM
@@ -300,6 +303,10 @@ class BurgersSolverTorch:
 
     def _advance(self, u, dt, steps=1, want_traj=False):
         L.require_gpu(u, self.correction_s)
+        if self._team_auto:
+            self._team_auto = False
+            self.team = autotune.tune_burgers(u.shape[-1], u.shape[0], u.dtype, self.time_scheme,
+                                              self.param.convection_term)
         s = self.correction_s
         if s is not None:
             s = s.to(dtype=u.dtype)
@@ -418,7 +425,9 @@ class KSSolverTorch:
         self.correction_s = None
         self.correction_u = None
         self.dealias = bool(getattr(sim_params, "dealias", True))
-        self.radix = int(getattr(sim_params, "radix", 0) or 0)
+        radix = getattr(sim_params, "radix", 0) or 0
+        self._radix_auto = radix == "auto" or (radix == 0 and autotune.enabled())
+        self.radix = 0 if radix == "auto" else int(radix)
         self.fft_indices = torch.fft.fftfreq(self.resolution, d=1.0, device=self.device) * self.resolution
         time_scheme_map = {"ETRK2": self.etrk2, "ETD1": self.etd1, "ETDRK4": self.etdrk4}
         self.step = time_scheme_map[sim_params.time_scheme]
@@ -498,6 +507,9 @@ class KSSolverTorch:
 
     def _advance(self, u, domain_size, scheme, substeps, steps=1, want_traj=False):
         L.require_gpu(u, self.correction_s)
+        if self._radix_auto:
+            self._radix_auto = False
+            self.radix = autotune.tune_ks(u.shape[-1], u.shape[0], u.dtype, self.time_scheme, dt=self.dt)
         s = self.correction_s
         if s is not None:
             s = s.to(dtype=u.dtype)

@@ -122,6 +122,45 @@ int inc_burgers_adj(const inc_burgers_desc* d, const void* stages,
                     void* stream);
 
 /*
+ * A ragged batch of Burgers trajectories with different resolution,
+ * viscosity, time step and horizon, packed end to end: trajectory b occupies
+ * [offset[b], offset[b] + n[b]) of every packed [total] array.  Batched form
+ * of BurgersSolverTorch.step (solver_1d.py:181-195) over the uneven batches
+ * of the INC training set (one trajectory per sample, data_loader_1d.py:8-67);
+ * the source is the correction s only.
+ */
+typedef struct inc_ragged_desc {
+  int32_t dtype;      /* INC_F32 / INC_F64 */
+  int32_t scheme;     /* INC_EULER / INC_SSPRK3 */
+  int32_t convection; /* INC_WENO / INC_FDM */
+  int32_t count;      /* trajectories */
+  int64_t total;      /* sum of n[b] */
+  int32_t nmin, nmax; /* smallest / largest n[b] (n[b] >= 8; LDS sizing) */
+  int32_t team;       /* threads per trajectory: 64, 128 or 256 (0 = 64) */
+  int32_t step0;      /* rollout step index of this call's first step */
+  const int32_t* n;       /* [count] points per trajectory (device) */
+  const int64_t* offset;  /* [count] start in the packed arrays (device) */
+  const int32_t* order;   /* [count] issue order (longest first), or NULL */
+  const int32_t* horizon; /* [count] steps a trajectory takes; at rollout step
+                             k >= horizon[b] its state is carried unchanged;
+                             NULL = every step */
+  const void* nu;         /* [count] viscosity (dtype) */
+  const void* dt;         /* [count] time step (dtype) */
+  const void* dx;         /* [count] grid spacing (dtype) */
+} inc_ragged_desc;
+
+/*
+ * `steps` steps of every trajectory of a ragged batch in one launch, with the
+ * correction s ([total] or NULL) fixed over the steps.  u, u_out: [total];
+ * traj: [steps, total] or NULL; stages: [steps, ns, total] or NULL.
+ */
+int inc_burgers_ragged_fwd(const inc_ragged_desc* d, const void* u, const void* s, void* u_out, int steps,
+                           void* traj, void* stages, void* stream);
+/* Adjoint of inc_burgers_ragged_fwd: g_traj [steps, total] -> g_u [total], g_s [total] (or NULL). */
+int inc_burgers_ragged_adj(const inc_ragged_desc* d, const void* stages, const void* g_traj, void* g_u, void* g_s,
+                           int steps, void* stream);
+
+/*
  * Description of a KS problem (KSSolverTorch, solver_1d.py:198-212).
  * n must be a power of two, 16 <= n <= 1024.
  */

@@ -35,6 +35,8 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(L.BurgersDesc) == 144
     # int32 x8 + double + pointer x3 = 32 + 8 + 24
     assert ctypes.sizeof(L.KSDesc) == 64
+    # int32 x4 + int64 + int32 x4 + pointer x7 = 16 + 8 + 16 + 56
+    assert ctypes.sizeof(L.RaggedDesc) == 96
 
 
 def test_argument_errors_map_to_codes_without_launch():
@@ -58,6 +60,10 @@ def test_argument_errors_map_to_codes_without_launch():
     # adjoint without saved stages
     k = L.KSDesc(dtype=L.INC_F32, scheme=L.INC_ETD1, batch=1, n=64, substeps=1, table=1)
     assert lib.inc_ks_adj(ctypes.byref(k), None, None, None, None, 1, None) == L.INC_EINVAL
+    # ragged batch with a trajectory below 8 points
+    r = L.RaggedDesc(dtype=L.INC_F32, count=1, total=4, nmin=4, nmax=4)
+    rc = lib.inc_burgers_ragged_fwd(ctypes.byref(r), None, None, None, 1, None, None, None)
+    assert rc == L.INC_EINVAL and b"8 points" in lib.inc_last_error()
     # successful-path state is reset by the next call
     assert lib.inc_weno_fwd(L.INC_F32, 0, None, None, 0, 16, None) == L.INC_OK
     assert lib.inc_last_error() == b""
