@@ -82,6 +82,136 @@ def cpu_baseline(budget_s=12.0):
             "sample": f"FP64 oracle (torch CPU) ETDRK4, B={B}, N={N}, {h} steps fwd+adj, {dt:.1f} s"}
 
 
+def _timed(fn, reps=3):
+    """Median wall time (s) of fn() over reps runs after one warm-up."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def sine_state(n, seed, kmax=8):
+    """u0 = sum_{k<=8} a_k sin(k x + phi_k), a_k ~ N(0,1)/k, scaled to max|u0| = 1."""
+    import torch
+    gen = torch.Generator().manual_seed(seed)
+    x = torch.arange(n, dtype=torch.float64) * (2 * math.pi / n)
+    u = torch.zeros(n, dtype=torch.float64)
+    for k in range(1, kmax + 1):
+        u += torch.randn(1, generator=gen, dtype=torch.float64) / k * torch.sin(
+            k * x + 2 * math.pi * torch.rand(1, generator=gen, dtype=torch.float64))
+    return u / u.abs().max()
+
+
+def extra_burgers_inc():
+    """Burgers INC fwd+adj: B=512, N=128, nu=0.01, dt=0.01, 16 steps, SSP-RK3 + WENO5,
+    correction from a 4-layer 32-channel k=5 circular CNN, MSE to a nu=0.012 target."""
+    import torch
+    import inc_amd
+    from inc_amd.correction import CorrectionCNN
+    b, n, h = 512, 128, 16
+    dev = torch.device("cuda")
+    u0 = torch.stack([sine_state(n, 2000 + i) for i in range(b)]).float().to(dev)
+    cnn = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(2)).to(dev)
+    with torch.no_grad():
+        tgt_solver = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=0.01, resolution=n, nu=0.012, batch_size=b,
+                                                                    time_scheme="SSPRK3", device="cuda"))
+        target = tgt_solver.rollout(u0, h)
+
+    def train_step():
+        sv = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=0.01, resolution=n, nu=0.01, batch_size=b,
+                                                            time_scheme="SSPRK3", device="cuda"))
+        res, _ = inc_amd.simulate_rollout(cnn, sv, u0, h, "INC")
+        loss = ((res - target) ** 2).mean()
+        loss.backward()
+        cnn.zero_grad(set_to_none=True)
+    t = _timed(train_step)
+    return {"workload": "Burgers INC fwd+adj B=512 N=128 16 steps SSP-RK3/WENO5 + CNN(4x32,k5)",
+            "trajectory_steps_per_s": b * h / t, "ms_per_train_step": t * 1e3}
+
+
+def extra_ks_inc():
+    """KS INC fwd+adj with checkpointing: L=100, N=512, dt=0.25, B=1024, 64 steps, checkpoint every 8,
+    correction from a 6-layer 48-channel k=7 circular CNN."""
+    import torch
+    import inc_amd
+    from inc_amd.correction import CorrectionCNN
+    b, n, h, Lk = 1024, 512, 64, 100.0
+    dev = torch.device("cuda")
+    u0 = initial_state(b, n, Lk, 1003, kmax=16).float().to(dev)
+    cnn = CorrectionCNN(6, 48, 7, generator=torch.Generator().manual_seed(3)).to(dev)
+    Ld = torch.tensor([Lk], device=dev)
+    sv = inc_amd.KSSolverTorch(inc_amd.KSSimParams([Lk], resolution=n, dt=0.25, time_scheme="ETDRK4",
+                                                   device="cuda"))
+    sv.doublestep = False
+    target = torch.zeros(b, h, n, device=dev)
+
+    def train_step():
+        res, _ = inc_amd.simulate_rollout(lambda v, L_: 0.01 * cnn(v), sv, u0, h, "INC", L=Ld, checkpoint_every=8)
+        ((res - target) ** 2).mean().backward()
+        cnn.zero_grad(set_to_none=True)
+    t = _timed(train_step, reps=2)
+    return {"workload": "KS INC fwd+adj B=1024 N=512 L=100 64 steps ETDRK4, checkpoint every 8, CNN(6x48,k7)",
+            "trajectory_steps_per_s": b * h / t, "ms_per_train_step": t * 1e3}
+
+
+def extra_ragged_inc(plain_sample=16):
+    """Uneven Burgers batch: 2048 trajectories, N in {64..1024}, nu ~ LogU[0.003, 0.03], dt = min(CFL 0.4,
+    viscous), horizons ~ U{16..64}, 64-step INC rollout with the config-2 CNN, fwd+adj, autotuned block size;
+    against the plain per-trajectory path (one launch per trajectory per step) on a sample."""
+    import torch
+    import inc_amd
+    from inc_amd import autotune
+    from inc_amd.batched import stable_dt
+    from inc_amd.correction import CorrectionCNN
+    count, steps = 2048, 64
+    gen = torch.Generator().manual_seed(1004)
+    choices = [64, 128, 256, 512, 1024]
+    ns = [choices[int(i)] for i in torch.randint(0, 5, (count,), generator=gen)]
+    nus = torch.exp(torch.empty(count, dtype=torch.float64).uniform_(math.log(0.003), math.log(0.03),
+                                                                      generator=gen)).tolist()
+    dts = [stable_dt(n, nu) for n, nu in zip(ns, nus)]
+    hz = torch.randint(16, 65, (count,), generator=gen).tolist()
+    rb = inc_amd.RaggedBurgers(ns, nus, dts, horizon=hz)
+    tuning = {}
+    rb.team = autotune.tune_ragged(rb, record=tuning)
+    dev = torch.device("cuda")
+    u0 = rb.pack([sine_state(n, 3000 + i) for i, n in enumerate(ns)])
+    cnn = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(2)).to(dev)
+    mask = rb.valid_mask(steps)
+    with torch.no_grad():
+        target = inc_amd.RaggedBurgers(ns, [1.2 * v for v in nus], dts, horizon=hz, team=rb.team).rollout(u0, steps)
+
+    def train_step():
+        traj = rb.rollout(u0, steps, model=lambda v: 0.01 * cnn(v))
+        ((((traj - target) ** 2) * mask).sum() / mask.sum()).backward()
+        cnn.zero_grad(set_to_none=True)
+    t = _timed(train_step, reps=2)
+    traj_steps = sum(hz)
+    # plain: one BurgersSolverTorch launch per trajectory per step (same CNN), fwd+adj
+    idx = list(range(0, count, count // plain_sample))[:plain_sample]
+
+    def plain_step():
+        for i in idx:
+            sv = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=dts[i], resolution=ns[i], nu=nus[i],
+                                                                batch_size=1, time_scheme="SSPRK3", device="cuda"))
+            res, _ = inc_amd.simulate_rollout(lambda v: 0.01 * cnn(v), sv, sine_state(ns[i], 3000 + i)
+                                              .float().to(dev)[None], hz[i], "INC")
+            (res ** 2).mean().backward()
+        cnn.zero_grad(set_to_none=True)
+    tp = _timed(plain_step, reps=1)
+    plain_rate = sum(hz[i] for i in idx) / tp
+    return {"workload": "ragged Burgers INC fwd+adj: 2048 traj, N in {64..1024}, horizons 16..64, CNN(4x32,k5)",
+            "trajectory_steps_per_s": traj_steps / t, "ms_per_train_step": t * 1e3, "team": rb.team,
+            "autotune_ms": tuning, "plain_trajectory_steps_per_s": plain_rate,
+            "batched_over_plain": traj_steps / t / plain_rate}
+
+
 def plain_path(solver, u0, Ld, ntraj=8, h=50):
     """One launch per trajectory per step (no batching), fwd + adjoint."""
     import torch
@@ -105,6 +235,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-trajectory plain path")
+    ap.add_argument("--no-extras", action="store_true", help="skip the INC and ragged-batch workloads")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy first (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -179,6 +310,9 @@ def main():
         plain = plain_path(solver, u0, Ld)
         out["plain_trajectory_steps_per_s"] = plain
         out["batched_over_plain"] = value / plain
+    if not args.no_extras:
+        out["other_workloads"] = {"burgers_inc": extra_burgers_inc(), "ks_inc_checkpointed": extra_ks_inc(),
+                                  "ragged_burgers_inc": extra_ragged_inc()}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -86,8 +86,8 @@ def _time_ms(fn, reps=3):
 
 def ks_radix_candidates(n, dtype):
     if dtype == torch.float64:
-        return [max(4, n // 64)]
-    return [r for r in (4, 8, 16) if r <= n and n // r <= 64]
+        return [4]
+    return [r for r in (4, 8, 16) if r <= n and n // r <= 256]
 
 
 def tune_ks(n, batch, dtype=torch.float32, scheme="ETDRK4", L=64.0, dt=0.25, steps=16, record=None):

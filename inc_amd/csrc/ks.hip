@@ -188,6 +188,15 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A team of TL <= 64 lanes lives in one wavefront (LDS traffic is ordered
+// within a wavefront, so a compiler barrier suffices); a wider team is one
+// whole block of TL / 64 wavefronts and synchronises with s_barrier.
+template <int TL>
+__device__ __forceinline__ void team_sync() {
+  if constexpr (TL > 64) __syncthreads();
+  else wave_sync();
+}
+
 // Per-lane twiddles W_N^k (forward sign) of passes 1..NP-1.
 template <typename T, int N, int R>
 __device__ __forceinline__ void load_twiddles(const T* tab, int j, cpx<T>* tw) {
@@ -248,7 +257,7 @@ __device__ __forceinline__ void fft_pass(cpx<T>* v, const cpx<T>* tw, cpx<T>* ld
       for (int r = 0; r < P; ++r) wr[lds_slot(base + r * Ns)] = x[r];
     }
   }
-  if constexpr (!last) wave_sync();
+  if constexpr (!last) team_sync<TL>();
 }
 
 template <typename T, int N, int R, int SIGN, int S>
@@ -263,7 +272,7 @@ __device__ __forceinline__ void fft_passes(cpx<T>* v, const cpx<T>* tw, cpx<T>* 
 // team's N values, strided layout in and out.
 template <typename T, int N, int R, int SIGN>
 __device__ __forceinline__ void fft(cpx<T>* v, const cpx<T>* tw, cpx<T>* lds, int j) {
-  wave_sync();
+  team_sync<N / R>();
   fft_passes<T, N, R, SIGN, 0>(v, tw, lds, j);
 }
 
@@ -377,10 +386,10 @@ __device__ __forceinline__ int pair_rows(const KArgs<T>& a, int p, int& b0, int&
 }
 
 template <typename T, int N, int R>
-__global__ void __launch_bounds__(64) ks_fwd_kernel(KArgs<T> a) {
+__global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
   using PL = Plan<T, N, R>;
   constexpr int TL = PL::TL;
-  constexpr int TPB = 64 / TL;
+  constexpr int TPB = TL >= 64 ? 1 : 64 / TL;
   __shared__ cpx<T> lds_all[TPB * lds_team_elems<N>()];
   const int team = threadIdx.x / TL, j = threadIdx.x % TL;
   const int p = blockIdx.x * TPB + team;
@@ -489,10 +498,10 @@ __global__ void __launch_bounds__(64) ks_fwd_kernel(KArgs<T> a) {
 }
 
 template <typename T, int N, int R>
-__global__ void __launch_bounds__(64) ks_adj_kernel(KArgs<T> a) {
+__global__ void __launch_bounds__(256) ks_adj_kernel(KArgs<T> a) {
   using PL = Plan<T, N, R>;
   constexpr int TL = PL::TL;
-  constexpr int TPB = 64 / TL;
+  constexpr int TPB = TL >= 64 ? 1 : 64 / TL;
   __shared__ cpx<T> lds_all[TPB * lds_team_elems<N>()];
   const int team = threadIdx.x / TL, j = threadIdx.x % TL;
   const int p = blockIdx.x * TPB + team;
@@ -518,7 +527,10 @@ __global__ void __launch_bounds__(64) ks_adj_kernel(KArgs<T> a) {
 
   // Stage states and loss cotangent of sub-step `idx` (reverse order),
   // prefetched into registers one sub-step ahead so the HBM latency hides
-  // behind the current sub-step's transforms.
+  // behind the current sub-step's transforms (R <= 8; at R = 16 the double
+  // buffer would not fit the register file, so the sub-step's own states are
+  // loaded at its start and overlap its first inverse transform).
+  constexpr bool kPrefetch = R <= 8;
   cpx<T> cur[4][R], nxt[4][R], lcur[R], lnxt[R];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
@@ -538,21 +550,25 @@ __global__ void __launch_bounds__(64) ks_adj_kernel(KArgs<T> a) {
   fft<T, N, R, -1>(Vb, tw, lds, j);
 
   // z-bar = 2 z * Re F^H(conj(G~) X) (+ loss cotangent), then F(z-bar)
-  auto back_nl = [&](const cpx<T>* X, const cpx<T>* zs, const cpx<T>* loss, cpx<T>* out) {
+  auto back_nl = [&](const cpx<T>* X, const cpx<T>* zs, bool with_loss, cpx<T>* out) {
     adj_g_ifft<T, N, R>(X, y, co, tw, lds, j);
 #pragma unroll
     for (int m = 0; m < R; ++m) out[m] = cpx<T>{T(2) * zs[m].x * y[m].x, T(2) * zs[m].y * y[m].y};
-    if (loss) {
+    if (with_loss) {
 #pragma unroll
-      for (int m = 0; m < R; ++m) out[m] = out[m] + loss[m];
+      for (int m = 0; m < R; ++m) out[m] = out[m] + lcur[m];
     }
     fft<T, N, R, -1>(out, tw, lds, j);
   };
 
   for (int idx = total - 1; idx >= 0; --idx) {
     const int step = idx / a.substeps, sub = idx - step * a.substeps;
-    if (idx > 0) fetch(idx - 1, nxt, lnxt);
-    const cpx<T>* loss = (sub == 0 && step > 0) ? lcur : nullptr;
+    if constexpr (kPrefetch) {
+      if (idx > 0) fetch(idx - 1, nxt, lnxt);
+    } else {
+      if (idx < total - 1) fetch(idx, cur, lcur);
+    }
+    const bool loss = (sub == 0 && step > 0);
     cpx<T> t[R];
     if (a.scheme == INC_ETD1) {
       cpx<T> Nb[R];
@@ -573,7 +589,7 @@ __global__ void __launch_bounds__(64) ks_adj_kernel(KArgs<T> a) {
         N1b[m] = (-co.c[2][m]) * Vb[m];
         N2b[m] = co.c[2][m] * Vb[m];
       }
-      back_nl(N2b, cur[1], nullptr, t);
+      back_nl(N2b, cur[1], false, t);
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         Mb[m] = Mb[m] + t[m];
@@ -596,20 +612,20 @@ __global__ void __launch_bounds__(64) ks_adj_kernel(KArgs<T> a) {
         Ncb[m] = co.c[5][m] * vp;
         Sb[m] = Sb[m] + Ncb[m];
       }
-      back_nl(Ncb, cur[3], nullptr, t);  // c bar
+      back_nl(Ncb, cur[3], false, t);  // c bar
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         Ab[m] = co.c[1][m] * t[m];
         Nbb[m] = Nbb[m] + (T(2) * co.c[2][m]) * t[m];
         Nvb[m] = Nvb[m] - co.c[2][m] * t[m];
       }
-      back_nl(Nbb, cur[2], nullptr, t);  // b bar
+      back_nl(Nbb, cur[2], false, t);  // b bar
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         Vb[m] = Vb[m] + co.c[1][m] * t[m];
         Nab[m] = Nab[m] + co.c[2][m] * t[m];
       }
-      back_nl(Nab, cur[1], nullptr, t);  // a bar (second part)
+      back_nl(Nab, cur[1], false, t);  // a bar (second part)
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         Ab[m] = Ab[m] + t[m];
@@ -624,7 +640,7 @@ __global__ void __launch_bounds__(64) ks_adj_kernel(KArgs<T> a) {
         Sb[m] = Sb[m] + Nvb[m];
       }
     }
-    if (idx > 0) {
+    if (kPrefetch && idx > 0) {
 #pragma unroll
       for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -731,16 +747,18 @@ void* pick_kernel(bool adj) {
              : reinterpret_cast<void*>(&ks_fwd_kernel<T, N, R>);
 }
 
+// Radix choices: R points per lane, TL = N / R lanes per trajectory pair
+// (at most 256 = four wavefronts).  float64 builds only the default radix.
+int default_radix(int) { return 4; }
+
 template <typename T, int N>
 void* pick_r(int r, bool adj) {
   if constexpr (std::is_same<T, double>::value) {
-    constexpr int RD = N / 64 > 4 ? N / 64 : 4;
-    constexpr int RR = RD > N ? N : RD;
-    return r == RR ? pick_kernel<T, N, RR>(adj) : nullptr;
+    return r == 4 ? pick_kernel<T, N, 4>(adj) : nullptr;
   } else {
-    if constexpr (N / 4 <= 64)
+    if constexpr (N / 4 <= 256)
       if (r == 4) return pick_kernel<T, N, 4>(adj);
-    if constexpr (N / 8 <= 64 && N >= 8)
+    if constexpr (N / 8 <= 256 && N >= 8)
       if (r == 8) return pick_kernel<T, N, 8>(adj);
     if constexpr (N >= 16)
       if (r == 16) return pick_kernel<T, N, 16>(adj);
@@ -760,13 +778,6 @@ void* pick_n(int n, int r, bool adj) {
     case 1024: return pick_r<T, 1024>(r, adj);
     default: return nullptr;
   }
-}
-
-int default_radix(int n) {
-  int r = n / 64;
-  if (r < 4) r = 4;
-  if (r > n) r = n;
-  return r;
 }
 
 template <typename T>
@@ -804,8 +815,8 @@ int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int
   a.twiddle = static_cast<const T*>(d->twiddle);
   INC_REQUIRE(a.twiddle != nullptr, INC_EINVAL, "twiddle table is NULL");
   const int tl = d->n / r;
-  const int tpb = 64 / tl;
-  dim3 grid((a.npairs + tpb - 1) / tpb), block(64);
+  const int tpb = tl >= 64 ? 1 : 64 / tl;
+  dim3 grid((a.npairs + tpb - 1) / tpb), block(tl >= 64 ? tl : 64);
   void* args[] = {&a};
   INC_HIP(hipLaunchKernel(fn, grid, block, args, 0, static_cast<hipStream_t>(stream)));
   return INC_OK;

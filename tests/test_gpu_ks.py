@@ -81,7 +81,7 @@ def test_ks_golden_gradients_fp64(cuda, golden, key):
 
 
 RADIX_CASES = [(16, 4), (16, 16), (32, 8), (64, 4), (64, 16), (128, 4), (128, 8), (128, 16), (256, 4), (256, 8),
-               (256, 16), (512, 8), (512, 16), (1024, 16)]
+               (256, 16), (512, 4), (512, 8), (512, 16), (1024, 4), (1024, 8), (1024, 16)]
 
 
 @pytest.mark.parametrize("n,radix", RADIX_CASES)
