@@ -77,6 +77,9 @@ SIGNATURES = {
                                      _vp, ctypes.c_int, _vp, _vp]),
     "inc_ks_fwd": (ctypes.c_int, [ctypes.POINTER(KSDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "inc_ks_adj": (ctypes.c_int, [ctypes.POINTER(KSDesc), _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "inc_conv1d_fwd": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp]),
+    "inc_conv1d_bwd_data": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp]),
+    "inc_conv1d_bwd_weight": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp]),
 }
 
 _lib = None

@@ -1,0 +1,329 @@
+// Periodic 1D convolution kernels of the INC correction network (gfx950).
+//
+// The correction s(u) that every INC rollout step evaluates is a stack of
+// circular Conv1d layers (the reference's correction networks use 5-tap
+// circular convolutions, models/models_1d/resnet.py:5-52).  These kernels
+// replace the library convolutions on that path:
+//   y[b,o,i] = bias[o] + sum_{c,k} W[o,c,k] x[b,c,(i+k-P) mod N],  P = K/2
+// with
+//   * forward: a block covers 128 positions of one row; its input tile (with
+//     the periodic halo) is staged in LDS; each wavefront owns a wave-uniform
+//     set of output channels whose weights come through the scalar cache;
+//   * input gradient: the same kernel on the flipped, transposed weights;
+//   * weight/bias gradient: per-block partial sums over a strided set of
+//     (row, tile) pairs written to a caller workspace, then a second kernel
+//     reduces the partials in a fixed order (deterministic).
+#include "common.hpp"
+
+namespace inc {
+namespace {
+
+constexpr int kPPT = 2;    // output positions per lane
+constexpr int kWaves = 4;  // wavefronts per block, each with its own output channels
+
+__device__ __forceinline__ int wrapn(int j, int n) {
+  j %= n;
+  return j < 0 ? j + n : j;
+}
+
+// One block: TI = 64 * kPPT positions of one row, kWaves * CPT output
+// channels.  The input tile (with its periodic halo) is staged in LDS and
+// shared by the block's wavefronts; every wavefront owns CPT output channels
+// (wave-uniform), so its weights are uniform loads served by the scalar
+// cache and each lane accumulates CPT x kPPT outputs in registers.
+// FLIP: read W as the input-gradient operator W'[c_out', c_in', k] =
+// W[c_in', c_out', K-1-k] (channels swapped, taps reversed).
+template <typename T, int K, int CPT, bool FLIP>
+__global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int N, const T* __restrict__ x,
+                                                      const T* __restrict__ w, const T* __restrict__ bias,
+                                                      T* __restrict__ y) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int P = K / 2;
+  constexpr int TI = kWave * kPPT;
+  constexpr int XW = TI + K - 1;
+  T* xs = reinterpret_cast<T*>(smem_raw);  // [Ci][XW]
+  const int b = blockIdx.z;
+  const int i0 = blockIdx.x * TI;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const int lane = tid % kWave;
+  const int o0 = (blockIdx.y * kWaves + wave) * CPT;
+  const T* xb = x + size_t(b) * Ci * N;
+  for (int e = tid; e < Ci * XW; e += blockDim.x) {
+    int c = e / XW, j = e - c * XW;
+    xs[e] = xb[size_t(c) * N + wrapn(i0 - P + j, N)];
+  }
+  __syncthreads();
+  if (o0 >= Co) return;
+  T acc[CPT][kPPT];
+#pragma unroll
+  for (int a = 0; a < CPT; ++a)
+#pragma unroll
+    for (int p = 0; p < kPPT; ++p) acc[a][p] = T(0);
+  const int p0 = lane * kPPT;
+  for (int c = 0; c < Ci; ++c) {
+    T xv[kPPT + K - 1];
+#pragma unroll
+    for (int j = 0; j < kPPT + K - 1; ++j) xv[j] = xs[c * XW + p0 + j];
+#pragma unroll
+    for (int a = 0; a < CPT; ++a) {
+      const int o = o0 + a < Co ? o0 + a : Co - 1;  // clamp: extra channels are computed and dropped
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const T wv = FLIP ? w[(size_t(c) * Co + o) * K + (K - 1 - k)] : w[(size_t(o) * Ci + c) * K + k];
+#pragma unroll
+        for (int p = 0; p < kPPT; ++p) acc[a][p] += wv * xv[p + k];
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < CPT; ++a) {
+    const int o = o0 + a;
+    if (o >= Co) continue;
+    const T bo = bias ? bias[o] : T(0);
+    T* yr = y + (size_t(b) * Co + o) * N;
+#pragma unroll
+    for (int p = 0; p < kPPT; ++p) {
+      const int i = i0 + p0 + p;
+      if (i < N) yr[i] = acc[a][p] + bo;
+    }
+  }
+}
+
+// Partial weight/bias gradients of a strided set of (row, tile) pairs.
+// Thread t owns weight pairs (o, c) = t, t + nthr, ... and all K taps.
+template <typename T, int K, int MAXP>
+__global__ void __launch_bounds__(256) conv_wgrad_kernel(int B, int Ci, int Co, int N, int TI, const T* __restrict__ g,
+                                                        const T* __restrict__ x, T* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int P = K / 2;
+  const int XW = TI + K - 1;
+  T* gs = reinterpret_cast<T*>(smem_raw);  // [Co][TI]
+  T* xs = gs + size_t(Co) * TI;            // [Ci][XW]
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int npairs = Co * Ci;
+  T acc[MAXP][K];
+  T bacc = T(0);
+#pragma unroll
+  for (int q = 0; q < MAXP; ++q)
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[q][k] = T(0);
+  const int tiles = (N + TI - 1) / TI;
+  for (int job = blockIdx.x; job < B * tiles; job += gridDim.x) {
+    const int b = job / tiles, i0 = (job - b * tiles) * TI;
+    __syncthreads();
+    const T* gb = g + size_t(b) * Co * N;
+    const T* xb = x + size_t(b) * Ci * N;
+    for (int e = tid; e < Co * TI; e += nthr) {
+      int o = e / TI, j = e - o * TI;
+      gs[e] = (i0 + j < N) ? gb[size_t(o) * N + i0 + j] : T(0);
+    }
+    for (int e = tid; e < Ci * XW; e += nthr) {
+      int c = e / XW, j = e - c * XW;
+      xs[e] = xb[size_t(c) * N + wrapn(i0 - P + j, N)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+      int pr = tid + q * nthr;
+      if (pr < npairs) {
+        int o = pr / Ci, c = pr - o * Ci;
+        const T* gr = gs + o * TI;
+        const T* xr = xs + c * XW;
+        int i = 0;
+        // 8 positions at a time: 8 + K - 1 inputs and 8 gradients per 8 K FMAs
+        for (; i + 8 <= TI; i += 8) {
+          T xv[8 + K - 1], gv[8];
+#pragma unroll
+          for (int j = 0; j < 8 + K - 1; ++j) xv[j] = xr[i + j];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) gv[j] = gr[i + j];
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc[q][k] += gv[j] * xv[j + k];
+        }
+        for (; i < TI; ++i) {
+          T gv = gr[i];
+#pragma unroll
+          for (int k = 0; k < K; ++k) acc[q][k] += gv * xr[i + k];
+        }
+      }
+    }
+    if (tid < Co)
+      for (int i = 0; i < TI; ++i) bacc += gs[tid * TI + i];
+  }
+  T* out = part + size_t(blockIdx.x) * (npairs * K + Co);
+#pragma unroll
+  for (int q = 0; q < MAXP; ++q) {
+    int pr = tid + q * nthr;
+    if (pr < npairs)
+#pragma unroll
+      for (int k = 0; k < K; ++k) out[size_t(pr) * K + k] = acc[q][k];
+  }
+  if (tid < Co) out[npairs * K + tid] = bacc;
+}
+
+// Sum of nparts partial rows: a block covers 16 consecutive elements with 16
+// partial-lanes each; lanes combine in a fixed order (deterministic).
+template <typename T>
+__global__ void __launch_bounds__(256) reduce_parts_kernel(int nparts, int width, const T* __restrict__ part,
+                                                          T* __restrict__ dw, T* __restrict__ db, int nw) {
+  __shared__ T red[16][17];
+  const int ex = threadIdx.x % 16, pl = threadIdx.x / 16;
+  const int e = blockIdx.x * 16 + ex;
+  T s = T(0);
+  if (e < width)
+    for (int p = pl; p < nparts; p += 16) s += part[size_t(p) * width + e];
+  red[pl][ex] = s;
+  __syncthreads();
+  if (pl == 0 && e < width) {
+    T t = red[0][ex];
+    for (int q = 1; q < 16; ++q) t += red[q][ex];
+    if (e < nw) dw[e] = t;
+    else if (db) db[e - nw] = t;
+  }
+}
+
+template <typename T, int K, bool FLIP>
+int conv_fwd_k(int cpt, dim3 grid, size_t lds, hipStream_t st, int Ci, int Co, int N, const T* x, const T* w,
+               const T* b, T* y) {
+  dim3 block(kWaves * kWave);
+#define INC_CF(C) hipLaunchKernelGGL((conv_fwd_kernel<T, K, C, FLIP>), grid, block, lds, st, Ci, Co, N, x, w, b, y)
+  switch (cpt) {
+    case 1: INC_CF(1); break;
+    case 2: INC_CF(2); break;
+    case 4: INC_CF(4); break;
+    case 8: INC_CF(8); break;
+    case 12: INC_CF(12); break;
+    default: INC_CF(16); break;
+  }
+#undef INC_CF
+  return check_launch("conv1d");
+}
+
+// Output channels per wavefront: the smallest of 1, 2, 4, 8, 12, 16 with
+// kWaves * cpt >= Co (capped at 16; wider layers use more blocks).
+int conv_cpt(int Co) {
+  static const int opts[] = {1, 2, 4, 8, 12, 16};
+  for (int c : opts)
+    if (kWaves * c >= Co) return c;
+  return 16;
+}
+
+template <typename T, bool FLIP>
+int conv_fwd_launch(int B, int Ci, int Co, int N, int K, const void* x, const void* w, const void* bias, void* y,
+                    hipStream_t st) {
+  const int TI = kWave * kPPT;
+  size_t lds = size_t(Ci) * (TI + K - 1) * sizeof(T);
+  INC_REQUIRE(lds <= 150 * 1024, INC_EUNSUPPORTED, "conv1d: %d input channels need %zu bytes of LDS", Ci, lds);
+  const int cpt = conv_cpt(Co);
+  dim3 grid((N + TI - 1) / TI, (Co + kWaves * cpt - 1) / (kWaves * cpt), B);
+  const T* xp = static_cast<const T*>(x);
+  const T* wp = static_cast<const T*>(w);
+  const T* bp = static_cast<const T*>(bias);
+  T* yp = static_cast<T*>(y);
+  switch (K) {
+    case 1: return conv_fwd_k<T, 1, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp);
+    case 3: return conv_fwd_k<T, 3, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp);
+    case 5: return conv_fwd_k<T, 5, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp);
+    case 7: return conv_fwd_k<T, 7, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp);
+    case 9: return conv_fwd_k<T, 9, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp);
+    default: set_error("conv1d kernel size must be 1, 3, 5, 7 or 9 (got %d)", K); return INC_EUNSUPPORTED;
+  }
+}
+
+template <typename T, int K>
+int wgrad_launch_k(int B, int Ci, int Co, int N, int TI, int nblk, size_t lds, const T* g, const T* x, T* part,
+                   hipStream_t st) {
+  int npairs = Co * Ci;
+  int maxp = (npairs + 255) / 256;
+  dim3 grid(nblk), block(256);
+#define INC_WG(MP) hipLaunchKernelGGL((conv_wgrad_kernel<T, K, MP>), grid, block, lds, st, B, Ci, Co, N, TI, g, x, part)
+  if (maxp <= 1) INC_WG(1);
+  else if (maxp <= 2) INC_WG(2);
+  else if (maxp <= 4) INC_WG(4);
+  else if (maxp <= 9) INC_WG(9);
+  else if (maxp <= 16) INC_WG(16);
+  else {
+    set_error("conv1d weight gradient: %d x %d channels exceed 4096 channel pairs", Co, Ci);
+    return INC_EUNSUPPORTED;
+  }
+#undef INC_WG
+  return check_launch("conv1d weight gradient");
+}
+
+template <typename T>
+int conv_wgrad(int B, int Ci, int Co, int N, int K, const void* g, const void* x, void* dw, void* db, void* work,
+               int64_t work_elems, hipStream_t st) {
+  int TI = N < 128 ? N : 128;
+  size_t lds = (size_t(Co) * TI + size_t(Ci) * (TI + K - 1)) * sizeof(T);
+  INC_REQUIRE(lds <= 150 * 1024, INC_EUNSUPPORTED, "conv1d weight gradient needs %zu bytes of LDS", lds);
+  const int width = Co * Ci * K + Co;
+  int jobs = B * ((N + TI - 1) / TI);
+  int nblk = jobs < 256 ? jobs : 256;  // about one block per CU, each looping over several tiles
+  if (int64_t(nblk) * width > work_elems) nblk = int(work_elems / width);
+  INC_REQUIRE(nblk >= 1, INC_EINVAL, "conv1d weight gradient workspace too small (need >= %d elements)", width);
+  const T* gp = static_cast<const T*>(g);
+  const T* xp = static_cast<const T*>(x);
+  T* part = static_cast<T*>(work);
+  int rc;
+  switch (K) {
+    case 1: rc = wgrad_launch_k<T, 1>(B, Ci, Co, N, TI, nblk, lds, gp, xp, part, st); break;
+    case 3: rc = wgrad_launch_k<T, 3>(B, Ci, Co, N, TI, nblk, lds, gp, xp, part, st); break;
+    case 5: rc = wgrad_launch_k<T, 5>(B, Ci, Co, N, TI, nblk, lds, gp, xp, part, st); break;
+    case 7: rc = wgrad_launch_k<T, 7>(B, Ci, Co, N, TI, nblk, lds, gp, xp, part, st); break;
+    case 9: rc = wgrad_launch_k<T, 9>(B, Ci, Co, N, TI, nblk, lds, gp, xp, part, st); break;
+    default: set_error("conv1d kernel size must be 1, 3, 5, 7 or 9 (got %d)", K); return INC_EUNSUPPORTED;
+  }
+  if (rc) return rc;
+  hipLaunchKernelGGL(reduce_parts_kernel<T>, dim3((width + 15) / 16), dim3(256), 0, st, nblk, width, part,
+                     static_cast<T*>(dw), static_cast<T*>(db), Co * Ci * K);
+  return check_launch("conv1d weight gradient reduction");
+}
+
+int conv_check(int dtype, int B, int Ci, int Co, int N, int K) {
+  INC_REQUIRE(dtype == INC_F32 || dtype == INC_F64, INC_EINVAL, "unsupported dtype %d", dtype);
+  INC_REQUIRE(B >= 0 && Ci >= 1 && Co >= 1 && N >= 1, INC_EINVAL, "conv1d: bad sizes");
+  INC_REQUIRE(K % 2 == 1 && K <= 9 && K <= N, INC_EUNSUPPORTED, "conv1d kernel size must be odd, <= 9 and <= n");
+  return INC_OK;
+}
+
+}  // namespace
+}  // namespace inc
+
+extern "C" {
+
+int inc_conv1d_fwd(int dtype, int B, int Ci, int Co, int N, int K, const void* x, const void* w, const void* bias,
+                   void* y, void* stream) {
+  inc::clear_error();
+  int rc = inc::conv_check(dtype, B, Ci, Co, N, K);
+  if (rc || B == 0) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return dtype == INC_F32 ? inc::conv_fwd_launch<float, false>(B, Ci, Co, N, K, x, w, bias, y, st)
+                          : inc::conv_fwd_launch<double, false>(B, Ci, Co, N, K, x, w, bias, y, st);
+}
+
+int inc_conv1d_bwd_data(int dtype, int B, int Ci, int Co, int N, int K, const void* g, const void* w, void* gx,
+                        void* stream) {
+  inc::clear_error();
+  int rc = inc::conv_check(dtype, B, Ci, Co, N, K);
+  if (rc || B == 0) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // gx = conv(g, W') with W'[c][o][k] = W[o][c][K-1-k]: a forward pass with Co -> Ci channels
+  return dtype == INC_F32 ? inc::conv_fwd_launch<float, true>(B, Co, Ci, N, K, g, w, nullptr, gx, st)
+                          : inc::conv_fwd_launch<double, true>(B, Co, Ci, N, K, g, w, nullptr, gx, st);
+}
+
+int inc_conv1d_bwd_weight(int dtype, int B, int Ci, int Co, int N, int K, const void* g, const void* x, void* gw,
+                          void* gb, void* work, int64_t work_elems, void* stream) {
+  inc::clear_error();
+  int rc = inc::conv_check(dtype, B, Ci, Co, N, K);
+  if (rc) return rc;
+  INC_REQUIRE(B > 0, INC_EINVAL, "conv1d weight gradient of an empty batch");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return dtype == INC_F32 ? inc::conv_wgrad<float>(B, Ci, Co, N, K, g, x, gw, gb, work, work_elems, st)
+                          : inc::conv_wgrad<double>(B, Ci, Co, N, K, g, x, gw, gb, work, work_elems, st);
+}
+
+}  // extern "C"

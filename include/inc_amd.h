@@ -206,6 +206,26 @@ int inc_ks_fwd(const inc_ks_desc* d, const void* u, const void* s, void* u_out,
 int inc_ks_adj(const inc_ks_desc* d, const void* stages, const void* g_traj,
                void* g_u, void* g_s, int steps, void* stream);
 
+/*
+ * Circular 1D convolution of the INC correction network s(u) (the reference's
+ * correction networks stack 5-tap circular Conv1d layers,
+ * models/models_1d/resnet.py:5-52; torch.nn.Conv1d(padding_mode='circular')).
+ *   y[b,o,i] = bias[o] + sum_{c,k} w[o,c,k] x[b,c,(i+k-K/2) mod n]
+ * x [B, Ci, n], w [Co, Ci, K] (K odd <= 9), bias [Co] or NULL, y [B, Co, n].
+ */
+int inc_conv1d_fwd(int dtype, int B, int Ci, int Co, int n, int K, const void* x, const void* w,
+                   const void* bias, void* y, void* stream);
+/* Input gradient gx [B, Ci, n] from the output gradient g [B, Co, n]. */
+int inc_conv1d_bwd_data(int dtype, int B, int Ci, int Co, int n, int K, const void* g, const void* w,
+                        void* gx, void* stream);
+/*
+ * Weight and bias gradients gw [Co, Ci, K], gb [Co] (or NULL), summed over
+ * the batch in a fixed order; work: caller scratch of work_elems elements
+ * (>= Co*Ci*K + Co; up to 256 times that is used when available).
+ */
+int inc_conv1d_bwd_weight(int dtype, int B, int Ci, int Co, int n, int K, const void* g, const void* x,
+                          void* gw, void* gb, void* work, int64_t work_elems, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

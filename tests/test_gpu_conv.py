@@ -1,0 +1,57 @@
+"""GPU parity of the circular conv1d kernels of the correction network against
+torch's FP64 CPU convolution (the same definition as
+torch.nn.Conv1d(padding_mode='circular')).  float64: 1e-12 relative;
+float32: 1e-5 relative."""
+import pytest
+import torch
+from torch.nn import functional as F
+
+from inc_amd.correction import CorrectionCNN, circular_conv1d
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def ref_conv(x, w, b):
+    p = w.shape[-1] // 2
+    return F.conv1d(F.pad(x, (p, p), mode="circular"), w, b)
+
+
+SHAPES = [(3, 1, 32, 128, 5), (3, 32, 32, 128, 5), (2, 32, 1, 128, 5), (2, 48, 48, 100, 7), (5, 16, 8, 64, 3),
+          (1, 8, 8, 9, 9), (4, 32, 32, 1024, 5)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_conv_forward_and_gradients(cuda, shape, dtype):
+    bsz, ci, co, n, k = shape
+    gen = torch.Generator().manual_seed(sum(shape))
+    x = torch.randn(bsz, ci, n, generator=gen, dtype=torch.float64)
+    w = torch.randn(co, ci, k, generator=gen, dtype=torch.float64) / (ci * k) ** 0.5
+    b = torch.randn(co, generator=gen, dtype=torch.float64)
+    g = torch.randn(bsz, co, n, generator=gen, dtype=torch.float64)
+    xr, wr, br = (t.clone().requires_grad_(True) for t in (x, w, b))
+    yr = ref_conv(xr, wr, br)
+    (yr * g).sum().backward()
+    xg, wg, bg = (t.to(cuda, dtype).requires_grad_(True) for t in (x, w, b))
+    y = circular_conv1d(xg, wg, bg)
+    (y * g.to(cuda, dtype)).sum().backward()
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    assert rel_l2(y, yr.detach()) < tol
+    assert rel_l2(xg.grad, xr.grad) < tol
+    assert rel_l2(wg.grad, wr.grad) < tol
+    assert rel_l2(bg.grad, br.grad) < tol
+
+
+def test_cnn_backends_agree(cuda):
+    gen = torch.Generator().manual_seed(0)
+    hip = CorrectionCNN(4, 32, 5, generator=gen).cuda().double()
+    ref = CorrectionCNN(4, 32, 5, backend="torch").cuda().double()
+    ref.load_state_dict(hip.state_dict())
+    u = torch.randn(6, 128, dtype=torch.float64).to(cuda)
+    a, r = hip(u), ref(u)
+    assert rel_l2(a, r) < 1e-11
+    a.sum().backward()
+    r.sum().backward()
+    for p, q in zip(hip.parameters(), ref.parameters()):
+        assert rel_l2(p.grad, q.grad) < 1e-10
