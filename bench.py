@@ -212,6 +212,38 @@ def extra_ragged_inc(plain_sample=16):
             "batched_over_plain": traj_steps / t / plain_rate}
 
 
+def extra_ks_saturation(b=8192, h=200):
+    """The headline kernels at a batch large enough to fill the SIMDs (4096
+    trajectory pairs): shows their throughput-bound rate next to the
+    latency-bound headline workload."""
+    import torch
+    import inc_amd
+    dev = torch.device("cuda")
+    sv = inc_amd.KSSolverTorch(inc_amd.KSSimParams([L], resolution=N, dt=DT, time_scheme="ETDRK4", device="cuda"))
+    sv.doublestep = False
+    Ld = torch.tensor([L], device=dev)
+    u0 = initial_state(b, N, L, 1005).float().to(dev)
+    cot = torch.randn(b, h, N, device=dev) * 1e-3
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+
+    def run():
+        u = u0.clone().requires_grad_(True)
+        ev[0].record()
+        traj = sv.rollout(u, h, Ld)
+        ev[1].record()
+        torch.autograd.backward(traj, cot)
+        ev[2].record()
+    run()
+    torch.cuda.synchronize()
+    run()
+    torch.cuda.synchronize()
+    tf, ta = ev[0].elapsed_time(ev[1]) * 1e-3, ev[1].elapsed_time(ev[2]) * 1e-3
+    ts = b * h
+    return {"workload": f"KS ETDRK4 fwd+adj B={b} N={N} {h} steps fp32", "trajectory_steps_per_s": ts / (tf + ta),
+            "fwd_GBps_algorithmic": 2 * 4 * N * ts / tf / 1e9, "adj_GBps_algorithmic": 3 * 4 * N * ts / ta / 1e9,
+            "fwd_ms": tf * 1e3, "adj_ms": ta * 1e3}
+
+
 def plain_path(solver, u0, Ld, ntraj=8, h=50):
     """One launch per trajectory per step (no batching), fwd + adjoint."""
     import torch
@@ -311,8 +343,8 @@ def main():
         out["plain_trajectory_steps_per_s"] = plain
         out["batched_over_plain"] = value / plain
     if not args.no_extras:
-        out["other_workloads"] = {"burgers_inc": extra_burgers_inc(), "ks_inc_checkpointed": extra_ks_inc(),
-                                  "ragged_burgers_inc": extra_ragged_inc()}
+        out["other_workloads"] = {"ks_saturated": extra_ks_saturation(), "burgers_inc": extra_burgers_inc(),
+                                  "ks_inc_checkpointed": extra_ks_inc(), "ragged_burgers_inc": extra_ragged_inc()}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -1,0 +1,54 @@
+"""Host-side logic that needs no GPU: ragged-batch packing, grouping and issue
+order, the stable time step, the autotuner cache, the KS table cache key."""
+import json
+import math
+
+import torch
+
+from inc_amd import autotune
+from inc_amd.batched import RaggedBurgers, stable_dt
+
+
+def test_ragged_packing_groups_and_order():
+    ns = [128, 64, 1024, 64, 256]
+    hz = [10, 30, 5, 30, 20]
+    rb = RaggedBurgers(ns, [0.01] * 5, [0.001] * 5, horizon=hz, device="cpu", dtype=torch.float64)
+    assert rb.total == sum(ns)
+    assert rb.n == sorted(ns)  # grouped by resolution
+    assert [(g[0], g[2]) for g in rb.groups] == [(64, 2), (128, 1), (256, 1), (1024, 1)]
+    fields = [torch.full((n,), float(b)) for b, n in enumerate(ns)]
+    packed = rb.pack(fields)
+    for b, v in enumerate(rb.unpack(packed)):
+        assert v.shape == (ns[b],) and torch.all(v == b)
+    views = rb.group_views(packed)
+    assert [tuple(v.shape) for v in views] == [(2, 64), (1, 128), (1, 256), (1, 1024)]
+    # longest-first issue order by n * horizon (internal indices)
+    cost = [rb.n[i] * rb.horizon[i] for i in range(rb.count)]
+    order = rb.order_d.tolist()
+    assert [cost[i] for i in order] == sorted(cost, reverse=True)
+    m = rb.valid_mask(32)
+    for i, h in enumerate(rb.horizon):
+        seg = m[:, rb.offsets[i]:rb.offsets[i] + rb.n[i]]
+        assert torch.all(seg[:min(h, 32)] == 1) and torch.all(seg[h:] == 0)
+
+
+def test_stable_dt_is_min_of_advective_and_viscous():
+    n, nu = 1024, 0.03
+    dx = 2 * math.pi / n
+    assert stable_dt(n, nu) == min(0.4 * dx, 0.628 * dx * dx / nu) == 0.628 * dx * dx / nu
+    assert stable_dt(64, 0.003) == 0.4 * (2 * math.pi / 64)
+
+
+def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
+    monkeypatch.setenv("INC_AMD_TUNE_CACHE", str(tmp_path / "t.json"))
+    monkeypatch.setattr(autotune, "_mem", None)
+    assert autotune.lookup("k") is None
+    autotune._store("k", 8)
+    assert autotune.lookup("k") == 8
+    assert json.loads((tmp_path / "t.json").read_text()) == {"k": 8}
+    monkeypatch.setattr(autotune, "_mem", None)
+    assert autotune.lookup("k") == 8  # reloaded from disk
+    assert autotune.ks_radix_candidates(1024, torch.float32) == [4, 8, 16]
+    assert autotune.ks_radix_candidates(16, torch.float32) == [4, 8, 16]
+    assert autotune.ks_radix_candidates(256, torch.float64) == [4]
+    assert 16 not in autotune.burgers_team_candidates(4096)
