@@ -935,8 +935,8 @@ int ragged_launch(const inc_ragged_desc* d, const void* u, const void* s, void* 
   INC_REQUIRE(d->nmax <= 16 * team, INC_EUNSUPPORTED, "n=%d exceeds 16 points per thread of a %d-thread team",
               d->nmax, team);
   INC_REQUIRE(d->n && d->offset && d->nu && d->dt && d->dx, INC_EINVAL, "ragged descriptor arrays missing");
-  INC_REQUIRE(!adj || stages, INC_EINVAL, "adjoint needs the saved stages");
   if (d->count == 0) return INC_OK;
+  INC_REQUIRE(!adj || stages, INC_EINVAL, "adjoint needs the saved stages");
   size_t lds = size_t(adj ? 8 : 2) * d->nmax * sizeof(T);
   INC_REQUIRE(lds <= 160 * 1024 - 1024, INC_EUNSUPPORTED, "n=%d needs %zu bytes of LDS", d->nmax, lds);
   RArgs<T> r{};

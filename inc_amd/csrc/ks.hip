@@ -790,7 +790,9 @@ int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int
   INC_REQUIRE(d->substeps >= 1, INC_EINVAL, "substeps must be >= 1");
   INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
   INC_REQUIRE(d->table != nullptr, INC_EINVAL, "coefficient table is NULL");
+  if (d->batch == 0) return INC_OK;
   INC_REQUIRE(!adj || stages != nullptr, INC_EINVAL, "adjoint needs the saved stages");
+  INC_REQUIRE(!adj || g_traj != nullptr, INC_EINVAL, "adjoint needs the output cotangents (g_traj is NULL)");
   int r = d->radix > 0 ? d->radix : default_radix(d->n);
   if (std::is_same<T, double>::value) r = default_radix(d->n);
   void* fn = pick_n<T>(d->n, r, adj);
@@ -868,10 +870,6 @@ int inc_ks_fwd(const inc_ks_desc* d, const void* u, const void* s, void* u_out, 
 
 int inc_ks_adj(const inc_ks_desc* d, const void* stages, const void* g_traj, void* g_u, void* g_s, int steps,
                void* stream) {
-  if (g_traj == nullptr) {
-    inc::set_error("g_traj is NULL");
-    return INC_EINVAL;
-  }
   return inc::ks_dispatch(d, nullptr, nullptr, g_u, steps, nullptr, const_cast<void*>(stages), g_traj, g_s,
                           true, stream);
 }
