@@ -91,75 +91,100 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int N, co
 }
 
 // Partial weight/bias gradients of a strided set of (row, tile) pairs.
-// Thread t owns weight pairs (o, c) = t, t + nthr, ... and all K taps.
+// Thread t owns a 2 x 2 block of (output, input) channel pairs (blocks t,
+// t + nthr, ...) and all K taps, so each staged gradient / input value
+// feeds two weight pairs.  LDS rows are skewed to an odd stride so the
+// threads of a wavefront (consecutive input channels) hit distinct banks.
 template <typename T, int K, int MAXP>
 __global__ void __launch_bounds__(256) conv_wgrad_kernel(int B, int Ci, int Co, int N, int TI, const T* __restrict__ g,
                                                         const T* __restrict__ x, T* __restrict__ part) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int P = K / 2;
-  const int XW = TI + K - 1;
-  T* gs = reinterpret_cast<T*>(smem_raw);  // [Co][TI]
-  T* xs = gs + size_t(Co) * TI;            // [Ci][XW]
+  const int GW = TI | 1;                   // gradient row stride (odd)
+  const int XW = (TI + K - 1) | 1;         // input row stride (odd)
+  T* gs = reinterpret_cast<T*>(smem_raw);  // [Co2][GW]
+  const int Co2 = (Co + 1) & ~1, Ci2 = (Ci + 1) & ~1;
+  T* xs = gs + size_t(Co2) * GW;           // [Ci2][XW]
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int npairs = Co * Ci;
-  T acc[MAXP][K];
+  const int nblk2 = (Co2 / 2) * (Ci2 / 2);  // 2 x 2 channel blocks
+  T acc[MAXP][4][K];
   T bacc = T(0);
 #pragma unroll
   for (int q = 0; q < MAXP; ++q)
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[q][k] = T(0);
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[q][r][k] = T(0);
   const int tiles = (N + TI - 1) / TI;
   for (int job = blockIdx.x; job < B * tiles; job += gridDim.x) {
     const int b = job / tiles, i0 = (job - b * tiles) * TI;
     __syncthreads();
     const T* gb = g + size_t(b) * Co * N;
     const T* xb = x + size_t(b) * Ci * N;
-    for (int e = tid; e < Co * TI; e += nthr) {
+    for (int e = tid; e < Co2 * TI; e += nthr) {
       int o = e / TI, j = e - o * TI;
-      gs[e] = (i0 + j < N) ? gb[size_t(o) * N + i0 + j] : T(0);
+      gs[o * GW + j] = (o < Co && i0 + j < N) ? gb[size_t(o) * N + i0 + j] : T(0);
     }
-    for (int e = tid; e < Ci * XW; e += nthr) {
-      int c = e / XW, j = e - c * XW;
-      xs[e] = xb[size_t(c) * N + wrapn(i0 - P + j, N)];
+    const int xw = TI + K - 1;
+    for (int e = tid; e < Ci2 * xw; e += nthr) {
+      int c = e / xw, j = e - c * xw;
+      xs[c * XW + j] = c < Ci ? xb[size_t(c) * N + wrapn(i0 - P + j, N)] : T(0);
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MAXP; ++q) {
-      int pr = tid + q * nthr;
-      if (pr < npairs) {
-        int o = pr / Ci, c = pr - o * Ci;
-        const T* gr = gs + o * TI;
-        const T* xr = xs + c * XW;
+      const int blk = tid + q * nthr;
+      if (blk < nblk2) {
+        const int o = 2 * (blk / (Ci2 / 2)), c = 2 * (blk % (Ci2 / 2));
+        const T* g0 = gs + o * GW;
+        const T* g1 = g0 + GW;
+        const T* x0 = xs + c * XW;
+        const T* x1 = x0 + XW;
         int i = 0;
-        // 8 positions at a time: 8 + K - 1 inputs and 8 gradients per 8 K FMAs
         for (; i + 8 <= TI; i += 8) {
-          T xv[8 + K - 1], gv[8];
+          T xa[8 + K - 1], xc[8 + K - 1], ga[8], gc[8];
 #pragma unroll
-          for (int j = 0; j < 8 + K - 1; ++j) xv[j] = xr[i + j];
+          for (int j = 0; j < 8 + K - 1; ++j) { xa[j] = x0[i + j]; xc[j] = x1[i + j]; }
 #pragma unroll
-          for (int j = 0; j < 8; ++j) gv[j] = gr[i + j];
+          for (int j = 0; j < 8; ++j) { ga[j] = g0[i + j]; gc[j] = g1[i + j]; }
 #pragma unroll
           for (int j = 0; j < 8; ++j)
 #pragma unroll
-            for (int k = 0; k < K; ++k) acc[q][k] += gv[j] * xv[j + k];
+            for (int k = 0; k < K; ++k) {
+              acc[q][0][k] += ga[j] * xa[j + k];
+              acc[q][1][k] += ga[j] * xc[j + k];
+              acc[q][2][k] += gc[j] * xa[j + k];
+              acc[q][3][k] += gc[j] * xc[j + k];
+            }
         }
-        for (; i < TI; ++i) {
-          T gv = gr[i];
+        for (; i < TI; ++i)
 #pragma unroll
-          for (int k = 0; k < K; ++k) acc[q][k] += gv * xr[i + k];
-        }
+          for (int k = 0; k < K; ++k) {
+            acc[q][0][k] += g0[i] * x0[i + k];
+            acc[q][1][k] += g0[i] * x1[i + k];
+            acc[q][2][k] += g1[i] * x0[i + k];
+            acc[q][3][k] += g1[i] * x1[i + k];
+          }
       }
     }
     if (tid < Co)
-      for (int i = 0; i < TI; ++i) bacc += gs[tid * TI + i];
+      for (int i = 0; i < TI; ++i) bacc += gs[tid * GW + i];
   }
+  const int npairs = Co * Ci;
   T* out = part + size_t(blockIdx.x) * (npairs * K + Co);
 #pragma unroll
   for (int q = 0; q < MAXP; ++q) {
-    int pr = tid + q * nthr;
-    if (pr < npairs)
+    const int blk = tid + q * nthr;
+    if (blk < nblk2) {
+      const int o = 2 * (blk / (Ci2 / 2)), c = 2 * (blk % (Ci2 / 2));
 #pragma unroll
-      for (int k = 0; k < K; ++k) out[size_t(pr) * K + k] = acc[q][k];
+      for (int r = 0; r < 4; ++r) {
+        const int oo = o + (r >> 1), cc = c + (r & 1);
+        if (oo < Co && cc < Ci)
+#pragma unroll
+          for (int k = 0; k < K; ++k) out[(size_t(oo) * Ci + cc) * K + k] = acc[q][r][k];
+      }
+    }
   }
   if (tid < Co) out[npairs * K + tid] = bacc;
 }
@@ -236,15 +261,14 @@ int conv_fwd_launch(int B, int Ci, int Co, int N, int K, const void* x, const vo
 template <typename T, int K>
 int wgrad_launch_k(int B, int Ci, int Co, int N, int TI, int nblk, size_t lds, const T* g, const T* x, T* part,
                    hipStream_t st) {
-  int npairs = Co * Ci;
-  int maxp = (npairs + 255) / 256;
+  const int nblk2 = ((Co + 1) / 2) * ((Ci + 1) / 2);
+  const int maxp = (nblk2 + 255) / 256;
   dim3 grid(nblk), block(256);
 #define INC_WG(MP) hipLaunchKernelGGL((conv_wgrad_kernel<T, K, MP>), grid, block, lds, st, B, Ci, Co, N, TI, g, x, part)
   if (maxp <= 1) INC_WG(1);
   else if (maxp <= 2) INC_WG(2);
+  else if (maxp <= 3) INC_WG(3);
   else if (maxp <= 4) INC_WG(4);
-  else if (maxp <= 9) INC_WG(9);
-  else if (maxp <= 16) INC_WG(16);
   else {
     set_error("conv1d weight gradient: %d x %d channels exceed 4096 channel pairs", Co, Ci);
     return INC_EUNSUPPORTED;
@@ -257,7 +281,7 @@ template <typename T>
 int conv_wgrad(int B, int Ci, int Co, int N, int K, const void* g, const void* x, void* dw, void* db, void* work,
                int64_t work_elems, hipStream_t st) {
   int TI = N < 128 ? N : 128;
-  size_t lds = (size_t(Co) * TI + size_t(Ci) * (TI + K - 1)) * sizeof(T);
+  size_t lds = (size_t((Co + 1) & ~1) * (TI | 1) + size_t((Ci + 1) & ~1) * ((TI + K - 1) | 1)) * sizeof(T);
   INC_REQUIRE(lds <= 150 * 1024, INC_EUNSUPPORTED, "conv1d weight gradient needs %zu bytes of LDS", lds);
   const int width = Co * Ci * K + Co;
   int jobs = B * ((N + TI - 1) / TI);
