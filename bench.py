@@ -162,7 +162,7 @@ def extra_ks_inc():
 
 def extra_ragged_inc(plain_sample=16):
     """Uneven Burgers batch: 2048 trajectories, N in {64..1024}, nu ~ LogU[0.003, 0.03], dt = min(CFL 0.4,
-    viscous), horizons ~ U{16..64}, 64-step INC rollout with the config-2 CNN, fwd+adj, autotuned block size;
+    viscous), horizons ~ U{16..64}, 64-step INC rollout with a CNN(4x32,k5) correction, fwd+adj, autotuned block size;
     against the plain per-trajectory path (one launch per trajectory per step) on a sample."""
     import torch
     import inc_amd

@@ -144,7 +144,7 @@ def test_ks_dealias_off_matches_oracle(cuda):
 
 
 def test_ks_fused_rollout_fp32_teacher_forced_and_gradient(cuda):
-    # config-1 shape, shortened: B=256, N=256, L=64, dt=0.25, ETDRK4, fp32
+    # headline KS shape with a shortened horizon: B=256, N=256, L=64, dt=0.25, ETDRK4, fp32
     n, b, L, dt, H = 256, 256, 64.0, 0.25, 40
     gen = torch.Generator().manual_seed(1001)
     u0 = modes(b, n, L, gen)
@@ -210,7 +210,7 @@ def test_ks_rollout_gradient_matches_fp64_oracle(cuda):
 
 
 def test_ks_inc_rollout_checkpointing_matches_full_graph(cuda):
-    # config-3 pattern, small: CNN correction, checkpoint every 4 steps
+    # KS INC pattern at a small size: CNN correction, checkpoint every 4 steps
     n, b, L, dt, H = 128, 8, 100.0, 0.25, 12
     gen = torch.Generator().manual_seed(12)
     u0 = modes(b, n, L, gen, kmax=16).float().to(cuda)
@@ -237,7 +237,7 @@ def test_ks_inc_rollout_checkpointing_matches_full_graph(cuda):
 
 
 def test_ks_fd_gradient_fp64_32_directions(cuda):
-    # FP64 central differences on 32 random directions (config-3 check, small)
+    # FP64 central differences on 32 random directions (L=100 KS INC setting, small)
     n, b, L, dt, H = 64, 2, 100.0, 0.25, 6
     gen = torch.Generator().manual_seed(77)
     u0 = modes(b, n, L, gen, kmax=16).to(cuda)
