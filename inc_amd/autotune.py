@@ -4,7 +4,8 @@ Times every valid launch shape of a solver kernel on the live GPU and keeps
 the fastest, cached in a JSON file keyed by GPU architecture and problem
 shape (``$INC_AMD_TUNE_CACHE``, default ``~/.cache/inc_amd/tune.json``):
 
-* KS: FFT points per lane (radix 4 / 8 / 16; lanes per trajectory pair = n/radix);
+* KS: FFT points per lane (radix 2 / 4 / 8, float64 2 / 4; lanes per
+  trajectory pair = n/radix, at most 256);
 * Burgers (uniform batch): threads per trajectory (team);
 * Burgers (ragged batch): threads per trajectory block (64 / 128 / 256).
 
@@ -85,9 +86,8 @@ def _time_ms(fn, reps=3):
 
 
 def ks_radix_candidates(n, dtype):
-    if dtype == torch.float64:
-        return [4]
-    return [r for r in (4, 8, 16) if r <= n and n // r <= 256]
+    radices = (2, 4) if dtype == torch.float64 else (2, 4, 8)
+    return [r for r in radices if r <= n and n // r <= 256]
 
 
 def tune_ks(n, batch, dtype=torch.float32, scheme="ETDRK4", L=64.0, dt=0.25, steps=16, record=None):

@@ -13,11 +13,16 @@
 //    separated: all steps run on the packed spectrum.  Pairs whose two
 //    trajectories have different coefficient tables run unpacked (imaginary
 //    part zero).
-//  * A team of TL = N/R lanes (within one wavefront) owns a pair; lane j
-//    holds points / modes j + m*TL, m < R, in registers (the "strided"
-//    layout).  A radix-R Stockham FFT reads its first pass from registers
-//    and writes its last pass to registers, exchanging through LDS in
-//    between (log_R N - 1 exchanges, double-buffered, wavefront-synchronous).
+//  * A team of TL = N/R lanes (part of a wavefront, or up to four whole
+//    wavefronts) owns a pair; lane j holds points j + m*TL, m < R, in
+//    registers (the "strided" layout).  The FFT is a decimation-in-frequency
+//    transform whose digit exchanges between registers and lanes are
+//    cross-lane register operations (DPP, v_permlane16/32_swap) inside a
+//    wavefront and one LDS round trip per digit across wavefronts; modes end
+//    up in a digit-permuted order that every spectral operator follows.
+//  * The radix trades lanes for registers: R = 2 spreads a pair over twice
+//    the lanes of R = 4 (half the instructions per lane), which wins while
+//    the batch leaves SIMDs idle; R = 4 wins once every SIMD has work.
 //  * Spectral quantities are kept scaled by 1/N, so the unnormalised inverse
 //    transform returns physical values directly.
 //  * The state stays in registers across all steps of a launch; HBM sees
@@ -147,133 +152,285 @@ __device__ __forceinline__ void dft(cpx<T>* x) {
   else dft16<SIGN>(x);
 }
 
-// ---- FFT plan -----------------------------------------------------------------
-constexpr int plan_passes(int N, int R) {
-  int c = 0;
-  for (int ns = 1; ns < N; ns *= (R < N / ns ? R : N / ns)) ++c;
-  return c;
-}
-constexpr int plan_ns(int N, int R, int s) {
-  int ns = 1;
-  for (int i = 0; i < s; ++i) ns *= (R < N / ns ? R : N / ns);
-  return ns;
-}
-constexpr int plan_p(int N, int R, int s) {
-  int ns = plan_ns(N, R, s);
-  return R < N / ns ? R : N / ns;
-}
-// twiddle registers used by passes 1..s-1
-constexpr int plan_tw_off(int N, int R, int s) {
-  int o = 0;
-  for (int i = 1; i < s; ++i) o += (plan_p(N, R, i) - 1) * (R / plan_p(N, R, i));
-  return o;
-}
-
-template <typename T, int N, int R>
-struct Plan {
-  static constexpr int TL = N / R;
-  static constexpr int NP = plan_passes(N, R);
-  static constexpr int NTW = plan_tw_off(N, R, NP) > 0 ? plan_tw_off(N, R, NP) : 1;
-};
-
-// LDS slot of complex element p: one padding slot per 32 elements keeps the
-// strided Stockham writes off a single bank
-__device__ __forceinline__ int lds_slot(int p) { return p + (p >> 5); }
-template <int N>
-constexpr int lds_team_elems() { return 2 * (N + N / 32); }
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// ---- register-transpose FFT -----------------------------------------------------
+// Decimation in frequency over N = R * TL points, TL = 2^t lanes.  Index bit
+// p of a point is register bit p (p < rho = log2 R) or lane bit p - rho.  In
+// the physical layout lane j holds n = j + m*TL in register m, so the first
+// digit (the top rho bits of n) is already in registers: pass 0 is a radix-R
+// butterfly over the registers.  Pass s >= 1 handles the next lane digit
+// (d_s <= rho bits at lane bits [c_s, c_s + d_s), taken from the top): the
+// low d_s register bits, which hold finished output bits, are exchanged with
+// those lane bits, then a radix-2^d_s butterfly runs over them.  Between
+// passes the twiddle W_{M_s}^{n_rest * k_s} (M_s = 2^(c_s + d_s), n_rest =
+// lane bits below c_s) is applied.  The exchange of register bit b with lane
+// bit c uses DPP row shifts (c <= 3, inside a row of 16 lanes) or
+// v_permlane16_swap / v_permlane32_swap (c = 4, 5): no LDS and no waits.
+// Teams of 128 or 256 lanes span two or four wavefronts; their lane bits 6
+// and 7 (the wavefront index) are exchanged through LDS, one round trip per
+// digit.  The output is in a digit-permuted spectral order (spec_index);
+// the inverse runs the same passes backwards with conjugate butterflies and
+// twiddles and returns to the physical layout.
+constexpr int ilog2c(int x) {
+  int r = 0;
+  while ((1 << r) < x) ++r;
+  return r;
 }
 
-// A team of TL <= 64 lanes lives in one wavefront (LDS traffic is ordered
-// within a wavefront, so a compiler barrier suffices); a wider team is one
-// whole block of TL / 64 wavefronts and synchronises with s_barrier.
+// A team of more than 64 lanes is one whole block of TL / 64 wavefronts and
+// synchronises with s_barrier (which waits for the block's LDS traffic).
 template <int TL>
 __device__ __forceinline__ void team_sync() {
   if constexpr (TL > 64) __syncthreads();
-  else wave_sync();
 }
 
-// Per-lane twiddles W_N^k (forward sign) of passes 1..NP-1.
-template <typename T, int N, int R>
-__device__ __forceinline__ void load_twiddles(const T* tab, int j, cpx<T>* tw) {
-  using PL = Plan<T, N, R>;
-  constexpr int TL = PL::TL;
-#pragma unroll
-  for (int s = 1; s < PL::NP; ++s) {
-    const int ns = plan_ns(N, R, s), P = plan_p(N, R, s), Q = R / P;
-    const int off = plan_tw_off(N, R, s);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      int jb = j + q * TL;
-#pragma unroll
-      for (int r = 1; r < P; ++r) {
-        int k = ((jb % ns) * r * (N / (ns * P))) % N;
-        tw[off + q * (P - 1) + (r - 1)] = cpx<T>{tab[2 * k], tab[2 * k + 1]};
+// Cross-wavefront exchange buffers of a team: two N-element halves used
+// alternately, so that an exchange never overwrites data another wavefront
+// may still be reading from the previous one (every wavefront passed that
+// exchange's barrier only after its own earlier reads completed).
+template <typename T>
+struct XBuf {
+  cpx<T>* base;
+  int par;
+};
+template <int N, int R>
+struct RPlan {
+  static constexpr int rho = ilog2c(R);
+  static constexpr int t = ilog2c(N) - rho;
+  static constexpr int S = 1 + (t + rho - 1) / rho;
+  // digit bits of pass s
+  static constexpr int d(int s) { return s == 0 ? rho : (t - (s - 1) * rho < rho ? t - (s - 1) * rho : rho); }
+  // lane bits below the digit of pass s
+  static constexpr int c(int s) { return s == 0 ? t : (t - s * rho > 0 ? t - s * rho : 0); }
+  static constexpr int tw_off(int s) {
+    int o = 0;
+    for (int i = 0; i < s; ++i) o += (1 << d(i)) - 1;
+    return o;
+  }
+  static constexpr int NTW = tw_off(S - 1) > 0 ? tw_off(S - 1) : 1;
+};
+
+// k-bit held at each index position after the forward passes
+struct BitMap {
+  int k[12];
+};
+template <int N, int R>
+constexpr BitMap spec_bits() {
+  using P = RPlan<N, R>;
+  BitMap b{};
+  for (int p = 0; p < 12; ++p) b.k[p] = 0;
+  int done = 0;
+  for (int s = 0; s < P::S; ++s) {
+    if (s > 0)
+      for (int i = 0; i < P::d(s); ++i) {
+        int q = P::rho + P::c(s) + i, tmp = b.k[i];
+        b.k[i] = b.k[q];
+        b.k[q] = tmp;
       }
-    }
+    for (int i = 0; i < P::d(s); ++i) b.k[i] = done + i;
+    done += P::d(s);
+  }
+  return b;
+}
+
+// mode index held by register m of lane j after a forward register FFT
+template <int N, int R>
+__device__ __forceinline__ int spec_index(int m, int j) {
+  using P = RPlan<N, R>;
+  constexpr BitMap b = spec_bits<N, R>();
+  int k = 0;
+#pragma unroll
+  for (int p = 0; p < P::rho; ++p) k |= ((m >> p) & 1) << b.k[p];
+#pragma unroll
+  for (int p = 0; p < P::t; ++p) k |= ((j >> p) & 1) << b.k[P::rho + p];
+  return k;
+}
+
+// Exchange word a's lanes with lane bit C set against word b's lanes with it
+// clear (a belongs to register bit 0, b to register bit 1 of the swapped pair).
+template <int C>
+__device__ __forceinline__ void swap_lane_bit(unsigned& a, unsigned& b) {
+  if constexpr (C == 5) {
+    auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+  } else if constexpr (C == 4) {
+    auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+  } else if constexpr (C >= 2) {
+    // the lanes with bit C set form whole 4-lane banks: merge by bank mask
+    constexpr int hi = C == 2 ? 0xA : 0xC;
+    unsigned na = __builtin_amdgcn_update_dpp(a, b, 0x110 + (1 << C), 0xF, hi, false);      // row_shr
+    unsigned nb = __builtin_amdgcn_update_dpp(b, a, 0x100 + (1 << C), 0xF, hi ^ 0xF, false);  // row_shl
+    a = na;
+    b = nb;
+  } else {
+    const bool hi = (__lane_id() >> C) & 1;
+    unsigned from_b = __builtin_amdgcn_mov_dpp(b, 0x110 + (1 << C), 0xF, 0xF, true);  // b[lane - 2^C]
+    unsigned from_a = __builtin_amdgcn_mov_dpp(a, 0x100 + (1 << C), 0xF, 0xF, true);  // a[lane + 2^C]
+    a = hi ? from_b : a;
+    b = hi ? b : from_a;
   }
 }
 
-template <typename T, int N, int R, int SIGN, int S>
-__device__ __forceinline__ void fft_pass(cpx<T>* v, const cpx<T>* tw, cpx<T>* lds, int j) {
-  constexpr int TL = N / R;
-  constexpr int NP = plan_passes(N, R);
-  constexpr int Ns = plan_ns(N, R, S);
-  constexpr int P = plan_p(N, R, S);
-  constexpr int Q = R / P;
-  constexpr bool last = (S == NP - 1);
-  constexpr int off = plan_tw_off(N, R, S);
-  constexpr int half = lds_team_elems<N>() / 2;
-  if constexpr (S > 0) {
-    const cpx<T>* rd = lds + ((S - 1) & 1) * half;
+template <int C, typename T>
+__device__ __forceinline__ void swap_lane_bit(T& a, T& b) {
+  constexpr int W = sizeof(T) / 4;
+  unsigned wa[W], wb[W];
+  __builtin_memcpy(wa, &a, sizeof(T));
+  __builtin_memcpy(wb, &b, sizeof(T));
 #pragma unroll
-    for (int m = 0; m < R; ++m) v[m] = rd[lds_slot(j + m * TL)];
+  for (int w = 0; w < W; ++w) swap_lane_bit<C>(wa[w], wb[w]);
+  __builtin_memcpy(&a, wa, sizeof(T));
+  __builtin_memcpy(&b, wb, sizeof(T));
+}
+
+// exchange register bits [0, d_s) with the in-wavefront lane bits (< 6)
+// among [c_s, c_s + d_s)
+template <typename T, int N, int R, int S, int I = 0>
+__device__ __forceinline__ void exchange_wave_bits(cpx<T>* v) {
+  using P = RPlan<N, R>;
+  if constexpr (I < P::d(S)) {
+    if constexpr (P::c(S) + I < 6) {
+#pragma unroll
+      for (int m = 0; m < R; ++m)
+        if (!((m >> I) & 1)) {
+          swap_lane_bit<P::c(S) + I>(v[m].x, v[m | (1 << I)].x);
+          swap_lane_bit<P::c(S) + I>(v[m].y, v[m | (1 << I)].y);
+        }
+    }
+    exchange_wave_bits<T, N, R, S, I + 1>(v);
   }
-  cpx<T>* wr = lds + (S & 1) * half;
+}
+
+// register bits of digit S that pair with wavefront-index lane bits (>= 6)
+template <int N, int R, int S>
+constexpr int cross_wave_mask() {
+  using P = RPlan<N, R>;
+  int mk = 0;
+  for (int i = 0; i < P::d(S); ++i)
+    if (P::c(S) + i >= 6) mk |= 1 << i;
+  return mk;
+}
+
+// Exchange of the register bits in cross_wave_mask with lane bits >= 6 (the
+// team spans TL / 64 wavefronts): every lane writes its R values to the slots
+// of their new owners and reads its own slots back.  Slot (m, L) = m*TL + L;
+// within one write instruction all lanes of a wavefront share m' (it takes
+// only wavefront-index bits), so both sides are unit-stride and conflict-free.
+template <typename T, int N, int R, int S>
+__device__ __forceinline__ void exchange_cross_wave(cpx<T>* v, XBuf<T>& xb, int j) {
+  using P = RPlan<N, R>;
+  constexpr int TL = N / R, mk = cross_wave_mask<N, R, S>();
+  if constexpr (mk != 0) {
+    cpx<T>* lds = xb.base + xb.par * N;
+    xb.par ^= 1;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
+    for (int m = 0; m < R; ++m) {
+      int m2 = m & ~mk, j2 = j;
+#pragma unroll
+      for (int i = 0; i < P::d(S); ++i)
+        if ((mk >> i) & 1) {
+          const int c = P::c(S) + i;
+          m2 |= ((j >> c) & 1) << i;
+          j2 = (j2 & ~(1 << c)) | (((m >> i) & 1) << c);
+        }
+      lds[m2 * TL + j2] = v[m];
+    }
+    team_sync<TL>();
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[m] = lds[m * TL + j];
+  }
+}
+
+// exchange register bits [0, d_s) with lane bits [c_s, c_s + d_s)
+template <typename T, int N, int R, int S>
+__device__ __forceinline__ void exchange_digit(cpx<T>* v, XBuf<T>& xb, int j) {
+  exchange_wave_bits<T, N, R, S>(v);
+  exchange_cross_wave<T, N, R, S>(v, xb, j);
+}
+
+// radix-2^d_s butterflies over register bits [0, d_s)
+template <typename T, int N, int R, int S, int SIGN>
+__device__ __forceinline__ void reg_butterfly(cpx<T>* v) {
+  constexpr int D = RPlan<N, R>::d(S), P = 1 << D;
+#pragma unroll
+  for (int q = 0; q < R / P; ++q) {
     cpx<T> x[P];
 #pragma unroll
-    for (int r = 0; r < P; ++r) x[r] = v[q + r * Q];
-    if constexpr (S > 0) {
-#pragma unroll
-      for (int r = 1; r < P; ++r) {
-        cpx<T> w = tw[off + q * (P - 1) + (r - 1)];
-        x[r] = SIGN < 0 ? cmul(x[r], w) : cmulc(x[r], w);
-      }
-    }
+    for (int r = 0; r < P; ++r) x[r] = v[(q << D) | r];
     dft<P, SIGN>(x);
-    if constexpr (last) {
 #pragma unroll
-      for (int r = 0; r < P; ++r) v[q + r * Q] = x[r];
-    } else {
-      const int jb = j + q * TL;
-      const int base = (jb / Ns) * Ns * P + (jb % Ns);
+    for (int r = 0; r < P; ++r) v[(q << D) | r] = x[r];
+  }
+}
+
+template <typename T, int N, int R, int S, bool CONJ>
+__device__ __forceinline__ void reg_twiddle(cpx<T>* v, const cpx<T>* tw) {
+  using PL = RPlan<N, R>;
+  constexpr int P = 1 << PL::d(S), off = PL::tw_off(S);
 #pragma unroll
-      for (int r = 0; r < P; ++r) wr[lds_slot(base + r * Ns)] = x[r];
+  for (int m = 0; m < R; ++m) {
+    const int r = m & (P - 1);
+    if (r) v[m] = CONJ ? cmulc(v[m], tw[off + r - 1]) : cmul(v[m], tw[off + r - 1]);
+  }
+}
+
+template <typename T, int N, int R, int S>
+__device__ __forceinline__ void reg_fft_fwd(cpx<T>* v, const cpx<T>* tw, XBuf<T>& xb, int j) {
+  using PL = RPlan<N, R>;
+  if constexpr (S < PL::S) {
+    if constexpr (S > 0) exchange_digit<T, N, R, S>(v, xb, j);
+    reg_butterfly<T, N, R, S, -1>(v);
+    if constexpr (S < PL::S - 1) reg_twiddle<T, N, R, S, false>(v, tw);
+    reg_fft_fwd<T, N, R, S + 1>(v, tw, xb, j);
+  }
+}
+
+template <typename T, int N, int R, int S>
+__device__ __forceinline__ void reg_fft_inv(cpx<T>* v, const cpx<T>* tw, XBuf<T>& xb, int j) {
+  if constexpr (S >= 0) {
+    reg_butterfly<T, N, R, S, +1>(v);
+    if constexpr (S > 0) {
+      exchange_digit<T, N, R, S>(v, xb, j);
+      reg_twiddle<T, N, R, S - 1, true>(v, tw);
+    }
+    reg_fft_inv<T, N, R, S - 1>(v, tw, xb, j);
+  }
+}
+
+// per-lane twiddles of the register transform
+template <typename T, int N, int R>
+__device__ __forceinline__ void load_reg_twiddles(const T* tab, int j, cpx<T>* tw) {
+  using PL = RPlan<N, R>;
+#pragma unroll
+  for (int s = 0; s < PL::S - 1; ++s) {
+    const int P = 1 << PL::d(s), cs = PL::c(s);
+    const int nrest = j & ((1 << cs) - 1);
+    const int scale = N >> (cs + PL::d(s));  // N / M_s
+#pragma unroll
+    for (int r = 1; r < P; ++r) {
+      const int k = (nrest * r * scale) & (N - 1);
+      tw[PL::tw_off(s) + r - 1] = cpx<T>{tab[2 * k], tab[2 * k + 1]};
     }
   }
-  if constexpr (!last) team_sync<TL>();
 }
 
-template <typename T, int N, int R, int SIGN, int S>
-__device__ __forceinline__ void fft_passes(cpx<T>* v, const cpx<T>* tw, cpx<T>* lds, int j) {
-  if constexpr (S < plan_passes(N, R)) {
-    fft_pass<T, N, R, SIGN, S>(v, tw, lds, j);
-    fft_passes<T, N, R, SIGN, S + 1>(v, tw, lds, j);
-  }
-}
+// ---- transform used by the kernels -----------------------------------------------
+template <typename T, int N, int R>
+struct FftCfg {
+  static_assert(N >= R && N / R <= 256, "a team is at most four wavefronts");
+  static constexpr int NTW = RPlan<N, R>::NTW;
+  // LDS per team: the two cross-wavefront exchange buffers (teams of > 64 lanes)
+  static constexpr int LDS_TEAM = N / R > 64 ? 2 * N : 0;
+};
 
-// Unnormalised DFT (SIGN = -1) or inverse DFT without 1/N (SIGN = +1) of the
-// team's N values, strided layout in and out.
+// Unnormalised DFT (SIGN = -1) from the physical strided layout to the
+// transform's spectral layout, or inverse DFT without 1/N (SIGN = +1) back.
 template <typename T, int N, int R, int SIGN>
-__device__ __forceinline__ void fft(cpx<T>* v, const cpx<T>* tw, cpx<T>* lds, int j) {
-  team_sync<N / R>();
-  fft_passes<T, N, R, SIGN, 0>(v, tw, lds, j);
+__device__ __forceinline__ void fft(cpx<T>* v, const cpx<T>* tw, XBuf<T>& xb, int j) {
+  if constexpr (SIGN < 0) reg_fft_fwd<T, N, R, 0>(v, tw, xb, j);
+  else reg_fft_inv<T, N, R, RPlan<N, R>::S - 1>(v, tw, xb, j);
 }
 
 // ---- kernel arguments -----------------------------------------------------------
@@ -305,11 +462,10 @@ struct Coef {
 
 template <typename T, int N, int R>
 __device__ __forceinline__ void load_coef(const T* tab, int j, Coef<T, N, R>& co) {
-  constexpr int TL = N / R;
   const T invn = T(1) / T(N);
 #pragma unroll
   for (int m = 0; m < R; ++m) {
-    int k = j + m * TL;
+    int k = spec_index<N, R>(m, j);
     int af = k <= N / 2 ? k : N - k;
     const T* row = tab + af * kTableStride;
 #pragma unroll
@@ -350,7 +506,7 @@ __device__ __forceinline__ void store_pair(T* base, size_t stride_b, int b0, int
 // nonlinear term in the scaled convention: G~ * F(z^2) with G~ = -i g/N
 template <typename T, int N, int R>
 __device__ __forceinline__ void square_fft_g(cpx<T>* z, const Coef<T, N, R>& co, const cpx<T>* tw,
-                                             cpx<T>* lds, int j) {
+                                             XBuf<T>& lds, int j) {
 #pragma unroll
   for (int m = 0; m < R; ++m) z[m] = cpx<T>{z[m].x * z[m].x, z[m].y * z[m].y};
   fft<T, N, R, -1>(z, tw, lds, j);
@@ -364,7 +520,7 @@ __device__ __forceinline__ void square_fft_g(cpx<T>* z, const Coef<T, N, R>& co,
 // w-bar = Re F^H(conj(G~) X): multiply by +i g/N, inverse transform
 template <typename T, int N, int R>
 __device__ __forceinline__ void adj_g_ifft(const cpx<T>* X, cpx<T>* y, const Coef<T, N, R>& co,
-                                           const cpx<T>* tw, cpx<T>* lds, int j) {
+                                           const cpx<T>* tw, XBuf<T>& lds, int j) {
 #pragma unroll
   for (int m = 0; m < R; ++m) {
     T g = co.c[6][m];
@@ -387,18 +543,18 @@ __device__ __forceinline__ int pair_rows(const KArgs<T>& a, int p, int& b0, int&
 
 template <typename T, int N, int R>
 __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
-  using PL = Plan<T, N, R>;
-  constexpr int TL = PL::TL;
+  using FC = FftCfg<T, N, R>;
+  constexpr int TL = N / R;
   constexpr int TPB = TL >= 64 ? 1 : 64 / TL;
-  __shared__ cpx<T> lds_all[TPB * lds_team_elems<N>()];
+  __shared__ cpx<T> lds_all[FC::LDS_TEAM > 0 ? TPB * FC::LDS_TEAM : 1];
   const int team = threadIdx.x / TL, j = threadIdx.x % TL;
   const int p = blockIdx.x * TPB + team;
   if (p >= a.npairs) return;
-  cpx<T>* lds = lds_all + team * lds_team_elems<N>();
+  XBuf<T> lds{lds_all + team * FC::LDS_TEAM, 0};
   int b0, b1;
   const int tab = pair_rows(a, p, b0, b1);
-  cpx<T> tw[PL::NTW];
-  load_twiddles<T, N, R>(a.twiddle, j, tw);
+  cpx<T> tw[FC::NTW];
+  load_reg_twiddles<T, N, R>(a.twiddle, j, tw);
   Coef<T, N, R> co;
   load_coef<T, N, R>(a.table + size_t(tab) * (N / 2 + 1) * kTableStride, j, co);
   const T invn = T(1) / T(N);
@@ -499,18 +655,18 @@ __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
 
 template <typename T, int N, int R>
 __global__ void __launch_bounds__(256) ks_adj_kernel(KArgs<T> a) {
-  using PL = Plan<T, N, R>;
-  constexpr int TL = PL::TL;
+  using FC = FftCfg<T, N, R>;
+  constexpr int TL = N / R;
   constexpr int TPB = TL >= 64 ? 1 : 64 / TL;
-  __shared__ cpx<T> lds_all[TPB * lds_team_elems<N>()];
+  __shared__ cpx<T> lds_all[FC::LDS_TEAM > 0 ? TPB * FC::LDS_TEAM : 1];
   const int team = threadIdx.x / TL, j = threadIdx.x % TL;
   const int p = blockIdx.x * TPB + team;
   if (p >= a.npairs) return;
-  cpx<T>* lds = lds_all + team * lds_team_elems<N>();
+  XBuf<T> lds{lds_all + team * FC::LDS_TEAM, 0};
   int b0, b1;
   const int tab = pair_rows(a, p, b0, b1);
-  cpx<T> tw[PL::NTW];
-  load_twiddles<T, N, R>(a.twiddle, j, tw);
+  cpx<T> tw[FC::NTW];
+  load_reg_twiddles<T, N, R>(a.twiddle, j, tw);
   Coef<T, N, R> co;
   load_coef<T, N, R>(a.table + size_t(tab) * (N / 2 + 1) * kTableStride, j, co);
   const T invn = T(1) / T(N);
@@ -748,22 +904,36 @@ void* pick_kernel(bool adj) {
 }
 
 // Radix choices: R points per lane, TL = N / R lanes per trajectory pair
-// (at most 256 = four wavefronts).  float64 builds only the default radix.
-int default_radix(int) { return 4; }
+// (at most 256 = four wavefronts).  float32 builds radix 2, 4 and 8, float64
+// radix 2 and 4 (wider radices run out of registers in the adjoint).  The
+// default is radix 2 while the radix-2 teams need at most two wavefronts per
+// SIMD of the device (the latency-bound regime: each step is a chain of
+// dependent transforms and a team's instruction count sets its length),
+// radix 4 beyond that, where throughput counts and radix 4 needs fewer
+// cross-lane exchanges per point.  (Measured on MI355X, ETDRK4 fwd+adj:
+// radix 2 is 1.2x faster at B=256 N=256, 1.4x at B=1024 N=512, 0.9x at
+// B=2048 N=256, 0.9x at B=8192 N=256.)
+int default_radix(int n, int npairs) {
+  static int simd_lanes = [] {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    return cus * 4 * 64;
+  }();
+  if (n / 2 > 256) return 4;
+  return static_cast<long long>(npairs) * (n / 2) <= 2LL * simd_lanes ? 2 : 4;
+}
 
 template <typename T, int N>
 void* pick_r(int r, bool adj) {
-  if constexpr (std::is_same<T, double>::value) {
-    return r == 4 ? pick_kernel<T, N, 4>(adj) : nullptr;
-  } else {
-    if constexpr (N / 4 <= 256)
-      if (r == 4) return pick_kernel<T, N, 4>(adj);
-    if constexpr (N / 8 <= 256 && N >= 8)
-      if (r == 8) return pick_kernel<T, N, 8>(adj);
-    if constexpr (N >= 16)
-      if (r == 16) return pick_kernel<T, N, 16>(adj);
-    return nullptr;
-  }
+  if constexpr (N / 2 <= 256)
+    if (r == 2) return pick_kernel<T, N, 2>(adj);
+  if constexpr (N / 4 <= 256)
+    if (r == 4) return pick_kernel<T, N, 4>(adj);
+  if constexpr (!std::is_same<T, double>::value && N / 8 <= 256)
+    if (r == 8) return pick_kernel<T, N, 8>(adj);
+  return nullptr;
 }
 
 template <typename T>
@@ -793,15 +963,16 @@ int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int
   if (d->batch == 0) return INC_OK;
   INC_REQUIRE(!adj || stages != nullptr, INC_EINVAL, "adjoint needs the saved stages");
   INC_REQUIRE(!adj || g_traj != nullptr, INC_EINVAL, "adjoint needs the output cotangents (g_traj is NULL)");
-  int r = d->radix > 0 ? d->radix : default_radix(d->n);
-  if (std::is_same<T, double>::value) r = default_radix(d->n);
+  const int packed = d->table_idx == nullptr ? 1 : 0;
+  const int npairs = packed ? (d->batch + 1) / 2 : d->batch;
+  int r = d->radix > 0 ? d->radix : default_radix(d->n, npairs);
+  if (std::is_same<T, double>::value && r != 2 && r != 4) r = default_radix(d->n, npairs);
   void* fn = pick_n<T>(d->n, r, adj);
   INC_REQUIRE(fn != nullptr, INC_EUNSUPPORTED, "no KS kernel for n=%d radix=%d", d->n, r);
-  if (d->batch == 0) return INC_OK;
   KArgs<T> a{};
   a.batch = d->batch;
-  a.packed = d->table_idx == nullptr ? 1 : 0;
-  a.npairs = a.packed ? (d->batch + 1) / 2 : d->batch;
+  a.packed = packed;
+  a.npairs = npairs;
   a.steps = steps;
   a.substeps = d->substeps;
   a.scheme = d->scheme;

@@ -171,7 +171,9 @@ typedef struct inc_ks_desc {
   int32_t n;
   int32_t substeps;   /* sub-steps per step() call: 2 = reference double step */
   int32_t dealias;    /* 2/3-rule mask on the nonlinear term (ETDRK4) */
-  int32_t radix;      /* points per lane of the FFT (0 = library default) */
+  int32_t radix;      /* points per lane of the FFT: 2, 4 or 8 (float64: 2 or 4) with n/radix <= 256;
+                         0 = library default (2 while radix-2 teams need at most two wavefronts
+                         per SIMD, else 4) */
   int32_t reserved;
   double dt;
   const void* table;        /* coefficient tables from inc_ks_tables */

@@ -80,28 +80,70 @@ def test_ks_golden_gradients_fp64(cuda, golden, key):
     np.testing.assert_allclose(ss.grad.cpu(), g[f"{key}_gs"], rtol=1e-9, atol=1e-12)
 
 
-RADIX_CASES = [(16, 4), (16, 16), (32, 8), (64, 4), (64, 16), (128, 4), (128, 8), (128, 16), (256, 4), (256, 8),
-               (256, 16), (512, 4), (512, 8), (512, 16), (1024, 4), (1024, 8), (1024, 16)]
+# every float32 (n, radix) kernel: radix 2/4/8 with at most 256 lanes per pair;
+# teams of 128 and 256 lanes span two and four wavefronts
+RADIX_CASES = [(n, r) for r in (2, 4, 8) for n in (16, 32, 64, 128, 256, 512, 1024) if n // r <= 256]
 
 
 @pytest.mark.parametrize("n,radix", RADIX_CASES)
 @pytest.mark.parametrize("scheme", ["ETRK2", "ETDRK4"])
 def test_ks_step_every_fft_shape(cuda, n, radix, scheme):
-    # fp32 kernel for every (n, radix); odd batch exercises an unpaired
-    # trajectory; uniform L keeps the packed-pair path
+    # fp32 kernel for every (n, radix), forward and adjoint; odd batch
+    # exercises an unpaired trajectory; uniform L keeps the packed-pair path
     b, L, dt = 5, 22.0 * n / 64, 0.1
     gen = torch.Generator().manual_seed(n * 31 + radix)
     u = modes(b, n, L, gen, kmax=min(10, n // 4))
     s = 0.1 * torch.randn(b, n, generator=gen, dtype=torch.float64)
+    w = torch.randn(b, n, generator=gen, dtype=torch.float64)
     so = KSOracle(ks_params([L] * b, resolution=n, dt=dt, time_scheme=scheme))
     so.doublestep = False
+    uo = u.clone().requires_grad_(True)
     so.correction_s = s
-    ref = so.step(u, so.domain_size)
+    ref = so.step(uo, so.domain_size)
+    (ref * w).sum().backward()
     sv = gpu_solver([L], n, dt, scheme, dtype=torch.float32, radix=radix)
     sv.doublestep = False
     sv.correction_s = s.float().to(cuda)
-    out = sv.step(u.float().to(cuda), torch.tensor([L], device=cuda))
-    assert rel_l2(out, ref) < 1e-5
+    ug = u.float().to(cuda).requires_grad_(True)
+    out = sv.step(ug, torch.tensor([L], device=cuda))
+    (out * w.float().to(cuda)).sum().backward()
+    assert rel_l2(out, ref.detach()) < 1e-5
+    assert rel_l2(ug.grad, uo.grad) < 1e-4
+
+
+@pytest.mark.parametrize("n,radix", [(64, 2), (128, 4), (256, 2), (256, 4), (512, 2), (512, 4), (1024, 4)])
+def test_ks_fp64_rollout_gradient_every_team_width(cuda, n, radix):
+    # float64 kernels at 1e-10 on 3-step rollouts: teams of 32, 64 (one
+    # wavefront), 128 and 256 lanes (exchanges through LDS); mixed L runs
+    # the unpacked path, uniform L the packed one
+    dt, H = 0.1, 3
+    for Ls in ([22.0 * n / 64] * 3, [22.0 * n / 64, 30.0 * n / 64, 40.0 * n / 64]):
+        b = len(Ls)
+        gen = torch.Generator().manual_seed(n + radix)
+        u = modes(b, n, Ls[0], gen, kmax=min(10, n // 4))
+        s = 0.1 * torch.randn(b, n, generator=gen, dtype=torch.float64)
+        w = torch.randn(b, H, n, generator=gen, dtype=torch.float64)
+        so = KSOracle(ks_params(Ls, resolution=n, dt=dt, time_scheme="ETDRK4"))
+        so.doublestep = False
+        uo = u.clone().requires_grad_(True)
+        sso = s.clone().requires_grad_(True)
+        so.correction_s = sso
+        cur, outs = uo, []
+        for _ in range(H):
+            cur = so.step(cur, so.domain_size)
+            outs.append(cur)
+        ref = torch.stack(outs, 1)
+        (ref * w).sum().backward()
+        sv = gpu_solver(Ls, n, dt, "ETDRK4", radix=radix)
+        sv.doublestep = False
+        ug = u.to(cuda).requires_grad_(True)
+        sg = s.to(cuda).requires_grad_(True)
+        sv.correction_s = sg
+        traj = sv.rollout(ug, H, torch.tensor(Ls, device=cuda))
+        (traj * w.to(cuda)).sum().backward()
+        assert rel_l2(traj, ref.detach()) < 1e-11, Ls
+        assert rel_l2(ug.grad, uo.grad) < 1e-10, Ls
+        assert rel_l2(sg.grad, sso.grad) < 1e-10, Ls
 
 
 @pytest.mark.parametrize("scheme", ["ETD1", "ETRK2", "ETDRK4"])

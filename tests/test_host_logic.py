@@ -48,7 +48,8 @@ def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
     assert json.loads((tmp_path / "t.json").read_text()) == {"k": 8}
     monkeypatch.setattr(autotune, "_mem", None)
     assert autotune.lookup("k") == 8  # reloaded from disk
-    assert autotune.ks_radix_candidates(1024, torch.float32) == [4, 8, 16]
-    assert autotune.ks_radix_candidates(16, torch.float32) == [4, 8, 16]
-    assert autotune.ks_radix_candidates(256, torch.float64) == [4]
+    assert autotune.ks_radix_candidates(1024, torch.float32) == [4, 8]
+    assert autotune.ks_radix_candidates(16, torch.float32) == [2, 4, 8]
+    assert autotune.ks_radix_candidates(256, torch.float64) == [2, 4]
+    assert autotune.ks_radix_candidates(1024, torch.float64) == [4]
     assert 16 not in autotune.burgers_team_candidates(4096)
