@@ -12,8 +12,12 @@ Prints one JSON line.  value = trajectory-steps per second, forward+adjoint
 (B * 1000 per bench step).  roofline: the dominant kernel's algorithmic HBM
 bytes (adjoint: read u^n, read lambda^{n+1}, write lambda^n = 3*4*N bytes
 per trajectory-step; forward: 2*4*N) over its duration measured with HIP
-events on the launch stream, against 8 TB/s.  cpu_baseline: the FP64 oracle
-(PyTorch CPU restatement of the same scheme) on a bounded sample, fwd+adj.
+events on the launch stream, against 8 TB/s, next to a measured 1 GiB
+device copy (measured_copy_GBps).  compute: the same kernel's FP32 flops per
+launch (rocprofv3 SQ_INSTS_VALU_FLOPS_FP32, recorded under profiles/) over
+its duration, against the 157.3 TFLOP/s vector peak.  cpu_baseline: the FP64
+oracle (PyTorch CPU restatement of the same scheme) on a bounded sample,
+fwd+adj.
 """
 import argparse
 import json
@@ -41,16 +45,44 @@ def initial_state(b, n, L, seed, kmax=10):
     return u
 
 
-def read_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
-    path = os.path.join(ROOT, "profiles", "ks_adj_traffic.md")
+VALU_PEAK_TFS = 157.3  # fp32 vector, packed-FP32 rate (the published figure)
+
+
+def read_profile_value(name, key):
+    """A figure recorded in a committed rocprofv3 summary under profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    return read_profile_value("ks_adj_traffic.md", "traffic_bytes_per_launch")
+
+
+def copy_bandwidth(dev, nbytes=1 << 30, reps=5):
+    """Device-to-device copy of nbytes, best of reps: read + write bytes per
+    second (a STREAM-copy measurement of the box's HBM)."""
+    import torch
+    src = torch.empty(nbytes // 4, device=dev)
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(reps):
+        a.record()
+        dst.copy_(src)
+        b.record()
+        torch.cuda.synchronize()
+        best = min(best, a.elapsed_time(b) * 1e-3)
+    del src, dst
+    return 2 * nbytes / best / 1e9
 
 
 def cpu_baseline(budget_s=12.0):
@@ -337,10 +369,17 @@ def main():
                                "2/3 dealias, fp32", "batch": B, "n": N, "horizon": H},
         "roofline": {"bound": "hbm", "kernel": "ks_adj_kernel" if dom_is_adj else "ks_fwd_kernel",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_traffic()},
+                     "traffic": read_traffic(), "measured_copy_GBps": copy_bandwidth(dev)},
         "forward_trajectory_steps_per_s": ts / t_fwd,
         "kernel_ms": {"fwd": t_fwd * 1e3, "adj": t_adj * 1e3},
     }
+    adj_flops = read_profile_value("ks_valu_counters.md", "adj_flops_per_launch")
+    if adj_flops is not None and dom_is_adj:
+        # FP32 flops of the dominant kernel (rocprofv3 SQ_INSTS_VALU_FLOPS_FP32 x 64 lanes, per
+        # launch of this workload) over its live-measured duration, against the vector peak
+        out["compute"] = {"bound": "valu", "kernel": "ks_adj_kernel", "achieved": adj_flops / t_adj / 1e12,
+                          "peak": VALU_PEAK_TFS, "unit": "TFLOP/s", "frac": adj_flops / t_adj / 1e12 / VALU_PEAK_TFS,
+                          "flops_per_launch": adj_flops}
     if not args.no_plain:
         plain = plain_path(solver, u0, Ld)
         out["plain_trajectory_steps_per_s"] = plain

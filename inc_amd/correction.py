@@ -48,7 +48,7 @@ class _CircConv1d(torch.autograd.Function):
         if ctx.needs_input_grad[1] or (ctx.has_bias and ctx.needs_input_grad[2]):
             width = co * ci * k + co
             tiles = bsz * ((n + 127) // 128)
-            work = x.new_empty(min(tiles, 256) * width)
+            work = x.new_empty(min(tiles, 512) * width)
             gw = torch.empty_like(w)
             gb = x.new_empty(co) if ctx.has_bias else None
             rc = _lib.load().inc_conv1d_bwd_weight(code, bsz, ci, co, n, k, _lib.ptr(g), _lib.ptr(x), _lib.ptr(gw),

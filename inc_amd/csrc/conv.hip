@@ -13,6 +13,9 @@
 //   * weight/bias gradient: per-block partial sums over a strided set of
 //     (row, tile) pairs written to a caller workspace, then a second kernel
 //     reduces the partials in a fixed order (deterministic).
+#include <algorithm>
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace inc {
@@ -210,6 +213,177 @@ __global__ void __launch_bounds__(256) reduce_parts_kernel(int nparts, int width
   }
 }
 
+// ---- weight gradient on the matrix cores (float32) --------------------------------
+// G[o][col] = sum_pos g[b,o,pos] * X[col][pos] with col = c*K + k and
+// X[c*K + k][pos] = x[b,c,(pos + k - P) mod N]; col = Ci*K is a column of ones
+// (the bias gradient).  v_mfma_f32_16x16x4f32 takes A = g[16 o][4 pos] and
+// B = X[4 pos][16 col]; lane l supplies A[l%16][l/16] and B[l/16][l%16] and
+// holds D[4*(l/16) + r][l%16], r < 4.  A block of four wavefronts loops over
+// (row, 128-position tile) jobs staged in LDS; wavefront w takes the column
+// tiles w % WT, w % WT + WT, ... and the 4-position steps s with
+// s % WP == w / WT (WT * WP <= 4: thin layers with few column tiles split
+// positions instead).  Every wavefront's accumulators are one partial sum,
+// written to slot blockIdx.x * WP + w / WT of the workspace.
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int kWgTP = 128;  // positions per job
+
+template <int MT, int NT>
+__global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int B, int Ci, int Co, int N, int K, int WT, int WP,
+                                                             const float* __restrict__ g,
+                                                             const float* __restrict__ x, float* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int P = K / 2;
+  constexpr int GS = kWgTP + 1;             // odd row strides: the 16 rows of a fragment
+  const int XS = (kWgTP + K - 1) | 1;       // read hit distinct banks
+  float* gs = reinterpret_cast<float*>(smem_raw);  // [MT*16][GS]
+  float* xs = gs + MT * 16 * GS;                   // [Ci][XS]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int tphase = wave % WT, pphase = wave / WT;
+  const bool active = pphase < WP;
+  const int ncw = Ci * K, ntile = (ncw + 1 + 15) / 16;
+  const int lr = lane & 15, lk = lane >> 4;
+  // this lane's B-fragment source per column tile: offset into xs, or the
+  // ones column (-1), or padding (-2)
+  int off[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int tile = tphase + t * WT, col = tile * 16 + lr;
+    off[t] = tile >= ntile || col > ncw ? -2 : (col == ncw ? -1 : (col / K) * XS + col % K);
+  }
+  v4f acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  const int tiles = (N + kWgTP - 1) / kWgTP;
+  const int xw = kWgTP + K - 1;
+  for (int job = blockIdx.x; job < B * tiles; job += gridDim.x) {
+    const int b = job / tiles, i0 = (job - b * tiles) * kWgTP;
+    const float* gb = g + size_t(b) * Co * N;
+    const float* xb = x + size_t(b) * Ci * N;
+    // Stage the job's tiles with every load of a thread in flight before its
+    // LDS writes (the staging is latency-bound otherwise): all MT * 8
+    // gradient values, then the input rows eight at a time.
+    float gv[MT * 8];
+#pragma unroll
+    for (int q = 0; q < MT * 8; ++q) {
+      const int e = tid + q * 256, o = e / kWgTP, j = e % kWgTP;
+      gv[q] = (o < Co && i0 + j < N) ? gb[size_t(o) * N + i0 + j] : 0.f;
+    }
+    __syncthreads();  // the previous job's fragments are consumed
+#pragma unroll
+    for (int q = 0; q < MT * 8; ++q) {
+      const int e = tid + q * 256;
+      gs[(e / kWgTP) * GS + e % kWgTP] = gv[q];
+    }
+    for (int e0 = tid; e0 < Ci * xw; e0 += 256 * 8) {
+      float xv[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int e = e0 + q * 256;
+        if (e < Ci * xw) {
+          const int c = e / xw, j = e - c * xw;
+          int pos = i0 - P + j;  // in [-P, N + kWgTP + K): one wrap unless N is small
+          pos = pos < 0 ? pos + N : (pos >= N ? pos - N : pos);
+          if (pos >= N) pos %= N;
+          xv[q] = xb[size_t(c) * N + pos];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int e = e0 + q * 256;
+        if (e < Ci * xw) xs[(e / xw) * XS + e % xw] = xv[q];
+      }
+    }
+    __syncthreads();
+    if (active) {
+      // fragments of step s + WP are read while step s's MFMAs run
+      float a[MT], bv[NT], an[MT], bn[NT];
+      auto load = [&](int s, float* fa, float* fb) {
+        const int p = 4 * s + lk;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) fa[m] = gs[(m * 16 + lr) * GS + p];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) fb[t] = off[t] >= 0 ? xs[off[t] + p] : (off[t] == -1 ? 1.f : 0.f);
+      };
+      load(pphase, a, bv);
+      for (int s = pphase; s < kWgTP / 4; s += WP) {
+        if (s + WP < kWgTP / 4) load(s + WP, an, bn);
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int t = 0; t < NT; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], bv[t], acc[m][t], 0, 0, 0);
+#pragma unroll
+        for (int m = 0; m < MT; ++m) a[m] = an[m];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bv[t] = bn[t];
+      }
+    }
+  }
+  if (!active) return;
+  const int width = Co * ncw + Co;
+  float* out = part + size_t(blockIdx.x * WP + pphase) * width;
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int col = (tphase + t * WT) * 16 + lr;
+      if (off[t] == -2) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int o = m * 16 + 4 * lk + r;
+        if (o >= Co) continue;
+        if (col < ncw) out[size_t(o) * ncw + col] = acc[m][t][r];
+        else out[size_t(Co) * ncw + o] = acc[m][t][r];
+      }
+    }
+}
+
+// MFMA weight gradient when the shape fits (Co <= 64, Ci*K + 1 <= 512, LDS);
+// returns INC_EUNSUPPORTED (without touching the error text) otherwise.
+int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const float* x, float* dw, float* db,
+                    float* part, int64_t work_elems, hipStream_t st) {
+  const int MT = (Co + 15) / 16;
+  const int ncw = Ci * K, ntile = (ncw + 1 + 15) / 16;
+  const int WT = ntile >= 4 ? 4 : (ntile >= 2 ? 2 : 1);
+  const int ntw = (ntile + WT - 1) / WT;
+  const int NT = ntw <= 4 ? ntw : (ntw <= 6 ? 6 : 8);
+  const size_t lds = (size_t(MT) * 16 * (kWgTP + 1) + size_t(Ci) * ((kWgTP + K - 1) | 1)) * sizeof(float);
+  if (MT > 4 || ntw > 8 || lds > 150 * 1024) return INC_EUNSUPPORTED;
+  const int width = Co * ncw + Co;
+  const int64_t slots = std::min<int64_t>(512, work_elems / width);  // two blocks per CU
+  if (slots < 1) return INC_EUNSUPPORTED;
+  const int WP = int(std::min<int64_t>(4 / WT, slots));
+  const int jobs = B * ((N + kWgTP - 1) / kWgTP);
+  const int nblk = int(std::min<int64_t>(jobs, slots / WP));
+  dim3 grid(nblk), block(256);
+#define INC_WGM(M, T) \
+  hipLaunchKernelGGL((conv_wgrad_mfma_kernel<M, T>), grid, block, lds, st, B, Ci, Co, N, K, WT, WP, g, x, part)
+#define INC_WGM_T(M)                  \
+  switch (NT) {                       \
+    case 1: INC_WGM(M, 1); break;     \
+    case 2: INC_WGM(M, 2); break;     \
+    case 3: INC_WGM(M, 3); break;     \
+    case 4: INC_WGM(M, 4); break;     \
+    case 6: INC_WGM(M, 6); break;     \
+    default: INC_WGM(M, 8); break;    \
+  }
+  switch (MT) {
+    case 1: INC_WGM_T(1); break;
+    case 2: INC_WGM_T(2); break;
+    case 3: INC_WGM_T(3); break;
+    default: INC_WGM_T(4); break;
+  }
+#undef INC_WGM_T
+#undef INC_WGM
+  int rc = check_launch("conv1d weight gradient (matrix cores)");
+  if (rc) return rc;
+  hipLaunchKernelGGL(reduce_parts_kernel<float>, dim3((width + 15) / 16), dim3(256), 0, st, nblk * WP, width, part,
+                     dw, db, Co * ncw);
+  return check_launch("conv1d weight gradient reduction");
+}
+
 template <typename T, int K, bool FLIP>
 int conv_fwd_k(int cpt, dim3 grid, size_t lds, hipStream_t st, int Ci, int Co, int N, const T* x, const T* w,
                const T* b, T* y) {
@@ -280,6 +454,12 @@ int wgrad_launch_k(int B, int Ci, int Co, int N, int TI, int nblk, size_t lds, c
 template <typename T>
 int conv_wgrad(int B, int Ci, int Co, int N, int K, const void* g, const void* x, void* dw, void* db, void* work,
                int64_t work_elems, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    const int rc = conv_wgrad_mfma(B, Ci, Co, N, K, static_cast<const float*>(g), static_cast<const float*>(x),
+                                   static_cast<float*>(dw), static_cast<float*>(db), static_cast<float*>(work),
+                                   work_elems, st);
+    if (rc != INC_EUNSUPPORTED) return rc;
+  }
   int TI = N < 128 ? N : 128;
   size_t lds = (size_t((Co + 1) & ~1) * (TI | 1) + size_t((Ci + 1) & ~1) * ((TI + K - 1) | 1)) * sizeof(T);
   INC_REQUIRE(lds <= 150 * 1024, INC_EUNSUPPORTED, "conv1d weight gradient needs %zu bytes of LDS", lds);

@@ -223,7 +223,7 @@ int inc_conv1d_bwd_data(int dtype, int B, int Ci, int Co, int n, int K, const vo
 /*
  * Weight and bias gradients gw [Co, Ci, K], gb [Co] (or NULL), summed over
  * the batch in a fixed order; work: caller scratch of work_elems elements
- * (>= Co*Ci*K + Co; up to 256 times that is used when available).
+ * (>= Co*Ci*K + Co; up to 512 times that is used when available).
  */
 int inc_conv1d_bwd_weight(int dtype, int B, int Ci, int Co, int n, int K, const void* g, const void* x,
                           void* gw, void* gb, void* work, int64_t work_elems, void* stream);

@@ -17,8 +17,15 @@ def ref_conv(x, w, b):
     return F.conv1d(F.pad(x, (p, p), mode="circular"), w, b)
 
 
+# (batch, in channels, out channels, n, kernel size).  float32 weight
+# gradients run on the matrix cores when Co <= 64 and Ci*K + 1 <= 512: one
+# output-channel tile (Co = 1), thin layers whose waves split positions
+# (Ci = 1), one workspace slot (B * tiles = 1, idle waves), a partial last
+# position tile (n = 130); (1, 64, 64, 256, 9) exceeds the column limit and
+# takes the vector-ALU kernel.
 SHAPES = [(3, 1, 32, 128, 5), (3, 32, 32, 128, 5), (2, 32, 1, 128, 5), (2, 48, 48, 100, 7), (5, 16, 8, 64, 3),
-          (1, 8, 8, 9, 9), (4, 32, 32, 1024, 5)]
+          (1, 8, 8, 9, 9), (4, 32, 32, 1024, 5), (2, 48, 1, 512, 7), (3, 1, 48, 130, 7), (1, 1, 32, 60, 5),
+          (1, 64, 64, 256, 9), (2, 48, 48, 512, 7)]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
