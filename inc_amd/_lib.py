@@ -20,6 +20,7 @@ INC_SRC_NONE, INC_SRC_LINEAR, INC_SRC_LEARNING = 0, 1, 2
 INC_PART_CONVECTION, INC_PART_DIFFUSION, INC_PART_SOURCE = 1, 2, 4
 INC_ETD1, INC_ETRK2, INC_ETDRK4 = 0, 1, 2
 INC_KS_TABLE_STRIDE = 8
+INC_ACT_NONE, INC_ACT_GELU = 0, 1
 
 _vp = ctypes.c_void_p
 
@@ -80,6 +81,8 @@ SIGNATURES = {
     "inc_conv1d_fwd": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp]),
     "inc_conv1d_bwd_data": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp]),
     "inc_conv1d_bwd_weight": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp]),
+    "inc_conv1d_fwd_act": (ctypes.c_int, [ctypes.c_int] * 7 + [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "inc_conv1d_bwd_data_mul": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

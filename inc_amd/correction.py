@@ -5,9 +5,11 @@ kernel ``k`` with GELU in between, He-normal weights and zero biases (the
 reference's correction networks take (u, L) and return [B, N],
 models/models_1d/resnet.py:22-52; this one keeps that call shape).
 
-The convolutions run in libinc_amd (``inc_conv1d_*``: LDS-tiled direct
-convolution with register blocking, deterministic two-phase weight gradient)
-by default; ``backend='torch'`` uses torch.nn.Conv1d instead.
+The convolutions run in libinc_amd by default (``inc_conv1d_*_act``:
+LDS-tiled direct convolution with register blocking; weight gradients on
+the matrix cores in float32, reduced deterministically; the GELU between
+layers applied while each convolution stages its input, so post-activations
+are never stored); ``backend='torch'`` uses torch.nn.Conv1d and nn.GELU.
 """
 import torch
 from torch import nn
@@ -16,18 +18,49 @@ from torch.nn import functional as F
 from . import _lib
 
 
+def _conv_fwd(x, w, b, act=_lib.INC_ACT_NONE):
+    """(act(conv(x) + b), act'(conv(x) + b) or None)."""
+    bsz, ci, n = x.shape
+    co, _, k = w.shape
+    y = x.new_empty(bsz, co, n)
+    dy = x.new_empty(bsz, co, n) if act != _lib.INC_ACT_NONE else None
+    rc = _lib.load().inc_conv1d_fwd_act(_lib.dtype_code(x.dtype), act, bsz, ci, co, n, k, _lib.ptr(x), _lib.ptr(w),
+                                        _lib.ptr(b), _lib.ptr(y), _lib.ptr(dy), _lib.stream())
+    _lib.check(rc, "circular conv1d")
+    return y, dy
+
+
+def _conv_bwd_data(g, w, ci, m=None):
+    """(conv^T g) * m, the gradient w.r.t. the layer input (m: the derivative
+    of the activation that formed that input, or None)."""
+    bsz, co, n = g.shape
+    k = w.shape[-1]
+    gx = g.new_empty(bsz, ci, n)
+    rc = _lib.load().inc_conv1d_bwd_data_mul(_lib.dtype_code(g.dtype), bsz, ci, co, n, k, _lib.ptr(g), _lib.ptr(w),
+                                             _lib.ptr(m), _lib.ptr(gx), _lib.stream())
+    _lib.check(rc, "circular conv1d input gradient")
+    return gx
+
+
+def _conv_bwd_weight(g, x, w, has_bias):
+    bsz, ci, n = x.shape
+    co, _, k = w.shape
+    width = co * ci * k + co
+    work = x.new_empty(min(bsz * ((n + 127) // 128), 512) * width)
+    gw = torch.empty_like(w)
+    gb = x.new_empty(co) if has_bias else None
+    rc = _lib.load().inc_conv1d_bwd_weight(_lib.dtype_code(x.dtype), bsz, ci, co, n, k, _lib.ptr(g), _lib.ptr(x),
+                                           _lib.ptr(gw), _lib.ptr(gb), _lib.ptr(work), work.numel(), _lib.stream())
+    _lib.check(rc, "circular conv1d weight gradient")
+    return gw, gb
+
+
 class _CircConv1d(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, w, b):
-        x = x.contiguous()
-        w = w.contiguous()
-        bsz, ci, n = x.shape
-        co, _, k = w.shape
-        y = x.new_empty(bsz, co, n)
-        bias = b.contiguous() if b is not None else None
-        rc = _lib.load().inc_conv1d_fwd(_lib.dtype_code(x.dtype), bsz, ci, co, n, k, _lib.ptr(x), _lib.ptr(w),
-                                        _lib.ptr(bias), _lib.ptr(y), _lib.stream())
-        _lib.check(rc, "circular conv1d")
+        x, w = x.contiguous(), w.contiguous()
+        b = b.contiguous() if b is not None else None
+        y, _ = _conv_fwd(x, w, b)
         ctx.save_for_backward(x, w)
         ctx.has_bias = b is not None
         return y
@@ -36,24 +69,11 @@ class _CircConv1d(torch.autograd.Function):
     def backward(ctx, g):
         x, w = ctx.saved_tensors
         g = g.contiguous()
-        bsz, ci, n = x.shape
-        co, _, k = w.shape
-        code = _lib.dtype_code(x.dtype)
         gx = gw = gb = None
         if ctx.needs_input_grad[0]:
-            gx = torch.empty_like(x)
-            rc = _lib.load().inc_conv1d_bwd_data(code, bsz, ci, co, n, k, _lib.ptr(g), _lib.ptr(w), _lib.ptr(gx),
-                                                 _lib.stream())
-            _lib.check(rc, "circular conv1d input gradient")
+            gx = _conv_bwd_data(g, w, x.shape[1])
         if ctx.needs_input_grad[1] or (ctx.has_bias and ctx.needs_input_grad[2]):
-            width = co * ci * k + co
-            tiles = bsz * ((n + 127) // 128)
-            work = x.new_empty(min(tiles, 512) * width)
-            gw = torch.empty_like(w)
-            gb = x.new_empty(co) if ctx.has_bias else None
-            rc = _lib.load().inc_conv1d_bwd_weight(code, bsz, ci, co, n, k, _lib.ptr(g), _lib.ptr(x), _lib.ptr(gw),
-                                                   _lib.ptr(gb), _lib.ptr(work), work.numel(), _lib.stream())
-            _lib.check(rc, "circular conv1d weight gradient")
+            gw, gb = _conv_bwd_weight(g, x, w, ctx.has_bias)
         return gx, gw, gb
 
 
@@ -62,6 +82,46 @@ def circular_conv1d(x, weight, bias=None):
     (torch.nn.Conv1d(padding=K//2, padding_mode='circular') semantics)."""
     _lib.require_gpu(x, weight, bias)
     return _CircConv1d.apply(x, weight, bias)
+
+
+class _CNN(torch.autograd.Function):
+    """The whole stack conv -> GELU -> ... -> conv as one autograd node.  Each
+    hidden layer's kernel writes gelu(v) (the next layer's input) and gelu'(v);
+    backward runs per layer a weight gradient on the saved input and one input
+    gradient that multiplies by the saved gelu' in its epilogue."""
+
+    @staticmethod
+    def forward(ctx, u, *params):
+        ws, bs = params[0::2], params[1::2]
+        depth = len(ws)
+        x = u.unsqueeze(1).contiguous()
+        inputs, dys = [x], []
+        for i in range(depth):
+            act = _lib.INC_ACT_GELU if i < depth - 1 else _lib.INC_ACT_NONE
+            x, dy = _conv_fwd(x, ws[i].contiguous(), bs[i].contiguous(), act)
+            if i < depth - 1:
+                inputs.append(x)
+                dys.append(dy)
+        ctx.depth = depth
+        ctx.save_for_backward(*inputs, *dys, *ws)
+        return x.squeeze(1)
+
+    @staticmethod
+    def backward(ctx, gs):
+        depth = ctx.depth
+        saved = ctx.saved_tensors
+        inputs, dys, ws = saved[:depth], saved[depth:2 * depth - 1], saved[2 * depth - 1:]
+        g = gs.unsqueeze(1).contiguous()
+        grads = [None] * (2 * depth)
+        gu = None
+        for i in reversed(range(depth)):
+            if ctx.needs_input_grad[1 + 2 * i] or ctx.needs_input_grad[2 + 2 * i]:
+                grads[2 * i], grads[2 * i + 1] = _conv_bwd_weight(g, inputs[i], ws[i], True)
+            if i > 0:
+                g = _conv_bwd_data(g, ws[i].contiguous(), inputs[i].shape[1], dys[i - 1])
+            elif ctx.needs_input_grad[0]:
+                gu = _conv_bwd_data(g, ws[0].contiguous(), 1).squeeze(1)
+        return (gu, *grads)
 
 
 class CircularConv1d(nn.Module):
@@ -95,6 +155,10 @@ class CorrectionCNN(nn.Module):
         self.act = nn.GELU()
 
     def forward(self, u, L=None):
+        if self.convs[0].backend == "hip":
+            params = [t for conv in self.convs for t in (conv.weight, conv.bias)]
+            _lib.require_gpu(u, *params)
+            return _CNN.apply(u, *params)
         x = u.unsqueeze(1)
         for i, conv in enumerate(self.convs):
             x = conv(x)

@@ -29,6 +29,16 @@ __device__ __forceinline__ int wrapn(int j, int n) {
   return j < 0 ? j + n : j;
 }
 
+// Exact (erf) GELU, torch.nn.GELU(approximate='none'), and its derivative.
+template <typename T>
+__device__ __forceinline__ T gelu(T v) {
+  return T(0.5) * v * (T(1) + erf(v * T(0.70710678118654752440)));
+}
+template <typename T>
+__device__ __forceinline__ T gelu_grad(T v) {
+  return T(0.5) * (T(1) + erf(v * T(0.70710678118654752440))) + v * T(0.39894228040143267794) * exp(T(-0.5) * v * v);
+}
+
 // One block: TI = 64 * kPPT positions of one row, kWaves * CPT output
 // channels.  The input tile (with its periodic halo) is staged in LDS and
 // shared by the block's wavefronts; every wavefront owns CPT output channels
@@ -36,10 +46,16 @@ __device__ __forceinline__ int wrapn(int j, int n) {
 // cache and each lane accumulates CPT x kPPT outputs in registers.
 // FLIP: read W as the input-gradient operator W'[c_out', c_in', k] =
 // W[c_in', c_out', K-1-k] (channels swapped, taps reversed).
+// Epilogue, per output v = acc + bias: times mul[b,o,i] when mul is given
+// (the input gradient of a layer whose input was an activation: mul holds
+// the activation's derivative); with act = GELU, y = gelu(v) and
+// dy = gelu'(v) (the next layer's input and the derivative its input
+// gradient needs).
 template <typename T, int K, int CPT, bool FLIP>
 __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int N, const T* __restrict__ x,
                                                       const T* __restrict__ w, const T* __restrict__ bias,
-                                                      T* __restrict__ y) {
+                                                      T* __restrict__ y, const T* __restrict__ mul, int act,
+                                                      T* __restrict__ dy) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int P = K / 2;
   constexpr int TI = kWave * kPPT;
@@ -84,11 +100,19 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int N, co
     const int o = o0 + a;
     if (o >= Co) continue;
     const T bo = bias ? bias[o] : T(0);
-    T* yr = y + (size_t(b) * Co + o) * N;
+    const size_t row = (size_t(b) * Co + o) * N;
 #pragma unroll
     for (int p = 0; p < kPPT; ++p) {
       const int i = i0 + p0 + p;
-      if (i < N) yr[i] = acc[a][p] + bo;
+      if (i < N) {
+        T v = acc[a][p] + bo;
+        if (mul) v *= mul[row + i];
+        if (act == INC_ACT_GELU) {
+          dy[row + i] = gelu_grad(v);
+          v = gelu(v);
+        }
+        y[row + i] = v;
+      }
     }
   }
 }
@@ -386,9 +410,10 @@ int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const f
 
 template <typename T, int K, bool FLIP>
 int conv_fwd_k(int cpt, dim3 grid, size_t lds, hipStream_t st, int Ci, int Co, int N, const T* x, const T* w,
-               const T* b, T* y) {
+               const T* b, T* y, const T* mul, int act, T* dy) {
   dim3 block(kWaves * kWave);
-#define INC_CF(C) hipLaunchKernelGGL((conv_fwd_kernel<T, K, C, FLIP>), grid, block, lds, st, Ci, Co, N, x, w, b, y)
+#define INC_CF(C) \
+  hipLaunchKernelGGL((conv_fwd_kernel<T, K, C, FLIP>), grid, block, lds, st, Ci, Co, N, x, w, b, y, mul, act, dy)
   switch (cpt) {
     case 1: INC_CF(1); break;
     case 2: INC_CF(2); break;
@@ -412,7 +437,7 @@ int conv_cpt(int Co) {
 
 template <typename T, bool FLIP>
 int conv_fwd_launch(int B, int Ci, int Co, int N, int K, const void* x, const void* w, const void* bias, void* y,
-                    hipStream_t st) {
+                    const void* mul, int act, void* dy, hipStream_t st) {
   const int TI = kWave * kPPT;
   size_t lds = size_t(Ci) * (TI + K - 1) * sizeof(T);
   INC_REQUIRE(lds <= 150 * 1024, INC_EUNSUPPORTED, "conv1d: %d input channels need %zu bytes of LDS", Ci, lds);
@@ -421,13 +446,15 @@ int conv_fwd_launch(int B, int Ci, int Co, int N, int K, const void* x, const vo
   const T* xp = static_cast<const T*>(x);
   const T* wp = static_cast<const T*>(w);
   const T* bp = static_cast<const T*>(bias);
+  const T* mp = static_cast<const T*>(mul);
   T* yp = static_cast<T*>(y);
+  T* dp = static_cast<T*>(dy);
   switch (K) {
-    case 1: return conv_fwd_k<T, 1, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp);
-    case 3: return conv_fwd_k<T, 3, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp);
-    case 5: return conv_fwd_k<T, 5, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp);
-    case 7: return conv_fwd_k<T, 7, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp);
-    case 9: return conv_fwd_k<T, 9, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp);
+    case 1: return conv_fwd_k<T, 1, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp);
+    case 3: return conv_fwd_k<T, 3, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp);
+    case 5: return conv_fwd_k<T, 5, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp);
+    case 7: return conv_fwd_k<T, 7, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp);
+    case 9: return conv_fwd_k<T, 9, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp);
     default: set_error("conv1d kernel size must be 1, 3, 5, 7 or 9 (got %d)", K); return INC_EUNSUPPORTED;
   }
 }
@@ -438,7 +465,8 @@ int wgrad_launch_k(int B, int Ci, int Co, int N, int TI, int nblk, size_t lds, c
   const int nblk2 = ((Co + 1) / 2) * ((Ci + 1) / 2);
   const int maxp = (nblk2 + 255) / 256;
   dim3 grid(nblk), block(256);
-#define INC_WG(MP) hipLaunchKernelGGL((conv_wgrad_kernel<T, K, MP>), grid, block, lds, st, B, Ci, Co, N, TI, g, x, part)
+#define INC_WG(MP) \
+  hipLaunchKernelGGL((conv_wgrad_kernel<T, K, MP>), grid, block, lds, st, B, Ci, Co, N, TI, g, x, part)
   if (maxp <= 1) INC_WG(1);
   else if (maxp <= 2) INC_WG(2);
   else if (maxp <= 3) INC_WG(3);
@@ -486,8 +514,9 @@ int conv_wgrad(int B, int Ci, int Co, int N, int K, const void* g, const void* x
   return check_launch("conv1d weight gradient reduction");
 }
 
-int conv_check(int dtype, int B, int Ci, int Co, int N, int K) {
+int conv_check(int dtype, int act, int B, int Ci, int Co, int N, int K) {
   INC_REQUIRE(dtype == INC_F32 || dtype == INC_F64, INC_EINVAL, "unsupported dtype %d", dtype);
+  INC_REQUIRE(act == INC_ACT_NONE || act == INC_ACT_GELU, INC_EINVAL, "unknown activation %d", act);
   INC_REQUIRE(B >= 0 && Ci >= 1 && Co >= 1 && N >= 1, INC_EINVAL, "conv1d: bad sizes");
   INC_REQUIRE(K % 2 == 1 && K <= 9 && K <= N, INC_EUNSUPPORTED, "conv1d kernel size must be odd, <= 9 and <= n");
   return INC_OK;
@@ -498,31 +527,45 @@ int conv_check(int dtype, int B, int Ci, int Co, int N, int K) {
 
 extern "C" {
 
-int inc_conv1d_fwd(int dtype, int B, int Ci, int Co, int N, int K, const void* x, const void* w, const void* bias,
-                   void* y, void* stream) {
+int inc_conv1d_fwd_act(int dtype, int act, int B, int Ci, int Co, int N, int K, const void* x, const void* w,
+                       const void* bias, void* y, void* dy, void* stream) {
   inc::clear_error();
-  int rc = inc::conv_check(dtype, B, Ci, Co, N, K);
+  int rc = inc::conv_check(dtype, act, B, Ci, Co, N, K);
+  if (rc || B == 0) return rc;
+  INC_REQUIRE(act == INC_ACT_NONE || dy != nullptr, INC_EINVAL, "conv1d with an activation needs dy");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return dtype == INC_F32
+             ? inc::conv_fwd_launch<float, false>(B, Ci, Co, N, K, x, w, bias, y, nullptr, act, dy, st)
+             : inc::conv_fwd_launch<double, false>(B, Ci, Co, N, K, x, w, bias, y, nullptr, act, dy, st);
+}
+
+int inc_conv1d_bwd_data_mul(int dtype, int B, int Ci, int Co, int N, int K, const void* g, const void* w,
+                            const void* m, void* gx, void* stream) {
+  inc::clear_error();
+  int rc = inc::conv_check(dtype, INC_ACT_NONE, B, Ci, Co, N, K);
   if (rc || B == 0) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  return dtype == INC_F32 ? inc::conv_fwd_launch<float, false>(B, Ci, Co, N, K, x, w, bias, y, st)
-                          : inc::conv_fwd_launch<double, false>(B, Ci, Co, N, K, x, w, bias, y, st);
+  // gx = conv(g, W') with W'[c][o][k] = W[o][c][K-1-k]: a forward pass with Co -> Ci channels,
+  // times m elementwise when given
+  return dtype == INC_F32
+             ? inc::conv_fwd_launch<float, true>(B, Co, Ci, N, K, g, w, nullptr, gx, m, INC_ACT_NONE, nullptr, st)
+             : inc::conv_fwd_launch<double, true>(B, Co, Ci, N, K, g, w, nullptr, gx, m, INC_ACT_NONE, nullptr, st);
+}
+
+int inc_conv1d_fwd(int dtype, int B, int Ci, int Co, int N, int K, const void* x, const void* w, const void* bias,
+                   void* y, void* stream) {
+  return inc_conv1d_fwd_act(dtype, INC_ACT_NONE, B, Ci, Co, N, K, x, w, bias, y, nullptr, stream);
 }
 
 int inc_conv1d_bwd_data(int dtype, int B, int Ci, int Co, int N, int K, const void* g, const void* w, void* gx,
                         void* stream) {
-  inc::clear_error();
-  int rc = inc::conv_check(dtype, B, Ci, Co, N, K);
-  if (rc || B == 0) return rc;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  // gx = conv(g, W') with W'[c][o][k] = W[o][c][K-1-k]: a forward pass with Co -> Ci channels
-  return dtype == INC_F32 ? inc::conv_fwd_launch<float, true>(B, Co, Ci, N, K, g, w, nullptr, gx, st)
-                          : inc::conv_fwd_launch<double, true>(B, Co, Ci, N, K, g, w, nullptr, gx, st);
+  return inc_conv1d_bwd_data_mul(dtype, B, Ci, Co, N, K, g, w, nullptr, gx, stream);
 }
 
 int inc_conv1d_bwd_weight(int dtype, int B, int Ci, int Co, int N, int K, const void* g, const void* x, void* gw,
                           void* gb, void* work, int64_t work_elems, void* stream) {
   inc::clear_error();
-  int rc = inc::conv_check(dtype, B, Ci, Co, N, K);
+  int rc = inc::conv_check(dtype, INC_ACT_NONE, B, Ci, Co, N, K);
   if (rc) return rc;
   INC_REQUIRE(B > 0, INC_EINVAL, "conv1d weight gradient of an empty batch");
   hipStream_t st = static_cast<hipStream_t>(stream);

@@ -228,6 +228,22 @@ int inc_conv1d_bwd_data(int dtype, int B, int Ci, int Co, int n, int K, const vo
 int inc_conv1d_bwd_weight(int dtype, int B, int Ci, int Co, int n, int K, const void* g, const void* x,
                           void* gw, void* gb, void* work, int64_t work_elems, void* stream);
 
+/*
+ * Activation fused into the convolution that produces it (the correction
+ * networks put GELU between their convolutions, models/models_1d/resnet.py:36-52):
+ * with v = conv(x) + bias, y = act(v) and dy = act'(v) [B, Co, n].  y feeds the
+ * next layer (and its weight gradient); dy is what the input gradient of that
+ * next layer multiplies by (inc_conv1d_bwd_data_mul).  act: INC_ACT_NONE (dy
+ * unused, the plain inc_conv1d_fwd) or INC_ACT_GELU (exact erf form, as
+ * torch.nn.GELU()).
+ */
+enum { INC_ACT_NONE = 0, INC_ACT_GELU = 1 };
+int inc_conv1d_fwd_act(int dtype, int act, int B, int Ci, int Co, int n, int K, const void* x,
+                       const void* w, const void* bias, void* y, void* dy, void* stream);
+/* gx = (conv^T g) * m elementwise, m [B, Ci, n] (NULL: the plain inc_conv1d_bwd_data). */
+int inc_conv1d_bwd_data_mul(int dtype, int B, int Ci, int Co, int n, int K, const void* g,
+                            const void* w, const void* m, void* gx, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,11 @@ def test_argument_errors_map_to_codes_without_launch():
     r = L.RaggedDesc(dtype=L.INC_F32, count=1, total=4, nmin=4, nmax=4)
     rc = lib.inc_burgers_ragged_fwd(ctypes.byref(r), None, None, None, 1, None, None, None)
     assert rc == L.INC_EINVAL and b"8 points" in lib.inc_last_error()
+    # convolution with an unknown activation, or a GELU without its derivative output
+    rc = lib.inc_conv1d_fwd_act(L.INC_F32, 7, 1, 1, 1, 16, 3, None, None, None, None, None, None)
+    assert rc == L.INC_EINVAL and b"activation" in lib.inc_last_error()
+    rc = lib.inc_conv1d_fwd_act(L.INC_F32, L.INC_ACT_GELU, 1, 1, 1, 16, 3, None, None, None, None, None, None)
+    assert rc == L.INC_EINVAL and b"needs dy" in lib.inc_last_error()
     # successful-path state is reset by the next call
     assert lib.inc_weno_fwd(L.INC_F32, 0, None, None, 0, 16, None) == L.INC_OK
     assert lib.inc_last_error() == b""
