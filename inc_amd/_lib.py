@@ -71,6 +71,10 @@ SIGNATURES = {
     "inc_burgers_rhs_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp]),
     "inc_burgers_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "inc_burgers_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "inc_burgers_adaptive_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_double, ctypes.c_int, _vp, _vp,
+                                                _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "inc_burgers_adaptive_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                                _vp, ctypes.c_int, _vp]),
     "inc_burgers_ragged_fwd": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp,
                                               _vp]),
     "inc_burgers_ragged_adj": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),

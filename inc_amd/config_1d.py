@@ -55,7 +55,8 @@ class BGSimParams:
 
     def __init__(self, dt=0.01, resolution=128, L=2 * math.pi, nu=0.01, equation_type="homogeneous",
                  batch_size=1, adaptive_CFL=None, convection_term="WENO", device="cuda", beta=0.0,
-                 time_scheme="Euler", A=None, omega=None, phi=None, l=None):
+                 time_scheme="Euler", A=None, omega=None, phi=None, l=None, adaptive_mode="batch",
+                 max_substeps=64):
         self.dt = dt
         self.resolution = resolution
         self.L = L
@@ -63,6 +64,12 @@ class BGSimParams:
         self.equation_type = equation_type
         self.batch_size = batch_size
         self.adaptive_CFL = adaptive_CFL
+        # 'batch': the reference's sub-stepping (one dt for the whole batch from
+        # max|u| over it, read back every sub-step); 'trajectory': each
+        # trajectory sub-steps by its own max|u| on the GPU, at most
+        # max_substeps per step, without host synchronisation
+        self.adaptive_mode = adaptive_mode
+        self.max_substeps = max_substeps
         self.convection_term = convection_term
         self.device = device
         self.beta = beta

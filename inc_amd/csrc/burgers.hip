@@ -184,6 +184,12 @@ struct BArgs {
   T* stages;
   const T* g_traj;
   T* g_s;
+  // adaptive sub-stepping (cfl > 0, holding CFL * dx): per step and
+  // trajectory, nsub sub-steps of sizes dts, stages indexed [b][step][sub][stage]
+  T cfl;
+  int max_sub;
+  int* nsub;
+  T* dts;
 };
 
 constexpr int kBlock = 256;
@@ -477,31 +483,79 @@ __global__ void __launch_bounds__(kBlock) burgers_fwd_kernel(BArgs<T> a) {
     sv[p] = (ok && a.s) ? a.s[row + i] : T(0);
     xv[p] = (ok && a.source == INC_SRC_LEARNING) ? a.x[i] : T(0);
   }
-  for (int step = 0; step < a.steps; ++step) {
-    const T t = t0 + T(step) * a.dt;
+  // One (sub-)step of size h from u0 with the forcing clock at ts0; stage
+  // states go to sdst when given.  on == false: the team takes part in the
+  // block's barriers only and its state is left as it is.
+  auto substep = [&](T h, T ts0, T* sdst, bool on) {
 #pragma unroll
     for (int p = 0; p < PPT; ++p) uc[p] = u0[p];
     for (int st = 0; st < ns; ++st) {
-      if (a.stages && own) {
-        T* dst = a.stages + ((size_t(c.b) * a.steps + step) * ns + st) * n;
+      if (sdst && on && own) {
+        T* dst = sdst + size_t(st) * n;
 #pragma unroll
         for (int p = 0; p < PPT; ++p)
           if (c.i0 + p < n) dst[c.i0 + p] = uc[p];
       }
-      const T ts = st == 0 ? t : (st == 1 ? t + a.dt : t + T(0.5) * a.dt);
+      const T ts = st == 0 ? ts0 : (st == 1 ? ts0 + h : ts0 + T(0.5) * h);
       rhs_eval<T, PPT, CONV>(a, c, uc, a.s ? sv : nullptr, xv, ts, nu, a.parts, us, Fs, red_v,
                              red_i, R);
       __syncthreads();
+      if (on) {
 #pragma unroll
-      for (int p = 0; p < PPT; ++p) {
-        T v = uc[p] + a.dt * R[p];  // forward-Euler stage u + dt*R(u)
-        if (ns == 1 || st == 0) uc[p] = v;
-        else if (st == 1) uc[p] = T(0.75) * u0[p] + T(0.25) * v;
-        else uc[p] = T(1.0 / 3.0) * u0[p] + T(2.0 / 3.0) * v;
+        for (int p = 0; p < PPT; ++p) {
+          T v = uc[p] + h * R[p];  // forward-Euler stage u + h*R(u)
+          if (ns == 1 || st == 0) uc[p] = v;
+          else if (st == 1) uc[p] = T(0.75) * u0[p] + T(0.25) * v;
+          else uc[p] = T(1.0 / 3.0) * u0[p] + T(2.0 / 3.0) * v;
+        }
       }
     }
+    if (on) {
 #pragma unroll
-    for (int p = 0; p < PPT; ++p) u0[p] = uc[p];
+      for (int p = 0; p < PPT; ++p) u0[p] = uc[p];
+    }
+  };
+  for (int step = 0; step < a.steps; ++step) {
+    const T t = t0 + T(step) * a.dt;
+    if (a.cfl <= T(0)) {
+      substep(a.dt, t, a.stages ? a.stages + (size_t(c.b) * a.steps + step) * ns * n : nullptr, true);
+    } else {
+      // Adaptive (solver_1d.py:148-179 with max|u| per trajectory): sub-steps
+      // h = min(remaining, cfl * dx / max|u|), the max_sub-th covering the
+      // rest.  The loop is block-uniform: a team that has covered its dt
+      // keeps joining the barriers until every team of the block has.
+      T remaining = a.dt, tl = t;
+      int j = 0;
+      bool going = c.active;
+      for (;;) {
+        T m;
+        int mi;
+        argmax_begin<T, PPT>(c, u0, m, mi, red_v, red_i);
+        __syncthreads();
+        argmax_end(c, m, mi, red_v, red_i);
+        T h = T(0);
+        if (going) {
+          h = remaining;
+          if (m > T(0)) {
+            const T allowed = a.cfl / m;  // (cfl * dx) / max|u|, the reference's order
+            if (allowed < h) h = allowed;
+          }
+          if (j == a.max_sub - 1) h = remaining;
+        }
+        if (!__syncthreads_or(going)) break;
+        T* sdst = (a.stages && going) ? a.stages + ((size_t(c.b) * a.steps + step) * a.max_sub + j) * ns * n
+                                      : nullptr;
+        substep(h, tl, sdst, going);
+        if (going) {
+          if (c.lt == 0) a.dts[(size_t(c.b) * a.steps + step) * a.max_sub + j] = h;
+          tl += h;
+          remaining -= h;
+          ++j;
+          going = remaining > T(0) && j < a.max_sub;
+        }
+      }
+      if (c.active && c.lt == 0) a.nsub[size_t(c.b) * a.steps + step] = j;
+    }
     if (a.traj && own) {
       T* dst = a.traj + (size_t(c.b) * a.steps + step) * n;
 #pragma unroll
@@ -604,54 +658,78 @@ __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
   gload(a.steps - 1, lam);
 #pragma unroll
   for (int p = 0; p < PPT; ++p) gs[p] = T(0);
-  for (int step = a.steps - 1; step >= 0; --step) {
+  // Transpose of one (sub-)step of size h whose stage states start at src.
+  // on == false: barriers only, the team's cotangents are left as they are.
+  auto substep_vjp = [&](T h, const T* src, bool on) {
     for (int k = 0; k < ns; ++k) {
-      const T* src = a.stages + ((size_t(c.b) * a.steps + step) * ns + k) * n;
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
         int i = c.i0 + p;
-        st[k][p] = (own && i < n) ? src[i] : T(0);
+        st[k][p] = (on && own && i < n) ? src[size_t(k) * n + i] : T(0);
       }
     }
     if (ns == 1) {
 #pragma unroll
-      for (int p = 0; p < PPT; ++p) cb[p] = dt * lam[p];
+      for (int p = 0; p < PPT; ++p) cb[p] = on ? h * lam[p] : T(0);
       rhs_vjp<T, PPT, CONV>(a, c, st[0], cb, nu, parts, us, cs, C, red_v, red_i, r);
       __syncthreads();
+      if (on) {
 #pragma unroll
-      for (int p = 0; p < PPT; ++p) { gs[p] += cb[p]; lam[p] = lam[p] + r[p]; }
+        for (int p = 0; p < PPT; ++p) { gs[p] += cb[p]; lam[p] = lam[p] + r[p]; }
+      }
     } else {
-      // u' = u0/3 + 2/3 (u2 + dt R(u2))
+      // u' = u0/3 + 2/3 (u2 + h R(u2))
 #pragma unroll
-      for (int p = 0; p < PPT; ++p) cb[p] = T(2.0 / 3.0) * dt * lam[p];
+      for (int p = 0; p < PPT; ++p) cb[p] = on ? T(2.0 / 3.0) * h * lam[p] : T(0);
       rhs_vjp<T, PPT, CONV>(a, c, st[2], cb, nu, parts, us, cs, C, red_v, red_i, r);
       __syncthreads();
+      if (on) {
 #pragma unroll
-      for (int p = 0; p < PPT; ++p) {
-        gs[p] += cb[p];
-        b2[p] = T(2.0 / 3.0) * lam[p] + r[p];
-        lam[p] = T(1.0 / 3.0) * lam[p];  // running cotangent of u0
+        for (int p = 0; p < PPT; ++p) {
+          gs[p] += cb[p];
+          b2[p] = T(2.0 / 3.0) * lam[p] + r[p];
+          lam[p] = T(1.0 / 3.0) * lam[p];  // running cotangent of u0
+        }
       }
-      // u2 = 3/4 u0 + 1/4 (u1 + dt R(u1))
+      // u2 = 3/4 u0 + 1/4 (u1 + h R(u1))
 #pragma unroll
-      for (int p = 0; p < PPT; ++p) cb[p] = T(0.25) * dt * b2[p];
+      for (int p = 0; p < PPT; ++p) cb[p] = on ? T(0.25) * h * b2[p] : T(0);
       rhs_vjp<T, PPT, CONV>(a, c, st[1], cb, nu, parts, us, cs, C, red_v, red_i, r);
       __syncthreads();
+      if (on) {
 #pragma unroll
-      for (int p = 0; p < PPT; ++p) {
-        gs[p] += cb[p];
-        b1[p] = T(0.25) * b2[p] + r[p];
-        lam[p] += T(0.75) * b2[p];
+        for (int p = 0; p < PPT; ++p) {
+          gs[p] += cb[p];
+          b1[p] = T(0.25) * b2[p] + r[p];
+          lam[p] += T(0.75) * b2[p];
+        }
       }
-      // u1 = u0 + dt R(u0)
+      // u1 = u0 + h R(u0)
 #pragma unroll
-      for (int p = 0; p < PPT; ++p) cb[p] = dt * b1[p];
+      for (int p = 0; p < PPT; ++p) cb[p] = on ? h * b1[p] : T(0);
       rhs_vjp<T, PPT, CONV>(a, c, st[0], cb, nu, parts, us, cs, C, red_v, red_i, r);
       __syncthreads();
+      if (on) {
 #pragma unroll
-      for (int p = 0; p < PPT; ++p) {
-        gs[p] += cb[p];
-        lam[p] += b1[p] + r[p];
+        for (int p = 0; p < PPT; ++p) {
+          gs[p] += cb[p];
+          lam[p] += b1[p] + r[p];
+        }
+      }
+    }
+  };
+  for (int step = a.steps - 1; step >= 0; --step) {
+    if (a.cfl <= T(0)) {
+      substep_vjp(dt, a.stages + (size_t(c.b) * a.steps + step) * ns * n, true);
+    } else {
+      // the step's sub-steps in reverse; block-uniform over the sub-step index
+      const size_t sb = size_t(c.b) * a.steps + step;
+      const int nj = c.active ? a.nsub[sb] : 0;
+      for (int jj = a.max_sub - 1; jj >= 0; --jj) {
+        const bool on = jj < nj;
+        if (!__syncthreads_or(on)) continue;
+        const T h = on ? a.dts[sb * a.max_sub + jj] : T(0);
+        substep_vjp(h, on ? a.stages + (sb * a.max_sub + jj) * ns * n : nullptr, on);
       }
     }
     if (step > 0) {
@@ -1115,13 +1193,24 @@ int weno_launch(int direction, const void* f, const void* g, void* out, int64_t 
   return check_launch("weno");
 }
 
+struct Adaptive {
+  double cfl;  // <= 0: fixed steps of dt
+  int max_sub;
+  void* nsub;
+  void* dts;
+};
+
 template <typename T>
 int burgers_call(Kind kind, const inc_burgers_desc* d, int parts, const void* u, const void* s,
                  void* out, int steps, void* traj, void* stages, const void* g_traj, void* g_s,
-                 void* stream) {
+                 void* stream, const Adaptive& ad) {
   BArgs<T> a;
   int rc = fill_args<T>(d, a);
   if (rc) return rc;
+  a.cfl = T(ad.cfl > 0 ? ad.cfl * d->dx : 0);
+  a.max_sub = ad.max_sub;
+  a.nsub = static_cast<int*>(ad.nsub);
+  a.dts = static_cast<T*>(ad.dts);
   a.parts = parts;
   a.steps = steps;
   a.u = static_cast<const T*>(u);
@@ -1135,13 +1224,14 @@ int burgers_call(Kind kind, const inc_burgers_desc* d, int parts, const void* u,
 }
 
 int dispatch(Kind kind, const inc_burgers_desc* d, int parts, const void* u, const void* s, void* out,
-             int steps, void* traj, void* stages, const void* g_traj, void* g_s, void* stream) {
+             int steps, void* traj, void* stages, const void* g_traj, void* g_s, void* stream,
+             const Adaptive& ad = Adaptive{0.0, 0, nullptr, nullptr}) {
   clear_error();
   INC_REQUIRE(d != nullptr, INC_EINVAL, "null descriptor");
   if (d->dtype == INC_F32)
-    return burgers_call<float>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream);
+    return burgers_call<float>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream, ad);
   if (d->dtype == INC_F64)
-    return burgers_call<double>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream);
+    return burgers_call<double>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream, ad);
   set_error("unsupported dtype %d", d->dtype);
   return INC_EINVAL;
 }
@@ -1211,6 +1301,32 @@ int inc_burgers_adj(const inc_burgers_desc* d, const void* stages, const void* g
   return inc::dispatch(Kind::Adj, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE,
                        nullptr, nullptr, g_u, steps, nullptr, const_cast<void*>(stages), g_traj, g_s,
                        stream);
+}
+
+int inc_burgers_adaptive_fwd(const inc_burgers_desc* d, double cfl, int max_sub, const void* u, const void* s,
+                             void* u_out, int steps, void* traj, void* nsub, void* dts, void* stages,
+                             void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(cfl > 0, INC_EINVAL, "adaptive CFL number must be positive (got %g)", cfl);
+  INC_REQUIRE(max_sub >= 1 && max_sub <= 1000, INC_EINVAL, "max_sub must be in [1, 1000] (got %d)", max_sub);
+  INC_REQUIRE(nsub != nullptr && dts != nullptr, INC_EINVAL, "adaptive stepping needs nsub and dts");
+  return inc::dispatch(Kind::Fwd, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE, u, s, u_out,
+                       steps, traj, stages, nullptr, nullptr, stream, inc::Adaptive{cfl, max_sub, nsub, dts});
+}
+
+int inc_burgers_adaptive_adj(const inc_burgers_desc* d, int max_sub, const void* nsub, const void* dts,
+                             const void* stages, const void* g_traj, void* g_u, void* g_s, int steps,
+                             void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(max_sub >= 1 && max_sub <= 1000, INC_EINVAL, "max_sub must be in [1, 1000] (got %d)", max_sub);
+  INC_REQUIRE(nsub != nullptr && dts != nullptr && stages != nullptr && g_traj != nullptr, INC_EINVAL,
+              "the adaptive adjoint needs nsub, dts, stages and g_traj");
+  // cfl only selects the adaptive sweep here: the sub-step sizes are read from dts
+  return inc::dispatch(Kind::Adj, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE, nullptr,
+                       nullptr, g_u, steps, nullptr, const_cast<void*>(stages), g_traj, g_s, stream,
+                       inc::Adaptive{1.0, max_sub, const_cast<void*>(nsub), const_cast<void*>(dts)});
 }
 
 }  // extern "C"

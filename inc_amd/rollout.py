@@ -25,7 +25,7 @@ def _fusable(solver, correction_term):
     if correction_term is not None:
         return False
     if isinstance(solver, BurgersSolverTorch):
-        return solver.adaptive_CFL is None
+        return solver.adaptive_CFL is None or solver.adaptive_mode == "trajectory"
     if isinstance(solver, KSSolverTorch):
         return solver.correction_u is None
     return False

@@ -106,6 +106,57 @@ class _BurgersAdvance(torch.autograd.Function):
         return g_u, g_s, None, None, None, None, None, None
 
 
+class _BurgersAdaptive(torch.autograd.Function):
+    """`steps` steps of dt, each sub-stepped per trajectory to the CFL limit
+    on the GPU (inc_burgers_adaptive_fwd); returns the final state or the
+    [B, steps, N] trajectory.  The sub-step sizes are constants for the
+    gradient, as with the reference's .item()."""
+
+    @staticmethod
+    def forward(ctx, u, s, solver, steps, t, want_traj, need_stages):
+        u = u.contiguous()
+        s = s.contiguous() if s is not None else None
+        b, n = u.shape
+        desc, keep = solver._desc(u, solver.dt, t)
+        ms = solver.max_substeps
+        ns = 3 if solver.time_scheme == "SSPRK3" else 1
+        out = torch.empty_like(u)
+        traj = u.new_empty(b, steps, n) if want_traj else None
+        nsub = torch.empty(b, steps, dtype=torch.int32, device=u.device)
+        dts = u.new_empty(b, steps, ms)
+        stages = u.new_empty(b, steps, ms, ns, n) if need_stages else None
+        rc = L.load().inc_burgers_adaptive_fwd(ctypes.byref(desc), float(solver.adaptive_CFL), ms, L.ptr(u),
+                                               L.ptr(s), L.ptr(out), steps, L.ptr(traj), L.ptr(nsub), L.ptr(dts),
+                                               L.ptr(stages), L.stream())
+        L.check(rc, "BurgersSolverTorch.step (adaptive)")
+        ctx.save_for_backward(stages, nsub, dts)
+        ctx.desc, ctx.keep, ctx.steps, ctx.want_traj, ctx.ms = desc, keep, steps, want_traj, ms
+        ctx.shape = (b, n)
+        solver.last_substeps = nsub  # [B, steps] sub-steps taken (device tensor, read without a sync)
+        return traj if want_traj else out
+
+    @staticmethod
+    def backward(ctx, g):
+        stages, nsub, dts = ctx.saved_tensors
+        if stages is None:
+            raise RuntimeError("Burgers step was run without saved stages; enable grad before the step")
+        b, n = ctx.shape
+        if ctx.want_traj:
+            g_traj = g.contiguous()
+        elif ctx.steps == 1:
+            g_traj = g.contiguous().view(b, 1, n)
+        else:
+            g_traj = g.new_zeros(b, ctx.steps, n)
+            g_traj[:, -1] = g
+        g_u = torch.empty(b, n, dtype=g.dtype, device=g.device)
+        g_s = torch.empty_like(g_u) if ctx.needs_input_grad[1] else None
+        rc = L.load().inc_burgers_adaptive_adj(ctypes.byref(ctx.desc), ctx.ms, L.ptr(nsub), L.ptr(dts),
+                                               L.ptr(stages), L.ptr(g_traj), L.ptr(g_u), L.ptr(g_s), ctx.steps,
+                                               L.stream())
+        L.check(rc, "BurgersSolverTorch.step (adaptive) backward")
+        return g_u, g_s, None, None, None, None, None
+
+
 class _BurgersRHS(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u, solver, parts, t):
@@ -149,6 +200,11 @@ class BurgersSolverTorch:
         self.x = sim_params.x
         self.t = 0.0
         self.adaptive_CFL = sim_params.adaptive_CFL
+        self.adaptive_mode = getattr(sim_params, "adaptive_mode", "batch")
+        self.max_substeps = int(getattr(sim_params, "max_substeps", 64))
+        if self.adaptive_mode not in ("batch", "trajectory"):
+            raise ValueError("adaptive_mode must be 'batch' or 'trajectory'")
+        self.last_substeps = None
         self.correction_s = None
         self.L = sim_params.L
         self.nu = sim_params.nu
@@ -336,9 +392,24 @@ class BurgersSolverTorch:
             self.LOG.info(f"CFL: {self.adaptive_CFL}, using substeps: {n_sub}, time_step: {self.dt}, max_u: {max_u}")
         return u
 
+    def _advance_adaptive(self, u, steps=1, want_traj=False):
+        """Per-trajectory adaptive sub-stepping on the GPU (adaptive_mode
+        'trajectory'): no host synchronisation; sub-step counts land in
+        ``self.last_substeps``."""
+        L.require_gpu(u, self.correction_s)
+        s = self.correction_s
+        if s is not None:
+            s = s.to(dtype=u.dtype)
+        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
+        return _BurgersAdaptive.apply(u, s, self, steps, self.t, want_traj, need)
+
     def step(self, u):
         """Advance by one dt (solver_1d.py:181-195)."""
         if self.adaptive_CFL is not None:
+            if self.adaptive_mode == "trajectory":
+                u_new = self._advance_adaptive(u)
+                self.t += self.dt
+                return u_new
             return self.step_adaptive(u)
         u_new = self._advance(u, self.dt)
         self.t += self.dt
@@ -349,10 +420,14 @@ class BurgersSolverTorch:
     def rollout(self, u, mstep):
         """``mstep`` steps in one launch with the correction held fixed;
         returns the [B, mstep, N] trajectory (the fused path of
-        simulate_rollout)."""
+        simulate_rollout).  With adaptive_CFL this needs adaptive_mode
+        'trajectory' (the batch mode reads max|u| back every sub-step)."""
         if self.adaptive_CFL is not None:
-            raise ValueError("fused rollout needs a fixed time step (adaptive_CFL is set)")
-        traj = self._advance(u, self.dt, steps=mstep, want_traj=True)
+            if self.adaptive_mode != "trajectory":
+                raise ValueError("fused rollout with adaptive_CFL needs adaptive_mode='trajectory'")
+            traj = self._advance_adaptive(u, steps=mstep, want_traj=True)
+        else:
+            traj = self._advance(u, self.dt, steps=mstep, want_traj=True)
         for _ in range(mstep):
             self.t += self.dt
         return traj

@@ -122,6 +122,28 @@ int inc_burgers_adj(const inc_burgers_desc* d, const void* stages,
                     void* stream);
 
 /*
+ * Adaptive sub-stepping per trajectory (BurgersSolverTorch.step_adaptive,
+ * solver_1d.py:148-179, which sub-steps the whole batch to
+ * dt_j = min(remaining, CFL * dx / max|u|) with max|u| read back to the host
+ * every sub-step): here max|u| is each trajectory's own, reduced on the
+ * device, so trajectories sub-step independently and nothing synchronises
+ * with the host.  Each step of dt takes at most max_sub (<= 1000) sub-steps,
+ * the last one covering what remains.  Gradients treat the sub-step sizes as
+ * constants, as the reference's .item() does.
+ *   nsub   [batch, steps] int32       sub-steps taken (output)
+ *   dts    [batch, steps, max_sub]    their sizes (output, d->dtype)
+ *   stages [batch, steps, max_sub, ns, n] stage states for the adjoint, or NULL
+ */
+int inc_burgers_adaptive_fwd(const inc_burgers_desc* d, double cfl, int max_sub,
+                             const void* u, const void* s, void* u_out, int steps,
+                             void* traj, void* nsub, void* dts, void* stages,
+                             void* stream);
+/* Adjoint of inc_burgers_adaptive_fwd from its nsub, dts and stages. */
+int inc_burgers_adaptive_adj(const inc_burgers_desc* d, int max_sub, const void* nsub,
+                             const void* dts, const void* stages, const void* g_traj,
+                             void* g_u, void* g_s, int steps, void* stream);
+
+/*
  * A ragged batch of Burgers trajectories with different resolution,
  * viscosity, time step and horizon, packed end to end: trajectory b occupies
  * [offset[b], offset[b] + n[b]) of every packed [total] array.  Batched form

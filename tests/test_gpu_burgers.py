@@ -227,6 +227,72 @@ def test_burgers_adaptive_cfl_matches_oracle(cuda):
     assert abs(sv.t - oracle.t) < 1e-14
 
 
+@pytest.mark.parametrize("scheme", ["Euler", "SSPRK3"])
+def test_burgers_adaptive_per_trajectory_matches_oracle(cuda, scheme):
+    # adaptive_mode='trajectory': every trajectory sub-steps by its own max|u|
+    # on the GPU.  The oracle run on one trajectory at a time is the
+    # reference's batch-wide sub-stepping restricted to that trajectory, so
+    # it pins both the states and the gradients (sub-step sizes constant, as
+    # with the reference's .item()).  Amplitudes 0.5..3 give different
+    # sub-step counts to trajectories that share a block.
+    n, b, dt, cfl, H = 64, 6, 0.05, 0.3, 3
+    gen = torch.Generator().manual_seed(21)
+    amps = torch.tensor([0.5, 1.0, 1.5, 2.0, 2.5, 3.0], dtype=torch.float64)[:, None]
+    u0 = amps * smooth(b, n, gen)
+    w = torch.randn(b, H, n, generator=gen, dtype=torch.float64)
+    ref_traj, ref_grad = [], []
+    for i in range(b):
+        po = burgers_params(n, dt, nu=0.01, batch_size=1, adaptive_CFL=cfl, time_scheme=scheme)
+        oracle = BurgersOracle(po)
+        ui = u0[i:i + 1].clone().requires_grad_(True)
+        cur, outs = ui, []
+        for _ in range(H):
+            cur = oracle.step(cur)
+            outs.append(cur)
+        tr = torch.stack(outs, 1)
+        (tr * w[i:i + 1]).sum().backward()
+        ref_traj.append(tr.detach())
+        ref_grad.append(ui.grad)
+    ref_traj, ref_grad = torch.cat(ref_traj), torch.cat(ref_grad)
+    p = BGSimParams(dt=dt, resolution=n, nu=0.01, batch_size=b, adaptive_CFL=cfl, adaptive_mode="trajectory",
+                    max_substeps=64, time_scheme=scheme, device="cuda")
+    sv = BurgersSolverTorch(p)
+    u = u0.to(cuda).requires_grad_(True)
+    traj, _ = inc_amd.simulate_rollout(None, sv, u, H, None)
+    (traj * w.to(cuda)).sum().backward()
+    assert rel_l2(traj, ref_traj) < F64_TOL
+    assert rel_l2(u.grad, ref_grad) < 1e-10
+    nsub = sv.last_substeps.cpu()
+    assert nsub.shape == (b, H) and int(nsub.min()) >= 1 and len(set(nsub[:, 0].tolist())) >= 3
+    assert abs(sv.t - H * dt) < 1e-14
+    # the step-by-step path agrees with the fused rollout
+    sv2 = BurgersSolverTorch(p)
+    cur = u0.to(cuda)
+    for k in range(H):
+        cur = sv2.step(cur)
+        assert rel_l2(cur, traj[:, k].detach()) < 1e-14
+
+
+def test_burgers_adaptive_substep_cap(cuda):
+    # max_substeps=2 with a CFL that would need more: each step takes the
+    # CFL-limited sub-step h1 = cfl*dx/max|u| and then one covering dt - h1
+    n, b, dt, cfl = 64, 3, 0.05, 0.05
+    gen = torch.Generator().manual_seed(22)
+    u0 = 2.0 * smooth(b, n, gen)
+    p = BGSimParams(dt=dt, resolution=n, nu=0.01, batch_size=b, adaptive_CFL=cfl, adaptive_mode="trajectory",
+                    max_substeps=2, time_scheme="SSPRK3", device="cuda")
+    sv = BurgersSolverTorch(p)
+    out = sv.step(u0.to(cuda))
+    assert sv.last_substeps.cpu().flatten().tolist() == [2] * b
+    oracle = BurgersOracle(burgers_params(n, dt, nu=0.01, batch_size=1, time_scheme="SSPRK3"))
+    dx = 2 * math.pi / n
+    for i in range(b):
+        ui = u0[i:i + 1]
+        h1 = cfl * dx / float(ui.abs().max())
+        ref = oracle._advance(oracle._advance(ui, h1), dt - h1)
+        assert rel_l2(out[i:i + 1], ref) < F64_TOL
+
+
 def test_burgers_argmax_tie_takes_first_index(cuda):
     # sin on a grid samples |u| = 1 at two points: the gradient through
     # alpha = max|u| goes to the first one, as torch.max(dim) does.

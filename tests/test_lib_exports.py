@@ -64,6 +64,14 @@ def test_argument_errors_map_to_codes_without_launch():
     r = L.RaggedDesc(dtype=L.INC_F32, count=1, total=4, nmin=4, nmax=4)
     rc = lib.inc_burgers_ragged_fwd(ctypes.byref(r), None, None, None, 1, None, None, None)
     assert rc == L.INC_EINVAL and b"8 points" in lib.inc_last_error()
+    # adaptive sub-stepping: CFL number and sub-step cap are checked before any launch
+    d = L.BurgersDesc(dtype=L.INC_F32, batch=1, n=64, dx=0.1, dt=0.01)
+    rc = lib.inc_burgers_adaptive_fwd(ctypes.byref(d), 0.0, 8, None, None, None, 1, None, None, None, None, None)
+    assert rc == L.INC_EINVAL and b"CFL" in lib.inc_last_error()
+    rc = lib.inc_burgers_adaptive_fwd(ctypes.byref(d), 0.5, 0, None, None, None, 1, None, None, None, None, None)
+    assert rc == L.INC_EINVAL and b"max_sub" in lib.inc_last_error()
+    rc = lib.inc_burgers_adaptive_adj(ctypes.byref(d), 8, None, None, None, None, None, None, 1, None)
+    assert rc == L.INC_EINVAL and b"nsub" in lib.inc_last_error()
     # convolution with an unknown activation, or a GELU without its derivative output
     rc = lib.inc_conv1d_fwd_act(L.INC_F32, 7, 1, 1, 1, 16, 3, None, None, None, None, None, None)
     assert rc == L.INC_EINVAL and b"activation" in lib.inc_last_error()
