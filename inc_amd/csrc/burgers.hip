@@ -10,6 +10,8 @@
 // 6-point WENO window; the Lax-Friedrichs alpha = max|u| (and its argmax,
 // for the adjoint) is a wavefront shuffle reduction plus an LDS combine
 // across the team's wavefronts.
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace inc {
@@ -486,7 +488,7 @@ __global__ void __launch_bounds__(kBlock) burgers_fwd_kernel(BArgs<T> a) {
   // One (sub-)step of size h from u0 with the forcing clock at ts0; stage
   // states go to sdst when given.  on == false: the team takes part in the
   // block's barriers only and its state is left as it is.
-  auto substep = [&](T h, T ts0, T* sdst, bool on) {
+  auto substep = [&](T h, T ts0, T* sdst, bool on) __attribute__((always_inline)) {
 #pragma unroll
     for (int p = 0; p < PPT; ++p) uc[p] = u0[p];
     for (int st = 0; st < ns; ++st) {
@@ -660,7 +662,8 @@ __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
   for (int p = 0; p < PPT; ++p) gs[p] = T(0);
   // Transpose of one (sub-)step of size h whose stage states start at src.
   // on == false: barriers only, the team's cotangents are left as they are.
-  auto substep_vjp = [&](T h, const T* src, bool on) {
+  // (always inlined: as a called function its arrays would live in scratch)
+  auto substep_vjp = [&](T h, const T* src, bool on) __attribute__((always_inline)) {
     for (int k = 0; k < ns; ++k) {
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
@@ -1061,10 +1064,33 @@ int ragged_dispatch(const inc_ragged_desc* d, const void* u, const void* s, void
 }
 
 // ---- host side ---------------------------------------------------------------
-int pick_ppt(int n, int team_req, int* team_out) {
-  // Default: 4 points per thread and team = next power of two of n/4; above
-  // 1024 points the team stays at 256 threads and each owns 8 or 16 points.
+int simd_count() {
+  static int simds = [] {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    return 4 * cus;
+  }();
+  return simds;
+}
+
+// Points per thread and team size.  An explicit team is honoured (the
+// smallest PPT that covers n).  The default takes, among PPT <= ppt_max with
+// a team of at most 256 threads, the largest PPT that still gives every SIMD
+// 4 wavefronts, or the smallest PPT when the batch cannot: more points per
+// thread amortise the stencil halo, more wavefronts hide LDS and barrier
+// latency.  ppt_max is 4, and lower for the WENO adjoint, whose exact VJP
+// needs ~170 (PPT 2) and 256 (PPT 4) float32 registers (measured on MI355X,
+// B=8192 N=256: adjoint 15.6 ms at PPT 2 against 26.9 ms at PPT 4).  Above
+// 1024 points the team stays at 256 threads with 8 or 16 points each.
+int pick_ppt(int n, int team_req, int* team_out, int batch, int ppt_max) {
   static const int ppts[] = {1, 2, 4, 8, 16};
+  auto team_for = [&](int ppt) {
+    int need = (n + ppt - 1) / ppt, team = 1;
+    while (team < need) team <<= 1;
+    return team;
+  };
   if (team_req > 0) {
     if (team_req > kBlock || (team_req & (team_req - 1)) != 0) return -1;
     for (int ppt : ppts) {
@@ -1075,16 +1101,18 @@ int pick_ppt(int n, int team_req, int* team_out) {
     }
     return -1;
   }
+  int best = -1;
   for (int ppt : ppts) {
-    if (ppt < 4) continue;
-    int need = (n + ppt - 1) / ppt;
-    if (need > kBlock) continue;
-    int team = 1;
-    while (team < need) team <<= 1;
-    *team_out = team;
-    return ppt;
+    const int team = team_for(ppt);
+    if (team > kBlock) continue;  // needs more points per thread
+    if (best < 0) best = ppt;     // the smallest feasible PPT
+    if (ppt > ppt_max && best != ppt) break;
+    const long long waves = (static_cast<long long>(batch) * team + kWave - 1) / kWave;
+    if (waves >= 4LL * simd_count()) best = ppt;
   }
-  return -1;
+  if (best < 0) return -1;
+  *team_out = team_for(best);
+  return best;
 }
 
 template <typename T>
@@ -1152,7 +1180,9 @@ void* kernel_for_ppt(Kind k, int ppt) {
 template <typename T>
 int launch(Kind kind, const inc_burgers_desc* d, BArgs<T>& a, void* stream) {
   int team = 0;
-  int ppt = pick_ppt(d->n, d->team, &team);
+  const bool weno_vjp = (kind == Kind::Adj || kind == Kind::RhsAdj) && d->convection == INC_WENO;
+  const int ppt_max = weno_vjp ? (std::is_same<T, float>::value ? 2 : 1) : 4;
+  int ppt = pick_ppt(d->n, d->team, &team, d->batch, ppt_max);
   INC_REQUIRE(ppt > 0 && team >= 1 && team <= kBlock && (team & (team - 1)) == 0, INC_EUNSUPPORTED,
               "no launch shape for n=%d team=%d (team must be a power of two <= 256, n <= 16*team)",
               d->n, d->team);
