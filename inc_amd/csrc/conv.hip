@@ -6,10 +6,14 @@
 // replace the library convolutions on that path:
 //   y[b,o,i] = bias[o] + sum_{c,k} W[o,c,k] x[b,c,(i+k-P) mod N],  P = K/2
 // with
-//   * forward: a block covers 128 positions of one row; its input tile (with
-//     the periodic halo) is staged in LDS; each wavefront owns a wave-uniform
-//     set of output channels whose weights come through the scalar cache;
-//   * input gradient: the same kernel on the flipped, transposed weights;
+//   * forward (float32, input channels a multiple of 16, 16..64 output
+//     channels): the matrix cores, weights staged once per block in LDS,
+//     blocks looping over position tiles with the next tile prefetched;
+//   * forward (other shapes, float64): a block covers 128 positions of one
+//     row; its input tile (with the periodic halo) is staged in LDS; each
+//     wavefront owns a wave-uniform set of output channels whose weights
+//     come through the scalar cache;
+//   * input gradient: the same kernels on the flipped, transposed weights;
 //   * weight/bias gradient: per-block partial sums over a strided set of
 //     (row, tile) pairs written to a caller workspace, then a second kernel
 //     reduces the partials in a fixed order (deterministic).
@@ -29,14 +33,14 @@ __device__ __forceinline__ int wrapn(int j, int n) {
   return j < 0 ? j + n : j;
 }
 
-// Exact (erf) GELU, torch.nn.GELU(approximate='none'), and its derivative.
+// Exact (erf) GELU, torch.nn.GELU(approximate='none'), and its derivative
+// from one erf evaluation: g = gelu(v) = v h with h = (1 + erf(v/sqrt 2))/2,
+// returns gelu'(v) = h + v exp(-v^2/2) / sqrt(2 pi).
 template <typename T>
-__device__ __forceinline__ T gelu(T v) {
-  return T(0.5) * v * (T(1) + erf(v * T(0.70710678118654752440)));
-}
-template <typename T>
-__device__ __forceinline__ T gelu_grad(T v) {
-  return T(0.5) * (T(1) + erf(v * T(0.70710678118654752440))) + v * T(0.39894228040143267794) * exp(T(-0.5) * v * v);
+__device__ __forceinline__ T gelu_and_grad(T v, T& g) {
+  const T h = T(0.5) * (T(1) + erf(v * T(0.70710678118654752440)));
+  g = v * h;
+  return h + v * T(0.39894228040143267794) * exp(T(-0.5) * v * v);
 }
 
 // One block: TI = 64 * kPPT positions of one row, kWaves * CPT output
@@ -108,8 +112,9 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int N, co
         T v = acc[a][p] + bo;
         if (mul) v *= mul[row + i];
         if (act == INC_ACT_GELU) {
-          dy[row + i] = gelu_grad(v);
-          v = gelu(v);
+          T gv;
+          dy[row + i] = gelu_and_grad(v, gv);
+          v = gv;
         }
         y[row + i] = v;
       }
@@ -408,6 +413,242 @@ int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const f
   return check_launch("conv1d weight gradient reduction");
 }
 
+// ---- forward / input gradient on the matrix cores (float32) -----------------------
+// D[o][i] = sum_r Wr[o][r] Xr[r][i] with r = k*Ci + c and Xr[k*Ci + c][i] =
+// x[b,c,(i+k-P) mod N].  A block of four wavefronts stages the weights once,
+// as ws[o][r] (o padded to CT*16 rows with zeros), then loops over (row,
+// TI-position tile) jobs, staging each input tile position-major:
+// xs[j][c] = x[b,c,(i0-P+j) mod N], j < TI + K - 1.  Wavefront w owns
+// positions i0 + 16*PW*w .. + 16*PW and all CT*16 output channels.  The
+// reduction runs in chunks of 16 values of r; Ci % 16 == 0 puts a chunk in
+// one tap k (uniform across the wavefront).  In chunk s, lane quarter
+// q = l/16 feeds r = 16s + 4q + j to MFMA j (j < 4), so its A operands are
+// ws[o][16s+4q .. +3] and its B operands xs[i+k][c0+4q .. +3] (c0 = 16s mod
+// Ci): one 16-byte LDS read per fragment.  Row strides are 4 mod 16 words,
+// so the 16 rows of a fragment read cover every bank.  The next chunk's
+// fragments are read while the current chunk's MFMAs run (two register sets
+// used in turn).
+// flip: the input-gradient operator, as in conv_fwd_kernel; the epilogue is
+// conv_fwd_kernel's (bias, mul, GELU with its derivative).
+template <int CT, int PW>
+__global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int Co, int N, int K, int flip,
+                                                            const float* __restrict__ x,
+                                                            const float* __restrict__ w,
+                                                            const float* __restrict__ bias, float* __restrict__ y,
+                                                            const float* __restrict__ mul, int act,
+                                                            float* __restrict__ dy) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int TI = 64 * PW;
+  const int P = K / 2;
+  const int R = Ci * K, RS = R + 4, CS = Ci + 4, XW = TI + K - 1;
+  float* ws = reinterpret_cast<float*>(smem_raw);  // [CT*16][RS]
+  float* xs = ws + CT * 16 * RS;                   // [XW + 1][CS] (row XW spare)
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, q = lane >> 4;
+  // weights: thread t takes the (o, c) pairs t, t + 256, ... and their K
+  // taps, which are contiguous in w (both layouts); two pairs' loads in
+  // flight before the LDS writes (a load-store loop would wait for every
+  // load in turn, and per-element index divisions cost more than the copy)
+  for (int p0 = tid; p0 < CT * 16 * Ci; p0 += 512) {
+    float v[2][9];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int p = p0 + u * 256, o = p / Ci, c = p - o * Ci;
+      const bool ok = p < CT * 16 * Ci && o < Co;
+      const float* src = w + (flip ? size_t(c) * Co + o : size_t(o) * Ci + c) * K;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) v[u][k] = (ok && k < K) ? src[flip ? K - 1 - k : k] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int p = p0 + u * 256, o = p / Ci, c = p - o * Ci;
+      if (p >= CT * 16 * Ci) continue;
+      float* dst = ws + o * RS + c;
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        if (k < K) dst[k * Ci] = v[u][k];
+    }
+  }
+  const int tiles = (N + TI - 1) / TI, nchunk = R / 16, c4n = Ci / 4;
+  const int njobs = B * tiles, items = c4n * XW;
+  const int p0 = wave * 16 * PW;
+  // A tile is c4n * XW items (channel quad c4, row j): item e = tid + 256 u
+  // of this thread is quad e / XW at row e % XW, fetched as four coalesced
+  // loads x[b, 4c4 .. 4c4+3, pos(j)] into registers and staged as one
+  // 16-byte LDS write.  The next job's tile is fetched before this job's
+  // MFMAs and staged after them, so its HBM latency overlaps the matrix work.
+  constexpr int U = 4 * PW + 1;  // items <= 16 * (64 PW + 8) <= 256 U (Ci <= 64, K <= 9)
+  auto fetch = [&](int job, v4f* v) __attribute__((always_inline)) {
+    const int b = job / tiles, i0 = (job - b * tiles) * TI;
+    const float* xb = x + size_t(b) * Ci * N;
+    int c4 = tid / XW, j = tid - c4 * XW;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      // unconditional loads (items past the tile read quad c4n - 1 and are
+      // not staged): a conditional load's value is copied at the branch
+      // join, and that copy would wait for it
+      int pos = i0 - P + j;  // in [-P, N + TI + K): one wrap unless N is small
+      pos = pos < 0 ? pos + N : (pos >= N ? pos - N : pos);
+      if (pos >= N) pos %= N;
+      const float* s = xb + size_t(4 * min(c4, c4n - 1)) * N + pos;
+      v[u] = v4f{s[0], s[N], s[2 * size_t(N)], s[3 * size_t(N)]};
+      j += 256;
+      while (j >= XW) {
+        j -= XW;
+        ++c4;
+      }
+    }
+  };
+  auto stage = [&](const v4f* v) __attribute__((always_inline)) {
+    int c4 = tid / XW, j = tid - c4 * XW;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (tid + 256 * u < items) *reinterpret_cast<v4f*>(xs + j * CS + 4 * c4) = v[u];
+      j += 256;
+      while (j >= XW) {
+        j -= XW;
+        ++c4;
+      }
+    }
+  };
+  v4f nx[U];
+  if (int(blockIdx.x) < njobs) fetch(blockIdx.x, nx);
+  for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
+    const int b = job / tiles, i0 = (job - b * tiles) * TI;
+    __syncthreads();  // the weights are staged and the previous tile consumed
+    stage(nx);
+    __syncthreads();
+    // unconditional for the same reason as the loads (the last job fetches
+    // its own tile again, unused)
+    fetch(min(job + int(gridDim.x), njobs - 1), nx);
+    v4f acc[CT][PW];
+#pragma unroll
+    for (int m = 0; m < CT; ++m)
+#pragma unroll
+      for (int t = 0; t < PW; ++t) acc[m][t] = v4f{0.f, 0.f, 0.f, 0.f};
+    // chunk s: the xs offset of its tap and first channel, k*CS + c0 (wave-uniform)
+    int c0 = 0, xo = 0;
+    auto next = [&]() __attribute__((always_inline)) {
+      c0 += 16;
+      xo += 16;
+      if (c0 == Ci) {
+        c0 = 0;
+        xo += CS - Ci;
+      }
+    };
+    const float* wl = ws + lr * RS + 4 * q;
+    const float* xl = xs + (p0 + lr) * CS + 4 * q;
+    auto load = [&](int s, v4f* fa, v4f* fb) __attribute__((always_inline)) {
+#pragma unroll
+      for (int m = 0; m < CT; ++m) fa[m] = *reinterpret_cast<const v4f*>(wl + m * 16 * RS + 16 * s);
+#pragma unroll
+      for (int t = 0; t < PW; ++t) fb[t] = *reinterpret_cast<const v4f*>(xl + 16 * t * CS + xo);
+    };
+    auto mma = [&](const v4f* fa, const v4f* fb) __attribute__((always_inline)) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int m = 0; m < CT; ++m)
+#pragma unroll
+          for (int t = 0; t < PW; ++t)
+            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[m][j], fb[t][j], acc[m][t], 0, 0, 0);
+    };
+    // The reads are unconditional (a conditional read defeats the compiler's
+    // count of LDS reads in flight, and each MFMA would then wait for the
+    // prefetch): after the last chunk they read the ws row that follows and
+    // the spare xs row XW, and the values are never used.  The scheduling
+    // barriers keep each prefetch ahead of the MFMAs it overlaps (the
+    // scheduler otherwise sinks it to its first use).
+    v4f a0[CT], b0[PW], a1[CT], b1[PW];
+    load(0, a0, b0);
+    for (int s = 0; s < nchunk; s += 2) {
+      next();
+      load(s + 1, a1, b1);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(a0, b0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (s + 1 >= nchunk) break;
+      next();
+      load(s + 2, a0, b0);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(a1, b1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int m = 0; m < CT; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int o = m * 16 + 4 * q + r;
+        if (o >= Co) continue;
+        const float bsv = bias ? bias[o] : 0.f;
+        const size_t row = (size_t(b) * Co + o) * N;
+#pragma unroll
+        for (int t = 0; t < PW; ++t) {
+          const int i = i0 + p0 + 16 * t + lr;
+          if (i >= N) continue;
+          float v = acc[m][t][r] + bsv;
+          if (mul) v *= mul[row + i];
+          if (act == INC_ACT_GELU) {
+            float gv;
+            dy[row + i] = gelu_and_grad(v, gv);
+            v = gv;
+          }
+          y[row + i] = v;
+        }
+      }
+  }
+}
+
+int cu_count() {
+  static int cus = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || n < 1)
+      n = 256;
+    return n;
+  }();
+  return cus;
+}
+
+// Matrix-core forward (or input gradient) when the shape fits: Ci a
+// multiple of 16 up to 64, 16 <= Co <= 64 and the LDS tiles fit; INC_EUNSUPPORTED
+// (without touching the error text) otherwise.  Positions per wavefront:
+// 16*PW with PW = 2 when N > 64 and two blocks' LDS fit a CU, else 1.
+// Blocks loop over jobs; the grid is what the CUs hold at once (the
+// occupancy query: a block that started late would extend the tail).
+int conv_fwd_mfma(int B, int Ci, int Co, int N, int K, bool flip, const float* x, const float* w,
+                  const float* bias, float* y, const float* mul, int act, float* dy, hipStream_t st) {
+  if (Ci % 16 != 0 || Ci > 64 || Co < 16 || Co > 64) return INC_EUNSUPPORTED;
+  const int CT = (Co + 15) / 16;
+  auto lds_of = [&](int pw) {
+    return (size_t(CT) * 16 * (Ci * K + 4) + size_t(64 * pw + K) * (Ci + 4)) * sizeof(float);
+  };
+  const int PW = (N > 64 && lds_of(2) <= 80 * 1024) ? 2 : 1;
+  const size_t lds = lds_of(PW);
+  if (lds > 150 * 1024) return INC_EUNSUPPORTED;
+  const int64_t jobs = int64_t(B) * ((N + 64 * PW - 1) / (64 * PW));
+  const void* fn;
+  switch (CT * 2 + PW - 1) {
+    case 2: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<1, 1>); break;
+    case 3: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<1, 2>); break;
+    case 4: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<2, 1>); break;
+    case 5: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<2, 2>); break;
+    case 6: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<3, 1>); break;
+    case 7: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<3, 2>); break;
+    case 8: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<4, 1>); break;
+    default: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<4, 2>); break;
+  }
+  int per_cu = 0;
+  INC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds));
+  if (per_cu < 1) return INC_EUNSUPPORTED;
+  const int nblk = int(std::min<int64_t>(jobs, int64_t(cu_count()) * per_cu));
+  int fl = flip ? 1 : 0;
+  void* args[] = {&B, &Ci, &Co, &N, &K, &fl, &x, &w, &bias, &y, &mul, &act, &dy};
+  INC_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(256), args, lds, st));
+  return check_launch("conv1d (matrix cores)");
+}
+
 template <typename T, int K, bool FLIP>
 int conv_fwd_k(int cpt, dim3 grid, size_t lds, hipStream_t st, int Ci, int Co, int N, const T* x, const T* w,
                const T* b, T* y, const T* mul, int act, T* dy) {
@@ -438,6 +679,12 @@ int conv_cpt(int Co) {
 template <typename T, bool FLIP>
 int conv_fwd_launch(int B, int Ci, int Co, int N, int K, const void* x, const void* w, const void* bias, void* y,
                     const void* mul, int act, void* dy, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    const int rc = conv_fwd_mfma(B, Ci, Co, N, K, FLIP, static_cast<const float*>(x), static_cast<const float*>(w),
+                                 static_cast<const float*>(bias), static_cast<float*>(y),
+                                 static_cast<const float*>(mul), act, static_cast<float*>(dy), st);
+    if (rc != INC_EUNSUPPORTED) return rc;
+  }
   const int TI = kWave * kPPT;
   size_t lds = size_t(Ci) * (TI + K - 1) * sizeof(T);
   INC_REQUIRE(lds <= 150 * 1024, INC_EUNSUPPORTED, "conv1d: %d input channels need %zu bytes of LDS", Ci, lds);

@@ -23,10 +23,17 @@ def ref_conv(x, w, b):
 # output-channel tile (Co = 1), thin layers whose waves split positions
 # (Ci = 1), one workspace slot (B * tiles = 1, idle waves), a partial last
 # position tile (n = 130); (1, 64, 64, 256, 9) exceeds the column limit and
+# takes the vector-ALU kernel.  float32 forward and input-gradient passes run
+# on the matrix cores when the pass's input channels are a multiple of 16 and
+# its output channels lie in [16, 64]: one to four output tiles (Co = 16, 32,
+# 48, 64), a padded tile (Co = 24), 64-, 128- and 256-position block tiles
+# with partial last tiles (n = 100, 300) and n below one tile with several
+# wraps of the halo (n = 16, 40), K = 1; (1, 64, 64, 256, 9) exceeds LDS and
 # takes the vector-ALU kernel.
 SHAPES = [(3, 1, 32, 128, 5), (3, 32, 32, 128, 5), (2, 32, 1, 128, 5), (2, 48, 48, 100, 7), (5, 16, 8, 64, 3),
           (1, 8, 8, 9, 9), (4, 32, 32, 1024, 5), (2, 48, 1, 512, 7), (3, 1, 48, 130, 7), (1, 1, 32, 60, 5),
-          (1, 64, 64, 256, 9), (2, 48, 48, 512, 7)]
+          (1, 64, 64, 256, 9), (2, 48, 48, 512, 7), (2, 32, 24, 40, 5), (1, 16, 16, 16, 9), (3, 32, 16, 64, 1),
+          (2, 64, 64, 300, 3)]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
