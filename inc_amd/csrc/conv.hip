@@ -222,7 +222,10 @@ __global__ void __launch_bounds__(256) conv_wgrad_kernel(int B, int Ci, int Co, 
 }
 
 // Sum of nparts partial rows: a block covers 16 consecutive elements with 16
-// partial-lanes each; lanes combine in a fixed order (deterministic).
+// partial-lanes each; lanes combine in a fixed order (deterministic).  Each
+// lane keeps eight running sums (rows p, p + 16, ..., p + 112 of every
+// 128-row group; the rows after the last whole group go to the first), so
+// eight loads are in flight rather than one.
 template <typename T>
 __global__ void __launch_bounds__(256) reduce_parts_kernel(int nparts, int width, const T* __restrict__ part,
                                                           T* __restrict__ dw, T* __restrict__ db, int nw) {
@@ -230,8 +233,16 @@ __global__ void __launch_bounds__(256) reduce_parts_kernel(int nparts, int width
   const int ex = threadIdx.x % 16, pl = threadIdx.x / 16;
   const int e = blockIdx.x * 16 + ex;
   T s = T(0);
-  if (e < width)
-    for (int p = pl; p < nparts; p += 16) s += part[size_t(p) * width + e];
+  if (e < width) {
+    T a[8] = {};
+    int p = pl;
+    for (; p + 112 < nparts; p += 128)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] += part[size_t(p + 16 * u) * width + e];
+    for (; p < nparts; p += 16) a[0] += part[size_t(p) * width + e];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += a[u];
+  }
   red[pl][ex] = s;
   __syncthreads();
   if (pl == 0 && e < width) {
