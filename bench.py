@@ -17,7 +17,10 @@ device copy (measured_copy_GBps).  compute: the same kernel's FP32 flops per
 launch (rocprofv3 SQ_INSTS_VALU_FLOPS_FP32, recorded under profiles/) over
 its duration, against the 157.3 TFLOP/s vector peak.  cpu_baseline: the FP64
 oracle (PyTorch CPU restatement of the same scheme) on a bounded sample,
-fwd+adj.
+fwd+adj.  other_workloads (not the metric): the same kernels at saturation,
+the rollout with its MSE loss fused in against torch's MSELoss, Burgers at
+saturation, and the INC training steps (Burgers, KS with checkpointing,
+ragged Burgers).
 """
 import argparse
 import json
@@ -313,6 +316,37 @@ def extra_ks_saturation(b=8192, h=200):
             "fwd_ms": tf * 1e3, "adj_ms": ta * 1e3}
 
 
+def extra_ks_mse(b=B, h=H):
+    """The headline rollout trained against a target trajectory: MSE loss and
+    its gradient w.r.t. the initial state, with the loss fused into the
+    kernels (simulate_rollout_mse) and with torch's MSELoss on the returned
+    trajectory (simulate_rollout + F.mse_loss), same inputs."""
+    import torch
+    import inc_amd
+    from torch.nn import functional as F
+    dev = torch.device("cuda")
+    sv = inc_amd.KSSolverTorch(inc_amd.KSSimParams([L], resolution=N, dt=DT, time_scheme="ETDRK4", device="cuda"))
+    sv.doublestep = False
+    Ld = torch.tensor([L], device=dev)
+    u0 = initial_state(b, N, L, 1006).float().to(dev)
+    with torch.no_grad():
+        target = sv.rollout(initial_state(b, N, L, 1007).float().to(dev), h, Ld)
+
+    def fused():
+        u = u0.clone().requires_grad_(True)
+        loss, _ = inc_amd.simulate_rollout_mse(sv, u, target, L=Ld)
+        loss.backward()
+
+    def unfused():
+        u = u0.clone().requires_grad_(True)
+        traj, _ = inc_amd.simulate_rollout(None, sv, u, h, None, L=Ld)
+        F.mse_loss(traj, target).backward()
+    tf, tu = _timed(fused, reps=5), _timed(unfused, reps=5)
+    return {"workload": f"KS ETDRK4 B={b} N={N} {h} steps fp32, MSE to a target, loss + gradient",
+            "fused_ms": tf * 1e3, "unfused_ms": tu * 1e3, "fused_trajectory_steps_per_s": b * h / tf,
+            "unfused_trajectory_steps_per_s": b * h / tu}
+
+
 def plain_path(solver, u0, Ld, ntraj=8, h=50):
     """One launch per trajectory per step (no batching), fwd + adjoint."""
     import torch
@@ -419,7 +453,7 @@ def main():
         out["plain_trajectory_steps_per_s"] = plain
         out["batched_over_plain"] = value / plain
     if not args.no_extras:
-        out["other_workloads"] = {"ks_saturated": extra_ks_saturation(),
+        out["other_workloads"] = {"ks_saturated": extra_ks_saturation(), "ks_mse_fused": extra_ks_mse(),
                                   "burgers_saturated": extra_burgers_saturation(), "burgers_inc": extra_burgers_inc(),
                                   "ks_inc_checkpointed": extra_ks_inc(), "ragged_burgers_inc": extra_ragged_inc()}
     if not args.no_cpu:

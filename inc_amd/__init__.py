@@ -6,9 +6,9 @@ for gfx950 (libinc_amd.so, C ABI in include/inc_amd.h).
 """
 from .batched import RaggedBurgers
 from .config_1d import BGSimParams, KSSimParams
-from .rollout import simulate_rollout
+from .rollout import simulate_rollout, simulate_rollout_mse
 from .solver_1d import BurgersSolverTorch, KSSolverTorch, weno_reconstruction
 
 __all__ = ["BurgersSolverTorch", "KSSolverTorch", "weno_reconstruction", "simulate_rollout",
-           "BGSimParams", "KSSimParams", "RaggedBurgers"]
+           "simulate_rollout_mse", "BGSimParams", "KSSimParams", "RaggedBurgers"]
 __version__ = "0.1.0"

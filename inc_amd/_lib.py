@@ -71,6 +71,10 @@ SIGNATURES = {
     "inc_burgers_rhs_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp]),
     "inc_burgers_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "inc_burgers_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "inc_burgers_fwd_mse": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp,
+                                           _vp, _vp]),
+    "inc_burgers_adj_mse": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                           ctypes.c_int, _vp]),
     "inc_burgers_adaptive_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_double, ctypes.c_int, _vp, _vp,
                                                 _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     "inc_burgers_adaptive_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
@@ -82,6 +86,8 @@ SIGNATURES = {
                                      _vp, ctypes.c_int, _vp, _vp]),
     "inc_ks_fwd": (ctypes.c_int, [ctypes.POINTER(KSDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "inc_ks_adj": (ctypes.c_int, [ctypes.POINTER(KSDesc), _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "inc_ks_fwd_mse": (ctypes.c_int, [ctypes.POINTER(KSDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "inc_ks_adj_mse": (ctypes.c_int, [ctypes.POINTER(KSDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "inc_conv1d_fwd": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp]),
     "inc_conv1d_bwd_data": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp]),
     "inc_conv1d_bwd_weight": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp]),

@@ -186,6 +186,13 @@ struct BArgs {
   T* stages;
   const T* g_traj;
   T* g_s;
+  // fused MSE to a target trajectory [B, steps, n]: the forward accumulates
+  // sum (u - target)^2 into loss_parts[b][4] (one partial per wavefront of
+  // the team); the adjoint adds *loss_scale * (traj - target) to the cotangent
+  // of every output, reading the stored trajectory from traj
+  const T* target;
+  T* loss_parts;
+  const T* loss_scale;  // device scalar
   // adaptive sub-stepping (cfl > 0, holding CFL * dx): per step and
   // trajectory, nsub sub-steps of sizes dts, stages indexed [b][step][sub][stage]
   T cfl;
@@ -196,6 +203,7 @@ struct BArgs {
 
 constexpr int kBlock = 256;
 constexpr int kMaxWaves = kBlock / kWave;
+constexpr int kLossSlots = 4;  // partial loss sums per trajectory (wavefronts per team <= 4)
 
 template <typename T>
 __device__ __forceinline__ bool better(T v, int i, T ov, int oi) {
@@ -476,6 +484,7 @@ __global__ void __launch_bounds__(kBlock) burgers_fwd_kernel(BArgs<T> a) {
   const T nu = a.nu_ptr ? (c.active ? a.nu_ptr[c.b] : T(0)) : a.nu;
   const T t0 = a.t_ptr ? (c.active ? a.t_ptr[c.b] : T(0)) : a.t0;
   T u0[PPT], uc[PPT], sv[PPT], xv[PPT], R[PPT];
+  T lsum = T(0);  // squared error to the target over this thread's points (fused MSE)
   const size_t row = size_t(c.b) * n;
 #pragma unroll
   for (int p = 0; p < PPT; ++p) {
@@ -564,11 +573,29 @@ __global__ void __launch_bounds__(kBlock) burgers_fwd_kernel(BArgs<T> a) {
       for (int p = 0; p < PPT; ++p)
         if (c.i0 + p < n) dst[c.i0 + p] = u0[p];
     }
+    if (a.target && own) {
+      const T* tg = a.target + (size_t(c.b) * a.steps + step) * n;
+#pragma unroll
+      for (int p = 0; p < PPT; ++p)
+        if (c.i0 + p < n) {
+          const T dd = u0[p] - tg[c.i0 + p];
+          lsum += dd * dd;
+        }
+    }
   }
   if (own) {
 #pragma unroll
     for (int p = 0; p < PPT; ++p)
       if (c.i0 + p < n) a.out[row + c.i0 + p] = u0[p];
+  }
+  if (a.target) {
+    // team sum: per wavefront by shuffles, one slot per wavefront of the team
+    const int w = c.team < kWave ? c.team : kWave;
+    lsum = wave_sum(lsum, w);
+    if (c.active && c.lt % w == 0) a.loss_parts[size_t(c.b) * kLossSlots + c.lt / w] = lsum;
+    if (c.active && c.lt == 0)
+      for (int k = c.team > kWave ? c.team / kWave : 1; k < kLossSlots; ++k)
+        a.loss_parts[size_t(c.b) * kLossSlots + k] = T(0);
   }
 }
 
@@ -649,12 +676,16 @@ __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
   const T dt = a.dt;
   const int parts = INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE;
   T lam[PPT], gs[PPT], st[3][PPT], cb[PPT], r[PPT], b1[PPT], b2[PPT];
+  // cotangent of output `step`: g_traj, plus *loss_scale * (traj - target)
+  // with a target
+  const T lsc = a.target ? *a.loss_scale : T(0);
   auto gload = [&](int step, T* dst) {
-    const T* src = a.g_traj + (size_t(c.b) * a.steps + step) * n;
+    const size_t at = (size_t(c.b) * a.steps + step) * n;
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
       int i = c.i0 + p;
-      dst[p] = (own && i < n) ? src[i] : T(0);
+      dst[p] = (own && i < n && a.g_traj) ? a.g_traj[at + i] : T(0);
+      if (a.target && own && i < n) dst[p] += lsc * (a.traj[at + i] - a.target[at + i]);
     }
   };
   gload(a.steps - 1, lam);
@@ -1230,10 +1261,18 @@ struct Adaptive {
   void* dts;
 };
 
+// Fused MSE arguments (see BArgs): target trajectory, forward partial sums,
+// adjoint scale.
+struct Mse {
+  const void* target = nullptr;
+  void* parts = nullptr;
+  const void* scale = nullptr;  // device scalar of the element type
+};
+
 template <typename T>
 int burgers_call(Kind kind, const inc_burgers_desc* d, int parts, const void* u, const void* s,
                  void* out, int steps, void* traj, void* stages, const void* g_traj, void* g_s,
-                 void* stream, const Adaptive& ad) {
+                 void* stream, const Adaptive& ad, const Mse& mse) {
   BArgs<T> a;
   int rc = fill_args<T>(d, a);
   if (rc) return rc;
@@ -1250,18 +1289,21 @@ int burgers_call(Kind kind, const inc_burgers_desc* d, int parts, const void* u,
   a.stages = static_cast<T*>(stages);
   a.g_traj = static_cast<const T*>(g_traj);
   a.g_s = static_cast<T*>(g_s);
+  a.target = static_cast<const T*>(mse.target);
+  a.loss_parts = static_cast<T*>(mse.parts);
+  a.loss_scale = static_cast<const T*>(mse.scale);
   return launch<T>(kind, d, a, stream);
 }
 
 int dispatch(Kind kind, const inc_burgers_desc* d, int parts, const void* u, const void* s, void* out,
              int steps, void* traj, void* stages, const void* g_traj, void* g_s, void* stream,
-             const Adaptive& ad = Adaptive{0.0, 0, nullptr, nullptr}) {
+             const Adaptive& ad = Adaptive{0.0, 0, nullptr, nullptr}, const Mse& mse = Mse{}) {
   clear_error();
   INC_REQUIRE(d != nullptr, INC_EINVAL, "null descriptor");
   if (d->dtype == INC_F32)
-    return burgers_call<float>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream, ad);
+    return burgers_call<float>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream, ad, mse);
   if (d->dtype == INC_F64)
-    return burgers_call<double>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream, ad);
+    return burgers_call<double>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream, ad, mse);
   set_error("unsupported dtype %d", d->dtype);
   return INC_EINVAL;
 }
@@ -1331,6 +1373,32 @@ int inc_burgers_adj(const inc_burgers_desc* d, const void* stages, const void* g
   return inc::dispatch(Kind::Adj, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE,
                        nullptr, nullptr, g_u, steps, nullptr, const_cast<void*>(stages), g_traj, g_s,
                        stream);
+}
+
+int inc_burgers_fwd_mse(const inc_burgers_desc* d, const void* u, const void* s, void* u_out, int steps,
+                        void* traj, void* stages, const void* target, void* loss_parts, void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(target != nullptr && loss_parts != nullptr, INC_EINVAL, "the MSE forward needs target and loss_parts");
+  inc::Mse m;
+  m.target = target;
+  m.parts = loss_parts;
+  return inc::dispatch(Kind::Fwd, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE, u, s, u_out,
+                       steps, traj, stages, nullptr, nullptr, stream, inc::Adaptive{0.0, 0, nullptr, nullptr}, m);
+}
+
+int inc_burgers_adj_mse(const inc_burgers_desc* d, const void* stages, const void* traj, const void* target,
+                        const void* scale, const void* g_traj, void* g_u, void* g_s, int steps, void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(stages != nullptr && traj != nullptr && target != nullptr && scale != nullptr, INC_EINVAL,
+              "the MSE adjoint needs stages, traj, target and scale");
+  inc::Mse m;
+  m.target = target;
+  m.scale = scale;
+  return inc::dispatch(Kind::Adj, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE, nullptr,
+                       nullptr, g_u, steps, const_cast<void*>(traj), const_cast<void*>(stages), g_traj, g_s,
+                       stream, inc::Adaptive{0.0, 0, nullptr, nullptr}, m);
 }
 
 int inc_burgers_adaptive_fwd(const inc_burgers_desc* d, double cfl, int max_sub, const void* u, const void* s,

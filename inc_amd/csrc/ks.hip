@@ -447,9 +447,37 @@ struct KArgs {
   T* stages;
   const T* g_traj;
   T* g_s;
+  // fused MSE to a target trajectory [B, steps, N]: the forward accumulates
+  // sum (u - target)^2 into loss_parts[b][4] (one partial per wavefront of the
+  // team); the adjoint adds *loss_scale * (traj - target) to the cotangent of
+  // every output, reading the stored trajectory from traj
+  const T* target;
+  T* loss_parts;
+  const T* loss_scale;  // device scalar
 };
 
 constexpr int kTableStride = INC_KS_TABLE_STRIDE;
+constexpr int kLossSlots = 4;  // partial loss sums per trajectory (wavefronts per team <= 4)
+
+// Sum over a team's lanes of two per-lane values; the first lane of every
+// wavefront of the team writes its wavefront's sums to parts[b][w] (and
+// the team's first lane zero-fills the slots past its wavefront count).
+template <int TL, typename T>
+__device__ __forceinline__ void write_loss_parts(T l0, T l1, int j, int b0, int b1, T* parts) {
+  constexpr int W = TL < kWave ? TL : kWave;
+  l0 = wave_sum(l0, W);
+  l1 = wave_sum(l1, W);
+  constexpr int NW = TL > kWave ? TL / kWave : 1;
+  if (j % W == 0) {
+    parts[size_t(b0) * kLossSlots + j / W] = l0;
+    if (b1 >= 0) parts[size_t(b1) * kLossSlots + j / W] = l1;
+  }
+  if (j == 0)
+    for (int w = NW; w < kLossSlots; ++w) {
+      parts[size_t(b0) * kLossSlots + w] = T(0);
+      if (b1 >= 0) parts[size_t(b1) * kLossSlots + w] = T(0);
+    }
+}
 
 __host__ __device__ constexpr int stages_per_sub(int scheme) {
   return scheme == INC_ETD1 ? 1 : (scheme == INC_ETRK2 ? 2 : 4);
@@ -486,6 +514,23 @@ __device__ __forceinline__ void load_pair(const T* base, size_t stride_b, int b0
   for (int m = 0; m < R; ++m) {
     v[m].x = r0[j + m * TL];
     v[m].y = r1 ? r1[j + m * TL] : T(0);
+  }
+}
+
+// load_pair with both rows read unconditionally (row b0 twice when the pair
+// has no second trajectory; the caller ignores that half).  A load whose
+// value merges from a branch is copied at the join, and the copy waits for
+// it: this form keeps a prefetch in flight.
+template <typename T, int N, int R>
+__device__ __forceinline__ void load_pair_both(const T* base, size_t stride_b, int b0, int b1, int j,
+                                               cpx<T>* v) {
+  constexpr int TL = N / R;
+  const T* r0 = base + size_t(b0) * stride_b;
+  const T* r1 = base + size_t(b1 >= 0 ? b1 : b0) * stride_b;
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    v[m].x = r0[j + m * TL];
+    v[m].y = r1[j + m * TL];
   }
 }
 
@@ -541,7 +586,7 @@ __device__ __forceinline__ int pair_rows(const KArgs<T>& a, int p, int& b0, int&
   return a.table_idx ? a.table_idx[b0] : 0;
 }
 
-template <typename T, int N, int R>
+template <typename T, int N, int R, bool LOSS>
 __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
   using FC = FftCfg<T, N, R>;
   constexpr int TL = N / R;
@@ -579,14 +624,42 @@ __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
     if (a.stages) store_pair<T, N, R>(a.stages + (size_t(step) * nst + st) * N, size_t(a.steps) * nst * N,
                                       b0, b1, j, zz);
   };
+  // output u^{step+1} (physical): trajectory row and, with a target, the
+  // squared-error partial sums of the pair's two trajectories.  The target
+  // row of an output is loaded when the previous output is stored (a step
+  // ahead, unconditionally: the last output loads its own row again), so
+  // its HBM latency stays off the step's chain of transforms.
+  T l0 = T(0), l1 = T(0);
+  cpx<T> tg[R];
+  if constexpr (LOSS) load_pair_both<T, N, R>(a.target, size_t(a.steps) * N, b0, b1, j, tg);
+  // (the squared errors come before the trajectory stores, held there by a
+  // scheduling barrier: the wait for the target loads would otherwise also
+  // wait for those stores)
+  auto state_out = [&](int step, const cpx<T>* zz) __attribute__((always_inline)) {
+    if constexpr (LOSS) {
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const T dx = zz[m].x - tg[m].x, dy = zz[m].y - tg[m].y;
+        l0 += dx * dx;
+        l1 += dy * dy;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      store_pair<T, N, R>(a.traj + size_t(step) * N, size_t(a.steps) * N, b0, b1, j, zz);  // always given
+    } else if (a.traj) {
+      store_pair<T, N, R>(a.traj + size_t(step) * N, size_t(a.steps) * N, b0, b1, j, zz);
+    }
+    if constexpr (LOSS) {
+      const int nxt = step + 1 < a.steps ? step + 1 : step;
+      load_pair_both<T, N, R>(a.target + size_t(nxt) * N, size_t(a.steps) * N, b0, b1, j, tg);
+    }
+  };
 
   for (int step = 0; step < a.steps; ++step) {
     for (int sub = 0; sub < a.substeps; ++sub) {
 #pragma unroll
       for (int m = 0; m < R; ++m) z[m] = V[m];
       fft<T, N, R, +1>(z, tw, lds, j);  // physical state
-      if (sub == 0 && step > 0 && a.traj)
-        store_pair<T, N, R>(a.traj + size_t(step - 1) * N, size_t(a.steps) * N, b0, b1, j, z);
+      if (sub == 0 && step > 0) state_out(step - 1, z);
       int st = sub * ns;
       stage_out(step, st++, z);
       if (a.scheme == INC_ETD1) {
@@ -649,11 +722,12 @@ __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
 #pragma unroll
   for (int m = 0; m < R; ++m) z[m] = V[m];
   fft<T, N, R, +1>(z, tw, lds, j);
-  if (a.traj) store_pair<T, N, R>(a.traj + size_t(a.steps - 1) * N, size_t(a.steps) * N, b0, b1, j, z);
+  state_out(a.steps - 1, z);
   store_pair<T, N, R>(a.out, N, b0, b1, j, z);
+  if constexpr (LOSS) write_loss_parts<TL>(l0, l1, j, b0, b1, a.loss_parts);
 }
 
-template <typename T, int N, int R>
+template <typename T, int N, int R, bool LOSS>
 __global__ void __launch_bounds__(256) ks_adj_kernel(KArgs<T> a) {
   using FC = FftCfg<T, N, R>;
   constexpr int TL = N / R;
@@ -678,8 +752,27 @@ __global__ void __launch_bounds__(256) ks_adj_kernel(KArgs<T> a) {
   cpx<T> Vb[R], Sb[R], y[R];
 #pragma unroll
   for (int m = 0; m < R; ++m) Sb[m] = cpx<T>{T(0), T(0)};
+  // cotangent of output `step` (u^{step+1}): g_traj, plus *loss_scale *
+  // (traj - target) with a target
+  T lsc = T(0);
+  if constexpr (LOSS) lsc = *a.loss_scale;
+  auto load_cot = [&](int step, cpx<T>* dst) __attribute__((always_inline)) {
+    if (!LOSS || a.g_traj) {
+      load_pair<T, N, R>(a.g_traj + size_t(step) * N, gstride, b0, b1, j, dst);
+    } else {
+#pragma unroll
+      for (int m = 0; m < R; ++m) dst[m] = cpx<T>{T(0), T(0)};
+    }
+    if constexpr (LOSS) {
+      cpx<T> tr[R], tg[R];
+      load_pair<T, N, R>(a.traj + size_t(step) * N, gstride, b0, b1, j, tr);
+      load_pair<T, N, R>(a.target + size_t(step) * N, gstride, b0, b1, j, tg);
+#pragma unroll
+      for (int m = 0; m < R; ++m) dst[m] = dst[m] + lsc * (tr[m] - tg[m]);
+    }
+  };
   // cotangent of the last output u^H = F^H(V^H): Vbar = F(gbar)
-  load_pair<T, N, R>(a.g_traj + size_t(a.steps - 1) * N, gstride, b0, b1, j, Vb);
+  load_cot(a.steps - 1, Vb);
 
   // Stage states and loss cotangent of sub-step `idx` (reverse order),
   // prefetched into registers one sub-step ahead so the HBM latency hides
@@ -699,11 +792,16 @@ __global__ void __launch_bounds__(256) ks_adj_kernel(KArgs<T> a) {
 #pragma unroll
     for (int k = 0; k < 4; ++k)
       if (k < ns) load_pair<T, N, R>(a.stages + (size_t(step) * nst + sub * ns + k) * N, sstride, b0, b1, j, dst[k]);
-    if (sub == 0 && step > 0) load_pair<T, N, R>(a.g_traj + size_t(step - 1) * N, gstride, b0, b1, j, ldst);
+    if (sub == 0 && step > 0) load_cot(step - 1, ldst);
   };
   const int total = a.steps * a.substeps;
   fetch(total - 1, cur, lcur);
   fft<T, N, R, -1>(Vb, tw, lds, j);
+  // Complete the loads above before the loop.  Loads still in flight into
+  // `cur` on entry would make the compiler wait for every outstanding load
+  // at the loop's first use of `cur`, i.e. for the prefetch issued at the
+  // top of each iteration, which then no longer overlaps anything.
+  __builtin_amdgcn_s_waitcnt(0);
 
   // z-bar = 2 z * Re F^H(conj(G~) X) (+ loss cotangent), then F(z-bar)
   auto back_nl = [&](const cpx<T>* X, const cpx<T>* zs, bool with_loss, cpx<T>* out) {
@@ -898,9 +996,12 @@ __global__ void ks_table_kernel(int scheme, int n, double dt, int dealias, const
 
 // ---- host dispatch -----------------------------------------------------------------
 template <typename T, int N, int R>
-void* pick_kernel(bool adj) {
-  return adj ? reinterpret_cast<void*>(&ks_adj_kernel<T, N, R>)
-             : reinterpret_cast<void*>(&ks_fwd_kernel<T, N, R>);
+void* pick_kernel(bool adj, bool loss) {
+  if (loss)
+    return adj ? reinterpret_cast<void*>(&ks_adj_kernel<T, N, R, true>)
+               : reinterpret_cast<void*>(&ks_fwd_kernel<T, N, R, true>);
+  return adj ? reinterpret_cast<void*>(&ks_adj_kernel<T, N, R, false>)
+             : reinterpret_cast<void*>(&ks_fwd_kernel<T, N, R, false>);
 }
 
 // Radix choices: R points per lane, TL = N / R lanes per trajectory pair
@@ -926,33 +1027,41 @@ int default_radix(int n, int npairs) {
 }
 
 template <typename T, int N>
-void* pick_r(int r, bool adj) {
+void* pick_r(int r, bool adj, bool loss) {
   if constexpr (N / 2 <= 256)
-    if (r == 2) return pick_kernel<T, N, 2>(adj);
+    if (r == 2) return pick_kernel<T, N, 2>(adj, loss);
   if constexpr (N / 4 <= 256)
-    if (r == 4) return pick_kernel<T, N, 4>(adj);
+    if (r == 4) return pick_kernel<T, N, 4>(adj, loss);
   if constexpr (!std::is_same<T, double>::value && N / 8 <= 256)
-    if (r == 8) return pick_kernel<T, N, 8>(adj);
+    if (r == 8) return pick_kernel<T, N, 8>(adj, loss);
   return nullptr;
 }
 
 template <typename T>
-void* pick_n(int n, int r, bool adj) {
+void* pick_n(int n, int r, bool adj, bool loss) {
   switch (n) {
-    case 16: return pick_r<T, 16>(r, adj);
-    case 32: return pick_r<T, 32>(r, adj);
-    case 64: return pick_r<T, 64>(r, adj);
-    case 128: return pick_r<T, 128>(r, adj);
-    case 256: return pick_r<T, 256>(r, adj);
-    case 512: return pick_r<T, 512>(r, adj);
-    case 1024: return pick_r<T, 1024>(r, adj);
+    case 16: return pick_r<T, 16>(r, adj, loss);
+    case 32: return pick_r<T, 32>(r, adj, loss);
+    case 64: return pick_r<T, 64>(r, adj, loss);
+    case 128: return pick_r<T, 128>(r, adj, loss);
+    case 256: return pick_r<T, 256>(r, adj, loss);
+    case 512: return pick_r<T, 512>(r, adj, loss);
+    case 1024: return pick_r<T, 1024>(r, adj, loss);
     default: return nullptr;
   }
 }
 
+// Fused MSE arguments (see KArgs): target trajectory, forward partial sums,
+// adjoint scale.
+struct Mse {
+  const void* target = nullptr;
+  void* parts = nullptr;
+  const void* scale = nullptr;  // device scalar of the element type
+};
+
 template <typename T>
 int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int steps, void* traj,
-              void* stages, const void* g_traj, void* g_s, bool adj, void* stream) {
+              void* stages, const void* g_traj, void* g_s, bool adj, void* stream, const Mse& mse) {
   INC_REQUIRE(d->n >= 16 && d->n <= 1024 && (d->n & (d->n - 1)) == 0, INC_EUNSUPPORTED,
               "KS resolution must be a power of two in [16, 1024] (got %d)", d->n);
   INC_REQUIRE(d->scheme >= INC_ETD1 && d->scheme <= INC_ETDRK4, INC_EINVAL, "unknown KS time scheme %d",
@@ -962,12 +1071,17 @@ int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int
   INC_REQUIRE(d->table != nullptr, INC_EINVAL, "coefficient table is NULL");
   if (d->batch == 0) return INC_OK;
   INC_REQUIRE(!adj || stages != nullptr, INC_EINVAL, "adjoint needs the saved stages");
-  INC_REQUIRE(!adj || g_traj != nullptr, INC_EINVAL, "adjoint needs the output cotangents (g_traj is NULL)");
+  INC_REQUIRE(!adj || g_traj != nullptr || mse.target != nullptr, INC_EINVAL,
+              "adjoint needs the output cotangents (g_traj is NULL)");
+  INC_REQUIRE(!adj || mse.target == nullptr || traj != nullptr, INC_EINVAL,
+              "the MSE adjoint needs the stored trajectory");
+  INC_REQUIRE(adj || mse.target == nullptr || (mse.parts != nullptr && traj != nullptr), INC_EINVAL,
+              "the MSE forward needs loss_parts and traj");
   const int packed = d->table_idx == nullptr ? 1 : 0;
   const int npairs = packed ? (d->batch + 1) / 2 : d->batch;
   int r = d->radix > 0 ? d->radix : default_radix(d->n, npairs);
   if (std::is_same<T, double>::value && r != 2 && r != 4) r = default_radix(d->n, npairs);
-  void* fn = pick_n<T>(d->n, r, adj);
+  void* fn = pick_n<T>(d->n, r, adj, mse.target != nullptr);
   INC_REQUIRE(fn != nullptr, INC_EUNSUPPORTED, "no KS kernel for n=%d radix=%d", d->n, r);
   KArgs<T> a{};
   a.batch = d->batch;
@@ -985,6 +1099,9 @@ int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int
   a.stages = static_cast<T*>(stages);
   a.g_traj = static_cast<const T*>(g_traj);
   a.g_s = static_cast<T*>(g_s);
+  a.target = static_cast<const T*>(mse.target);
+  a.loss_parts = static_cast<T*>(mse.parts);
+  a.loss_scale = static_cast<const T*>(mse.scale);
   a.twiddle = static_cast<const T*>(d->twiddle);
   INC_REQUIRE(a.twiddle != nullptr, INC_EINVAL, "twiddle table is NULL");
   const int tl = d->n / r;
@@ -996,11 +1113,13 @@ int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int
 }
 
 int ks_dispatch(const inc_ks_desc* d, const void* u, const void* s, void* out, int steps, void* traj,
-                void* stages, const void* g_traj, void* g_s, bool adj, void* stream) {
+                void* stages, const void* g_traj, void* g_s, bool adj, void* stream, const Mse& mse = Mse{}) {
   clear_error();
   INC_REQUIRE(d != nullptr, INC_EINVAL, "null descriptor");
-  if (d->dtype == INC_F32) return ks_launch<float>(d, u, s, out, steps, traj, stages, g_traj, g_s, adj, stream);
-  if (d->dtype == INC_F64) return ks_launch<double>(d, u, s, out, steps, traj, stages, g_traj, g_s, adj, stream);
+  if (d->dtype == INC_F32)
+    return ks_launch<float>(d, u, s, out, steps, traj, stages, g_traj, g_s, adj, stream, mse);
+  if (d->dtype == INC_F64)
+    return ks_launch<double>(d, u, s, out, steps, traj, stages, g_traj, g_s, adj, stream, mse);
   set_error("unsupported dtype %d", d->dtype);
   return INC_EINVAL;
 }
@@ -1043,6 +1162,28 @@ int inc_ks_adj(const inc_ks_desc* d, const void* stages, const void* g_traj, voi
                void* stream) {
   return inc::ks_dispatch(d, nullptr, nullptr, g_u, steps, nullptr, const_cast<void*>(stages), g_traj, g_s,
                           true, stream);
+}
+
+int inc_ks_fwd_mse(const inc_ks_desc* d, const void* u, const void* s, void* u_out, int steps, void* traj,
+                   void* stages, const void* target, void* loss_parts, void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(target != nullptr && loss_parts != nullptr, INC_EINVAL, "the MSE forward needs target and loss_parts");
+  inc::Mse m;
+  m.target = target;
+  m.parts = loss_parts;
+  return inc::ks_dispatch(d, u, s, u_out, steps, traj, stages, nullptr, nullptr, false, stream, m);
+}
+
+int inc_ks_adj_mse(const inc_ks_desc* d, const void* stages, const void* traj, const void* target,
+                   const void* scale, const void* g_traj, void* g_u, void* g_s, int steps, void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(target != nullptr && traj != nullptr && scale != nullptr, INC_EINVAL,
+              "the MSE adjoint needs traj, target and scale");
+  inc::Mse m;
+  m.target = target;
+  m.scale = scale;
+  return inc::ks_dispatch(d, nullptr, nullptr, g_u, steps, const_cast<void*>(traj), const_cast<void*>(stages),
+                          g_traj, g_s, true, stream, m);
 }
 
 }  // extern "C"

@@ -10,6 +10,10 @@ as the reference does.  Two MI355X additions:
 * ``checkpoint_every=k`` recomputes the INC rollout in segments of k steps
   during the backward pass (segment-start states are the only activations
   kept), for long unrolled horizons.
+
+``simulate_rollout_mse(solver, u_initial, target, L)`` is a plain rollout
+followed by the trainer's MSE loss (trainer/trainer_1d.py:723) with the loss
+fused into the kernels.
 """
 import torch
 from torch.utils.checkpoint import checkpoint
@@ -88,3 +92,17 @@ def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, c
         u = res[:, -1]
     res = torch.cat(results, 1)
     return res, (torch.cat(corrections, 1) if correction_term == "INC" else None)
+
+
+def simulate_rollout_mse(solver, u_initial, target, L=None):
+    """``simulate_rollout(None, solver, u_initial, target.shape[1], None, L)``
+    followed by ``torch.nn.MSELoss()(results, target)`` (trainer_1d.py:154-183
+    and :723), with the loss computed in the forward kernel and its gradient
+    seeded inside the adjoint kernel.  Returns (loss, results).  Needs a
+    fused rollout: a fixed time step (Burgers) or no correction_u (KS)."""
+    solver.correction_s = None
+    if isinstance(solver, KSSolverTorch):
+        return solver.rollout_mse(u_initial, target, L)
+    if isinstance(solver, BurgersSolverTorch):
+        return solver.rollout_mse(u_initial, target)
+    raise ValueError(f"simulate_rollout_mse needs an inc_amd solver (got {type(solver).__name__})")

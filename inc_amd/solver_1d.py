@@ -106,6 +106,68 @@ class _BurgersAdvance(torch.autograd.Function):
         return g_u, g_s, None, None, None, None, None, None
 
 
+def _mse_target(target, u, steps):
+    b, n = u.shape
+    if tuple(target.shape) != (b, steps, n):
+        raise ValueError(f"target must be [batch, steps, n] = {(b, steps, n)} (got {tuple(target.shape)})")
+    L.require_gpu(target)
+    return target.detach().to(dtype=u.dtype).contiguous()
+
+
+def _mse_backward_common(ctx, g_loss, g_traj, traj, target):
+    """Device scale 2 g_loss / M for the adjoint, the optional extra
+    trajectory cotangent, and the target's gradient when it is needed."""
+    m = traj.numel()
+    scale = (g_loss.detach().to(traj.dtype) * (2.0 / m)).reshape(1).contiguous()
+    g_traj = g_traj.contiguous() if g_traj is not None else None
+    g_target = -scale * (traj - target) if ctx.needs_input_grad[2] else None
+    return scale, g_traj, g_target
+
+
+class _BurgersRolloutMSE(torch.autograd.Function):
+    """`steps` steps of the Burgers map with the MSE to a target trajectory
+    fused into the kernels (inc_burgers_fwd_mse / _adj_mse); returns
+    (loss, [B, steps, N] trajectory).  The loss is the mean over all of
+    batch, steps and points, torch.nn.MSELoss's default."""
+
+    @staticmethod
+    def forward(ctx, u, s, target, solver, dt, steps, t, need_stages):
+        ctx.set_materialize_grads(False)
+        u = u.contiguous()
+        s = s.contiguous() if s is not None else None
+        b, n = u.shape
+        tgt = _mse_target(target, u, steps)
+        desc, keep = solver._desc(u, dt, t)
+        out = torch.empty_like(u)
+        traj = u.new_empty(b, steps, n)
+        ns = 3 if solver.time_scheme == "SSPRK3" else 1
+        stages = u.new_empty(b, steps, ns, n) if need_stages else None
+        parts = u.new_empty(b, 4)
+        rc = L.load().inc_burgers_fwd_mse(ctypes.byref(desc), L.ptr(u), L.ptr(s), L.ptr(out), steps, L.ptr(traj),
+                                          L.ptr(stages), L.ptr(tgt), L.ptr(parts), L.stream())
+        L.check(rc, "BurgersSolverTorch.rollout_mse")
+        ctx.save_for_backward(stages, traj, tgt)
+        ctx.desc, ctx.keep, ctx.steps, ctx.shape = desc, keep, steps, (b, n)
+        return parts.sum() / traj.numel(), traj
+
+    @staticmethod
+    def backward(ctx, g_loss, g_traj):
+        stages, traj, tgt = ctx.saved_tensors
+        if stages is None:
+            raise RuntimeError("Burgers rollout was run without saved stages; enable grad before the rollout")
+        if g_loss is None:
+            g_loss = traj.new_zeros(())
+        scale, g_traj, g_target = _mse_backward_common(ctx, g_loss, g_traj, traj, tgt)
+        b, n = ctx.shape
+        g_u = torch.empty(b, n, dtype=traj.dtype, device=traj.device)
+        g_s = torch.empty_like(g_u) if ctx.needs_input_grad[1] else None
+        rc = L.load().inc_burgers_adj_mse(ctypes.byref(ctx.desc), L.ptr(stages), L.ptr(traj), L.ptr(tgt),
+                                          L.ptr(scale), L.ptr(g_traj), L.ptr(g_u), L.ptr(g_s), ctx.steps,
+                                          L.stream())
+        L.check(rc, "BurgersSolverTorch.rollout_mse backward")
+        return g_u, g_s, g_target, None, None, None, None, None
+
+
 class _BurgersAdaptive(torch.autograd.Function):
     """`steps` steps of dt, each sub-stepped per trajectory to the CFL limit
     on the GPU (inc_burgers_adaptive_fwd); returns the final state or the
@@ -432,6 +494,30 @@ class BurgersSolverTorch:
             self.t += self.dt
         return traj
 
+    def rollout_mse(self, u, target):
+        """Fused rollout over ``target.shape[1]`` steps with the MSE to
+        ``target`` [B, steps, N] computed in the kernels: returns (loss,
+        trajectory), the loss being torch.nn.MSELoss()(trajectory, target)
+        (trainer_1d.py:723 after simulate_rollout).  The backward seeds the
+        adjoint with 2 (u - target) / M inside the kernel; no cotangent
+        tensor is formed.  Fixed time step only."""
+        if self.adaptive_CFL is not None:
+            raise ValueError("rollout_mse runs a fixed time step; adaptive_CFL is set")
+        L.require_gpu(u, self.correction_s, target)
+        if self._team_auto:
+            self._team_auto = False
+            self.team = autotune.tune_burgers(u.shape[-1], u.shape[0], u.dtype, self.time_scheme,
+                                              self.param.convection_term)
+        steps = target.shape[1]
+        s = self.correction_s
+        if s is not None:
+            s = s.to(dtype=u.dtype)
+        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
+        loss, traj = _BurgersRolloutMSE.apply(u, s, target, self, self.dt, steps, self.t, need)
+        for _ in range(steps):
+            self.t += self.dt
+        return loss, traj
+
 
 # ---------------------------------------------------------------------------
 # Kuramoto-Sivashinsky
@@ -477,6 +563,47 @@ class _KSAdvance(torch.autograd.Function):
                                  ctx.steps, L.stream())
         L.check(rc, "KSSolverTorch.step backward")
         return g_u, g_s, None, None, None, None, None, None, None
+
+
+class _KSRolloutMSE(torch.autograd.Function):
+    """Fused KS rollout with the MSE to a target trajectory
+    (inc_ks_fwd_mse / _adj_mse); returns (loss, [B, steps, N] trajectory)."""
+
+    @staticmethod
+    def forward(ctx, u, s, target, solver, scheme, substeps, steps, domain_size, need_stages):
+        ctx.set_materialize_grads(False)
+        u = u.contiguous()
+        s = s.contiguous() if s is not None else None
+        b, n = u.shape
+        tgt = _mse_target(target, u, steps)
+        desc, keep = solver._desc(u, scheme, substeps, domain_size)
+        out = torch.empty_like(u)
+        traj = u.new_empty(b, steps, n)
+        nst = _KS_NS[scheme] * substeps
+        stages = u.new_empty(b, steps, nst, n) if need_stages else None
+        parts = u.new_empty(b, 4)
+        rc = L.load().inc_ks_fwd_mse(ctypes.byref(desc), L.ptr(u), L.ptr(s), L.ptr(out), steps, L.ptr(traj),
+                                     L.ptr(stages), L.ptr(tgt), L.ptr(parts), L.stream())
+        L.check(rc, "KSSolverTorch.rollout_mse")
+        ctx.save_for_backward(stages, traj, tgt)
+        ctx.desc, ctx.keep, ctx.steps, ctx.shape = desc, keep, steps, (b, n)
+        return parts.sum() / traj.numel(), traj
+
+    @staticmethod
+    def backward(ctx, g_loss, g_traj):
+        stages, traj, tgt = ctx.saved_tensors
+        if stages is None:
+            raise RuntimeError("KS rollout was run without saved stages; enable grad before the rollout")
+        if g_loss is None:
+            g_loss = traj.new_zeros(())
+        scale, g_traj, g_target = _mse_backward_common(ctx, g_loss, g_traj, traj, tgt)
+        b, n = ctx.shape
+        g_u = torch.empty(b, n, dtype=traj.dtype, device=traj.device)
+        g_s = torch.empty_like(g_u) if ctx.needs_input_grad[1] else None
+        rc = L.load().inc_ks_adj_mse(ctypes.byref(ctx.desc), L.ptr(stages), L.ptr(traj), L.ptr(tgt), L.ptr(scale),
+                                     L.ptr(g_traj), L.ptr(g_u), L.ptr(g_s), ctx.steps, L.stream())
+        L.check(rc, "KSSolverTorch.rollout_mse backward")
+        return g_u, g_s, g_target, None, None, None, None, None, None
 
 
 class KSSolverTorch:
@@ -625,3 +752,23 @@ class KSSolverTorch:
             raise ValueError("fused rollout does not apply correction_u; use step()")
         scheme = _KS_SCHEMES[self.time_scheme]
         return self._advance(u, domain_size, scheme, 2 if self.doublestep else 1, steps=mstep, want_traj=True)
+
+    def rollout_mse(self, u, target, domain_size=None):
+        """Fused rollout over ``target.shape[1]`` step() calls with the MSE
+        to ``target`` [B, steps, N] computed in the kernels: returns (loss,
+        trajectory) as BurgersSolverTorch.rollout_mse does."""
+        if self.correction_u is not None:
+            raise ValueError("fused rollout does not apply correction_u; use step()")
+        L.require_gpu(u, self.correction_s, target)
+        scheme = _KS_SCHEMES[self.time_scheme]
+        substeps = 2 if self.doublestep else 1
+        if domain_size is None:
+            domain_size = self.domain_size
+        if self._radix_auto:
+            self._radix_auto = False
+            self.radix = autotune.tune_ks(u.shape[-1], u.shape[0], u.dtype, self.time_scheme, dt=self.dt)
+        s = self.correction_s
+        if s is not None:
+            s = s.to(dtype=u.dtype)
+        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
+        return _KSRolloutMSE.apply(u, s, target, self, scheme, substeps, target.shape[1], domain_size, need)

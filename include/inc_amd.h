@@ -122,6 +122,29 @@ int inc_burgers_adj(const inc_burgers_desc* d, const void* stages,
                     void* stream);
 
 /*
+ * Rollout with its MSE to a target trajectory fused in (simulate_rollout,
+ * trainer/trainer_1d.py:154-183, followed by the loss at :723,
+ * MSELoss(results, target)):
+ *   target      [batch, steps, n] target trajectory
+ *   loss_parts  [batch, 4] written by the forward: partial sums whose total is
+ *               sum over steps and points of (u - target)^2 (one partial per
+ *               wavefront of a trajectory's team, unused slots zero)
+ * inc_burgers_fwd_mse is inc_burgers_fwd plus loss_parts; traj is still
+ * written when given (the adjoint reads it).
+ * inc_burgers_adj_mse is inc_burgers_adj with the cotangent of output t
+ *   g_traj[t] (optional, NULL for none) + scale[0] * (traj[t] - target[t]),
+ * scale a device scalar of the element type (2 dL/dloss / (batch*steps*n)
+ * for the mean), read by the kernel, so no host synchronisation.
+ */
+int inc_burgers_fwd_mse(const inc_burgers_desc* d, const void* u, const void* s,
+                        void* u_out, int steps, void* traj, void* stages,
+                        const void* target, void* loss_parts, void* stream);
+int inc_burgers_adj_mse(const inc_burgers_desc* d, const void* stages,
+                        const void* traj, const void* target, const void* scale,
+                        const void* g_traj, void* g_u, void* g_s, int steps,
+                        void* stream);
+
+/*
  * Adaptive sub-stepping per trajectory (BurgersSolverTorch.step_adaptive,
  * solver_1d.py:148-179, which sub-steps the whole batch to
  * dt_j = min(remaining, CFL * dx / max|u|) with max|u| read back to the host
@@ -229,6 +252,13 @@ int inc_ks_fwd(const inc_ks_desc* d, const void* u, const void* s, void* u_out,
 /* Adjoint of inc_ks_fwd; same contract as inc_burgers_adj. */
 int inc_ks_adj(const inc_ks_desc* d, const void* stages, const void* g_traj,
                void* g_u, void* g_s, int steps, void* stream);
+/* Fused-MSE rollout; same contract as inc_burgers_fwd_mse / _adj_mse. */
+int inc_ks_fwd_mse(const inc_ks_desc* d, const void* u, const void* s, void* u_out,
+                   int steps, void* traj, void* stages, const void* target,
+                   void* loss_parts, void* stream);
+int inc_ks_adj_mse(const inc_ks_desc* d, const void* stages, const void* traj,
+                   const void* target, const void* scale, const void* g_traj,
+                   void* g_u, void* g_s, int steps, void* stream);
 
 /*
  * Circular 1D convolution of the INC correction network s(u) (the reference's

@@ -77,6 +77,18 @@ def test_argument_errors_map_to_codes_without_launch():
     assert rc == L.INC_EINVAL and b"activation" in lib.inc_last_error()
     rc = lib.inc_conv1d_fwd_act(L.INC_F32, L.INC_ACT_GELU, 1, 1, 1, 16, 3, None, None, None, None, None, None)
     assert rc == L.INC_EINVAL and b"needs dy" in lib.inc_last_error()
+    # fused MSE rollouts: the forward needs a target and loss partials, the
+    # adjoint the stored trajectory, the target and the device scale
+    d = L.BurgersDesc(dtype=L.INC_F32, batch=1, n=64, dx=0.1, dt=0.01)
+    rc = lib.inc_burgers_fwd_mse(ctypes.byref(d), None, None, None, 1, None, None, None, None, None)
+    assert rc == L.INC_EINVAL and b"target" in lib.inc_last_error()
+    rc = lib.inc_burgers_adj_mse(ctypes.byref(d), 1, 1, 1, None, None, None, None, 1, None)
+    assert rc == L.INC_EINVAL and b"scale" in lib.inc_last_error()
+    k = L.KSDesc(dtype=L.INC_F32, scheme=L.INC_ETDRK4, batch=1, n=64, substeps=1, table=1)
+    rc = lib.inc_ks_fwd_mse(ctypes.byref(k), None, None, None, 1, None, None, 1, None, None)
+    assert rc == L.INC_EINVAL and b"loss_parts" in lib.inc_last_error()
+    rc = lib.inc_ks_adj_mse(ctypes.byref(k), 1, None, 1, 1, None, None, None, 1, None)
+    assert rc == L.INC_EINVAL and b"traj" in lib.inc_last_error()
     # successful-path state is reset by the next call
     assert lib.inc_weno_fwd(L.INC_F32, 0, None, None, 0, 16, None) == L.INC_OK
     assert lib.inc_last_error() == b""
