@@ -204,6 +204,38 @@ def extra_burgers_inc():
             "trajectory_steps_per_s": b * h / t, "ms_per_train_step": t * 1e3}
 
 
+def extra_burgers_inc_eval(b=512, n=128, h=1000):
+    """Burgers INC evaluation rollout (no gradient, as the trainer's validation
+    and test passes run it): B=512, N=128, 1000 SSP-RK3/WENO5 steps with the
+    4-layer 32-channel k=5 correction network, evaluated inside the rollout
+    kernel against the per-step path (network + solver launch per step)."""
+    import torch
+    import inc_amd
+    from inc_amd.correction import CorrectionCNN
+    dev = torch.device("cuda")
+    u0 = torch.stack([sine_state(n, 5000 + i) for i in range(b)]).float().to(dev)
+    cnn = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(2)).to(dev)
+
+    def solver():
+        return inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=0.01, resolution=n, nu=0.01, batch_size=b,
+                                                              time_scheme="SSPRK3", device="cuda"))
+
+    def fused():
+        with torch.no_grad():
+            inc_amd.simulate_rollout(cnn, solver(), u0, h, "INC")
+
+    def per_step():
+        with torch.no_grad():
+            sv, u = solver(), u0
+            for _ in range(h):
+                sv.correction_s = cnn(u)
+                u = sv.step(u)
+    tf, tp = _timed(fused, reps=3), _timed(per_step, reps=1)
+    return {"workload": f"Burgers INC evaluation rollout B={b} N={n} {h} steps SSP-RK3/WENO5 + CNN(4x32,k5), "
+                        "no gradient", "in_kernel_ms": tf * 1e3, "per_step_ms": tp * 1e3,
+            "in_kernel_trajectory_steps_per_s": b * h / tf, "per_step_trajectory_steps_per_s": b * h / tp}
+
+
 def extra_ks_inc():
     """KS INC fwd+adj with checkpointing: L=100, N=512, dt=0.25, B=1024, 64 steps, checkpoint every 8,
     correction from a 6-layer 48-channel k=7 circular CNN."""
@@ -455,6 +487,7 @@ def main():
     if not args.no_extras:
         out["other_workloads"] = {"ks_saturated": extra_ks_saturation(), "ks_mse_fused": extra_ks_mse(),
                                   "burgers_saturated": extra_burgers_saturation(), "burgers_inc": extra_burgers_inc(),
+                                  "burgers_inc_eval": extra_burgers_inc_eval(),
                                   "ks_inc_checkpointed": extra_ks_inc(), "ragged_burgers_inc": extra_ragged_inc()}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

@@ -61,6 +61,15 @@ class RaggedDesc(ctypes.Structure):
     ]
 
 
+class CnnDesc(ctypes.Structure):
+    """inc_cnn_desc (include/inc_amd.h)."""
+    _fields_ = [
+        ("layers", ctypes.c_int32), ("channels", ctypes.c_int32), ("kernel", ctypes.c_int32),
+        ("reserved", ctypes.c_int32), ("scale", ctypes.c_double),
+        ("weight", _vp * 8), ("bias", _vp * 8),
+    ]
+
+
 #: every symbol include/inc_amd.h declares, with its ctypes signature
 SIGNATURES = {
     "inc_version": (ctypes.c_int, []),
@@ -75,6 +84,8 @@ SIGNATURES = {
                                            _vp, _vp]),
     "inc_burgers_adj_mse": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                            ctypes.c_int, _vp]),
+    "inc_burgers_inc_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.POINTER(CnnDesc), _vp, _vp,
+                                           ctypes.c_int, _vp, _vp, _vp]),
     "inc_burgers_adaptive_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_double, ctypes.c_int, _vp, _vp,
                                                 _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     "inc_burgers_adaptive_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp, _vp,

@@ -154,6 +154,27 @@ class CorrectionCNN(nn.Module):
         self.convs = nn.ModuleList(layers)
         self.act = nn.GELU()
 
+    def fused_desc(self, dtype):
+        """(inc_cnn_desc, tensors to keep alive) for evaluating this network
+        inside a rollout kernel (inc_burgers_inc_fwd), or None when its shape
+        is not one the kernel runs (width 32 or 48, kernel size 5 or 7, 2..8
+        layers, the HIP backend)."""
+        convs = list(self.convs)
+        c = convs[0].weight.shape[0]
+        k = convs[0].k
+        if (convs[0].backend != "hip" or not 2 <= len(convs) <= 8 or c not in (32, 48) or k not in (5, 7)
+                or any(cv.k != k for cv in convs)):
+            return None
+        d = _lib.CnnDesc(layers=len(convs), channels=c, kernel=k, scale=1.0)
+        keep = []
+        for i, cv in enumerate(convs):
+            w = cv.weight.detach().to(dtype).contiguous()
+            b = cv.bias.detach().to(dtype).contiguous()
+            keep += [w, b]
+            d.weight[i] = w.data_ptr()
+            d.bias[i] = b.data_ptr()
+        return d, keep
+
     def forward(self, u, L=None):
         if self.convs[0].backend == "hip":
             params = [t for conv in self.convs for t in (conv.weight, conv.bias)]

@@ -599,6 +599,201 @@ __global__ void __launch_bounds__(kBlock) burgers_fwd_kernel(BArgs<T> a) {
   }
 }
 
+// ---- INC rollout with the correction network inside the kernel ---------------------
+// simulate_rollout(model, solver, u, steps, 'INC') (trainer/trainer_1d.py:154-183)
+// for a circular Conv1d + GELU correction network (models/models_1d/resnet.py:5-52
+// stacks such layers): before every step s = scale * CNN(u), held over the
+// step's stages, as solver.correction_s.  One block per trajectory; the
+// network runs on the block's own state in LDS, so no activation leaves the
+// CU and the whole rollout is one launch.
+constexpr int kCnnMaxLayers = 8;
+
+template <typename T>
+struct CnnArgs {
+  int layers;  // >= 2: Conv(1->C), GELU, (layers-2) x [Conv(C->C), GELU], Conv(C->1)
+  T scale;
+  const T* w[kCnnMaxLayers];  // [Co][Ci][K]
+  const T* b[kCnnMaxLayers];  // [Co] or NULL
+};
+
+// Exact (erf) GELU.
+template <typename T>
+__device__ __forceinline__ T gelu_exact(T v) {
+  return T(0.5) * v * (T(1) + erf(v * T(0.70710678118654752440)));
+}
+
+// s[p] = scale * CNN(u) at this thread's points i0 + p (every thread of the
+// block calls it; u is in us, written and synchronised by the caller).
+// Each thread computes every output channel of its own points, so a layer's
+// weights are wave-uniform (scalar cache) and its outputs accumulate in
+// registers; buf ([C][n] in LDS) holds one layer's activations, overwritten
+// in place between barriers.  The last layer's output stays in registers,
+// in the solver's layout.
+template <typename T, int C, int K, int PPT>
+__device__ __forceinline__ void cnn_eval(const CnnArgs<T>& cn, const T* us, T* buf, int n, int i0, bool own,
+                                         T* s) {
+  constexpr int P = K / 2;
+  constexpr int WN = PPT + K - 1;
+  auto at = [&](int j) {  // periodic index of i0 - P + j, |i0 - P + j| < 2n
+    int i = i0 - P + j;
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+  };
+  T acc[C][PPT];
+  {
+    T win[WN];
+#pragma unroll
+    for (int j = 0; j < WN; ++j) win[j] = own ? us[at(j)] : T(0);
+    const T* w = cn.w[0];
+#pragma unroll
+    for (int o = 0; o < C; ++o) {
+      const T bo = cn.b[0] ? cn.b[0][o] : T(0);
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        T v = bo;
+#pragma unroll
+        for (int k = 0; k < K; ++k) v += w[o * K + k] * win[p + k];
+        acc[o][p] = gelu_exact(v);
+      }
+    }
+  }
+  auto store = [&]() {
+    if (own) {
+#pragma unroll
+      for (int o = 0; o < C; ++o)
+#pragma unroll
+        for (int p = 0; p < PPT; ++p)
+          if (i0 + p < n) buf[o * n + i0 + p] = acc[o][p];
+    }
+  };
+  store();
+  __syncthreads();
+  for (int l = 1; l < cn.layers - 1; ++l) {
+    const T* w = cn.w[l];
+    const T* bl = cn.b[l];
+#pragma unroll
+    for (int o = 0; o < C; ++o)
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) acc[o][p] = bl ? bl[o] : T(0);
+    for (int c = 0; c < C; ++c) {
+      T win[WN];
+#pragma unroll
+      for (int j = 0; j < WN; ++j) win[j] = own ? buf[c * n + at(j)] : T(0);
+#pragma unroll
+      for (int o = 0; o < C; ++o)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const T wv = w[(o * C + c) * K + k];
+#pragma unroll
+          for (int p = 0; p < PPT; ++p) acc[o][p] += wv * win[p + k];
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < C; ++o)
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) acc[o][p] = gelu_exact(acc[o][p]);
+    __syncthreads();  // every thread has read the layer's input
+    store();
+    __syncthreads();
+  }
+  const T* w = cn.w[cn.layers - 1];
+  const T bl = cn.b[cn.layers - 1] ? cn.b[cn.layers - 1][0] : T(0);
+  T v[PPT];
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) v[p] = bl;
+  for (int c = 0; c < C; ++c) {
+    T win[WN];
+#pragma unroll
+    for (int j = 0; j < WN; ++j) win[j] = own ? buf[c * n + at(j)] : T(0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const T wv = w[c * K + k];
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) v[p] += wv * win[p + k];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) s[p] = cn.scale * v[p];
+}
+
+// Block = one trajectory's team (blockDim.x threads, PPT points each).
+// Writes traj[b][step] and, when corr is given, corr[b][step] = the
+// correction of that step (simulate_rollout's second result).
+template <typename T, int PPT, int C, int K>
+__global__ void __launch_bounds__(kBlock) burgers_inc_fwd_kernel(BArgs<T> a, CnnArgs<T> cn, T* corr) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ T red_v[2 * kMaxWaves];
+  __shared__ int red_i[kMaxWaves];
+  TeamCtx<T> c;
+  c.team = blockDim.x;
+  c.lt = threadIdx.x;
+  c.b = blockIdx.x;
+  c.active = true;
+  c.wave_in_block = threadIdx.x / kWave;
+  c.first_wave = 0;
+  c.i0 = c.lt * PPT;
+  c.n = a.n;
+  const int n = a.n;
+  T* us = reinterpret_cast<T*>(smem_raw);
+  T* Fs = us + n;
+  T* buf = Fs + n;
+  const bool own = c.i0 < n;
+  const int ns = a.scheme == INC_SSPRK3 ? 3 : 1;
+  const T nu = a.nu_ptr ? a.nu_ptr[c.b] : a.nu;
+  const T t0 = a.t_ptr ? a.t_ptr[c.b] : a.t0;
+  const size_t row = size_t(c.b) * n;
+  T u0[PPT], uc[PPT], sv[PPT], xv[PPT], R[PPT];
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    const int i = c.i0 + p;
+    const bool ok = own && i < n;
+    u0[p] = ok ? a.u[row + i] : T(0);
+    xv[p] = (ok && a.source == INC_SRC_LEARNING) ? a.x[i] : T(0);
+  }
+  for (int step = 0; step < a.steps; ++step) {
+    // correction of this step from the current state
+#pragma unroll
+    for (int p = 0; p < PPT; ++p)
+      if (c.i0 + p < n) us[c.i0 + p] = u0[p];
+    __syncthreads();
+    cnn_eval<T, C, K, PPT>(cn, us, buf, n, c.i0, own, sv);
+    __syncthreads();  // us and buf are free again
+    const size_t at = (size_t(c.b) * a.steps + step) * n;
+    if (corr && own) {
+#pragma unroll
+      for (int p = 0; p < PPT; ++p)
+        if (c.i0 + p < n) corr[at + c.i0 + p] = sv[p];
+    }
+    // one step of the solver with the correction held over its stages
+    const T ts0 = t0 + T(step) * a.dt;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) uc[p] = u0[p];
+    for (int st = 0; st < ns; ++st) {
+      const T ts = st == 0 ? ts0 : (st == 1 ? ts0 + a.dt : ts0 + T(0.5) * a.dt);
+      rhs_eval<T, PPT, INC_WENO>(a, c, uc, sv, xv, ts, nu, a.parts, us, Fs, red_v, red_i, R);
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        T v = uc[p] + a.dt * R[p];
+        if (ns == 1 || st == 0) uc[p] = v;
+        else if (st == 1) uc[p] = T(0.75) * u0[p] + T(0.25) * v;
+        else uc[p] = T(1.0 / 3.0) * u0[p] + T(2.0 / 3.0) * v;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) u0[p] = uc[p];
+    if (a.traj && own) {
+#pragma unroll
+      for (int p = 0; p < PPT; ++p)
+        if (c.i0 + p < n) a.traj[at + c.i0 + p] = u0[p];
+    }
+  }
+  if (own) {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p)
+      if (c.i0 + p < n) a.out[row + c.i0 + p] = u0[p];
+  }
+}
+
 // Right-hand side parts of one state (calc_nonlinear / calc_linear / source).
 template <typename T, int PPT, int CONV>
 __global__ void __launch_bounds__(kBlock) burgers_rhs_kernel(BArgs<T> a) {
@@ -1308,6 +1503,67 @@ int dispatch(Kind kind, const inc_burgers_desc* d, int parts, const void* u, con
   return INC_EINVAL;
 }
 
+template <typename T, int PPT, int C>
+void* inc_fwd_kernel_k(int k) {
+  return k == 5 ? reinterpret_cast<void*>(&burgers_inc_fwd_kernel<T, PPT, C, 5>)
+                : reinterpret_cast<void*>(&burgers_inc_fwd_kernel<T, PPT, C, 7>);
+}
+
+template <typename T>
+int inc_fwd_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void* u, void* out, int steps,
+                   void* traj, void* corr, void* stream) {
+  BArgs<T> a;
+  int rc = fill_args<T>(d, a);
+  if (rc) return rc;
+  INC_REQUIRE(d->convection == INC_WENO, INC_EUNSUPPORTED, "the in-kernel correction network runs WENO convection");
+  INC_REQUIRE(cd->layers >= 2 && cd->layers <= kCnnMaxLayers, INC_EINVAL, "correction network needs 2..%d layers",
+              kCnnMaxLayers);
+  INC_REQUIRE((cd->channels == 32 || cd->channels == 48) && (cd->kernel == 5 || cd->kernel == 7), INC_EUNSUPPORTED,
+              "in-kernel correction network: width 32 or 48, kernel size 5 or 7 (got %d, %d)", cd->channels,
+              cd->kernel);
+  INC_REQUIRE(cd->kernel <= d->n, INC_EINVAL, "kernel size exceeds the resolution");
+  // float: up to 2 points per thread; double: 1 (two would spill registers)
+  constexpr int kPptMax = std::is_same<T, float>::value ? 2 : 1;
+  int team = 0;
+  const int ppt = pick_ppt(d->n, d->team, &team, d->batch, kPptMax);
+  INC_REQUIRE(ppt >= 1 && ppt <= kPptMax, INC_EUNSUPPORTED,
+              "in-kernel correction network: n=%d needs at most %d point(s) per thread of a 256-thread team", d->n,
+              kPptMax);
+  if (team < kWave) team = kWave;
+  const size_t lds = size_t(2 + cd->channels) * d->n * sizeof(T);
+  INC_REQUIRE(lds <= 150 * 1024, INC_EUNSUPPORTED, "in-kernel correction network: n=%d, width %d need %zu bytes of LDS",
+              d->n, cd->channels, lds);
+  CnnArgs<T> cn{};
+  cn.layers = cd->layers;
+  cn.scale = T(cd->scale);
+  for (int l = 0; l < cd->layers; ++l) {
+    INC_REQUIRE(cd->weight[l] != nullptr, INC_EINVAL, "correction network layer %d has no weight", l);
+    cn.w[l] = static_cast<const T*>(cd->weight[l]);
+    cn.b[l] = static_cast<const T*>(cd->bias[l]);
+  }
+  a.team = team;
+  a.steps = steps;
+  a.parts = INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE;
+  a.u = static_cast<const T*>(u);
+  a.s = nullptr;
+  a.out = static_cast<T*>(out);
+  a.traj = static_cast<T*>(traj);
+  if (a.batch == 0) return INC_OK;
+  void* fn;
+  if constexpr (kPptMax == 2) {
+    if (ppt == 2)
+      fn = cd->channels == 32 ? inc_fwd_kernel_k<T, 2, 32>(cd->kernel) : inc_fwd_kernel_k<T, 2, 48>(cd->kernel);
+    else
+      fn = cd->channels == 32 ? inc_fwd_kernel_k<T, 1, 32>(cd->kernel) : inc_fwd_kernel_k<T, 1, 48>(cd->kernel);
+  } else {
+    fn = cd->channels == 32 ? inc_fwd_kernel_k<T, 1, 32>(cd->kernel) : inc_fwd_kernel_k<T, 1, 48>(cd->kernel);
+  }
+  T* cp = static_cast<T*>(corr);
+  void* args[] = {&a, &cn, &cp};
+  INC_HIP(hipLaunchKernel(fn, dim3(a.batch), dim3(team), args, lds, static_cast<hipStream_t>(stream)));
+  return INC_OK;
+}
+
 }  // namespace
 }  // namespace inc
 
@@ -1399,6 +1655,17 @@ int inc_burgers_adj_mse(const inc_burgers_desc* d, const void* stages, const voi
   return inc::dispatch(Kind::Adj, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE, nullptr,
                        nullptr, g_u, steps, const_cast<void*>(traj), const_cast<void*>(stages), g_traj, g_s,
                        stream, inc::Adaptive{0.0, 0, nullptr, nullptr}, m);
+}
+
+int inc_burgers_inc_fwd(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* u, void* u_out, int steps,
+                        void* traj, void* corr, void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(d != nullptr && cnn != nullptr, INC_EINVAL, "null descriptor");
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  if (d->dtype == INC_F32) return inc::inc_fwd_launch<float>(d, cnn, u, u_out, steps, traj, corr, stream);
+  if (d->dtype == INC_F64) return inc::inc_fwd_launch<double>(d, cnn, u, u_out, steps, traj, corr, stream);
+  inc::set_error("unsupported dtype %d", d->dtype);
+  return INC_EINVAL;
 }
 
 int inc_burgers_adaptive_fwd(const inc_burgers_desc* d, double cfl, int max_sub, const void* u, const void* s,

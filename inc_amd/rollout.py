@@ -7,6 +7,9 @@ as the reference does.  Two MI355X additions:
 * with ``correction_term=None`` and an inc_amd solver on a fixed time step,
   the whole rollout is ONE kernel launch (state resident in registers/LDS,
   every step's output written once) and its gradient ONE adjoint launch;
+* with ``correction_term='INC'``, no gradient wanted (validation / test
+  rollouts), a Burgers solver and a CorrectionCNN the kernel runs, the
+  correction network is evaluated inside that one launch as well;
 * ``checkpoint_every=k`` recomputes the INC rollout in segments of k steps
   during the backward pass (segment-start states are the only activations
   kept), for long unrolled horizons.
@@ -18,6 +21,7 @@ fused into the kernels.
 import torch
 from torch.utils.checkpoint import checkpoint
 
+from .correction import CorrectionCNN
 from .solver_1d import BurgersSolverTorch, KSSolverTorch
 
 
@@ -35,6 +39,23 @@ def _fusable(solver, correction_term):
     return False
 
 
+def _fusable_inc(model, solver, u):
+    """An INC rollout the Burgers kernel runs with the correction network
+    inside (no gradient wanted, fixed time step, WENO, a CorrectionCNN of a
+    supported shape, within the kernel's size limits)."""
+    if not isinstance(solver, BurgersSolverTorch) or not isinstance(model, CorrectionCNN):
+        return False
+    if torch.is_grad_enabled() and (u.requires_grad or any(p.requires_grad for p in model.parameters())):
+        return False
+    if solver.adaptive_CFL is not None or solver.param.convection_term != "WENO" or not u.is_cuda:
+        return False
+    n = u.shape[-1]
+    if u.dtype not in (torch.float32, torch.float64) or n > (512 if u.dtype == torch.float32 else 256):
+        return False
+    fd = model.fused_desc(u.dtype)
+    return fd is not None and (fd[0].channels + 2) * n * u.element_size() <= 150 * 1024 and fd[0].kernel <= n
+
+
 def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, checkpoint_every=None):
     """Unroll ``mstep`` solver steps (trainer_1d.py:154-183)."""
     b, d = u_initial.shape
@@ -44,6 +65,8 @@ def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, c
     if mstep <= 0:
         empty = u_initial.new_zeros(b, 0, d)
         return empty, (empty.clone() if correction_term else None)
+    if correction_term == "INC" and _fusable_inc(model, solver, u_initial):
+        return solver.rollout_inc(model, u_initial, mstep)
     if _fusable(solver, correction_term):
         if isinstance(solver, KSSolverTorch):
             return solver.rollout(u_initial, mstep, L), None

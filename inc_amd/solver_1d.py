@@ -494,6 +494,34 @@ class BurgersSolverTorch:
             self.t += self.dt
         return traj
 
+    def rollout_inc(self, model, u, mstep):
+        """simulate_rollout(model, self, u, mstep, 'INC') (trainer_1d.py:154-183)
+        with the correction network evaluated inside the rollout kernel
+        (inc_burgers_inc_fwd): returns (results, corrections), both
+        [B, mstep, N].  No gradient (validation / test rollouts); ``model``
+        must be a CorrectionCNN whose shape the kernel runs
+        (CorrectionCNN.fused_desc)."""
+        L.require_gpu(u)
+        if self.adaptive_CFL is not None:
+            raise ValueError("rollout_inc runs a fixed time step; adaptive_CFL is set")
+        cnn = model.fused_desc(u.dtype)
+        if cnn is None:
+            raise ValueError("the correction network's shape is not one the in-kernel path runs")
+        cdesc, keep_w = cnn
+        u = u.detach().contiguous()
+        b, n = u.shape
+        desc, keep = self._desc(u, self.dt, self.t)
+        out = torch.empty_like(u)
+        traj = u.new_empty(b, mstep, n)
+        corr = u.new_empty(b, mstep, n)
+        rc = L.load().inc_burgers_inc_fwd(ctypes.byref(desc), ctypes.byref(cdesc), L.ptr(u), L.ptr(out), mstep,
+                                          L.ptr(traj), L.ptr(corr), L.stream())
+        L.check(rc, "BurgersSolverTorch.rollout_inc")
+        for _ in range(mstep):
+            self.t += self.dt
+        self.correction_s = corr[:, -1] if mstep > 0 else None
+        return traj, corr
+
     def rollout_mse(self, u, target):
         """Fused rollout over ``target.shape[1]`` steps with the MSE to
         ``target`` [B, steps, N] computed in the kernels: returns (loss,

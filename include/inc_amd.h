@@ -145,6 +145,36 @@ int inc_burgers_adj_mse(const inc_burgers_desc* d, const void* stages,
                         void* stream);
 
 /*
+ * INC rollout with the correction network inside the kernel:
+ * simulate_rollout(model, solver, u, steps, 'INC') (trainer/trainer_1d.py:154-183)
+ * for a stack of circular Conv1d layers with GELU between them (the
+ * reference's correction networks stack such convolutions,
+ * models/models_1d/resnet.py:5-52):
+ *   s = scale * conv_{L-1}(gelu(... gelu(conv_0(u)) ...)),
+ *   conv_0: 1 -> C channels, conv_1..L-2: C -> C, conv_{L-1}: C -> 1,
+ * evaluated from the current state before every step and held over the
+ * step's stages (solver.correction_s).  One launch for the whole rollout; no
+ * gradient (validation / test rollouts).
+ *   weight[l]  [Co][Ci][K] of the state's element type; bias[l] [Co] or NULL
+ *   traj       [batch, steps, n] states after each step, or NULL
+ *   corr       [batch, steps, n] corrections of each step, or NULL
+ * Supported: WENO convection, C = 32 or 48, K = 5 or 7, 2..8 layers, n up to
+ * 512 (float) or 256 (double) (LDS: (C + 2) * n elements per block).
+ */
+typedef struct inc_cnn_desc {
+  int32_t layers;
+  int32_t channels;
+  int32_t kernel;
+  int32_t reserved;
+  double scale;
+  const void* weight[8];
+  const void* bias[8];
+} inc_cnn_desc;
+
+int inc_burgers_inc_fwd(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* u,
+                        void* u_out, int steps, void* traj, void* corr, void* stream);
+
+/*
  * Adaptive sub-stepping per trajectory (BurgersSolverTorch.step_adaptive,
  * solver_1d.py:148-179, which sub-steps the whole batch to
  * dt_j = min(remaining, CFL * dx / max|u|) with max|u| read back to the host
