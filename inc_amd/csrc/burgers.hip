@@ -715,10 +715,161 @@ __device__ __forceinline__ void cnn_eval(const CnnArgs<T>& cn, const T* us, T* b
   for (int p = 0; p < PPT; ++p) s[p] = cn.scale * v[p];
 }
 
+// ---- the same network with its hidden layers on the matrix cores (float) ----------
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// One hidden layer, C -> C, in place on act (position-major: act[i][c], row
+// stride C + 4), weights ws[o][k*C + c] (row stride C*K + 4) staged in LDS.
+// D[o][i] = sum_r W[o][r] X[r][i] with r = k*C + c, as v_mfma_f32_16x16x4f32
+// in chunks of 16 r (one tap per chunk: C % 16 == 0); lane (lr, q) feeds
+// A = ws[o][16s+4q .. +3] and B = act[(i+k-P) mod n][c0+4q .. +3], one
+// 16-byte LDS read each, and holds D[4q + r][lr].  Wavefront w computes the
+// 16-position tiles w*PW .. w*PW + PW - 1 (positions past n are computed
+// and dropped).  Reads of the next chunk are issued before the current
+// chunk's MFMAs (two register sets, unconditional reads: the read after
+// the last chunk lands inside LDS and is unused).  bias + GELU on the way
+// back; the caller synchronises before the next reader.
+template <int C, int K, int PW>
+__device__ __forceinline__ void cnn_hidden_mfma(const float* ws, const float* bias, float* act, int n) {
+  constexpr int CT = C / 16, P = K / 2, CS = C + 4, RS = C * K + 4, NCH = C * K / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, q = lane >> 4;
+  int ib[PW];
+#pragma unroll
+  for (int t = 0; t < PW; ++t) ib[t] = (wave * PW + t) * 16 + lr;
+  auto row = [&](int i) {  // (i) mod n for i in (-n, 3n)
+    i = i < 0 ? i + n : i;
+    i = i >= n ? i - n : i;
+    return i >= n ? i - n : i;
+  };
+  v4f acc[CT][PW];
+#pragma unroll
+  for (int m = 0; m < CT; ++m)
+#pragma unroll
+    for (int t = 0; t < PW; ++t) acc[m][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  int k = 0, c0 = 0;
+  const float* wl = ws + lr * RS + 4 * q;
+  auto load = [&](int s, v4f* fa, v4f* fb) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < CT; ++m) fa[m] = *reinterpret_cast<const v4f*>(wl + m * 16 * RS + 16 * s);
+#pragma unroll
+    for (int t = 0; t < PW; ++t)
+      fb[t] = *reinterpret_cast<const v4f*>(act + row(ib[t] + k - P) * CS + c0 + 4 * q);
+  };
+  auto next = [&]() __attribute__((always_inline)) {
+    c0 += 16;
+    if (c0 == C) {
+      c0 = 0;
+      ++k;
+    }
+  };
+  auto mma = [&](const v4f* fa, const v4f* fb) __attribute__((always_inline)) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int m = 0; m < CT; ++m)
+#pragma unroll
+        for (int t = 0; t < PW; ++t)
+          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[m][j], fb[t][j], acc[m][t], 0, 0, 0);
+  };
+  v4f a0[CT], b0[PW], a1[CT], b1[PW];
+  load(0, a0, b0);
+  for (int s = 0; s < NCH; s += 2) {
+    next();
+    load(s + 1, a1, b1);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(a0, b0);
+    __builtin_amdgcn_sched_barrier(0);
+    if (s + 1 >= NCH) break;
+    next();
+    load(s + 2, a0, b0);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(a1, b1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  __syncthreads();  // every wavefront has read the layer's input
+#pragma unroll
+  for (int m = 0; m < CT; ++m) {
+    const int o = m * 16 + 4 * q;
+    const v4f bo = bias ? *reinterpret_cast<const v4f*>(bias + o) : v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < PW; ++t)
+      if (ib[t] < n) {
+        v4f v = acc[m][t] + bo;
+        *reinterpret_cast<v4f*>(act + ib[t] * CS + o) =
+            v4f{gelu_exact(v[0]), gelu_exact(v[1]), gelu_exact(v[2]), gelu_exact(v[3])};
+      }
+  }
+}
+
+// s = scale * CNN(u) with the activations position-major in act and the
+// hidden layers' weights staged in ws (layer l at ws + (l-1) * C * (C*K+4)).
+template <int C, int K, int PPT, int PW>
+__device__ __forceinline__ void cnn_eval_mfma(const CnnArgs<float>& cn, const float* ws, const float* us, float* act,
+                                              int n, int i0, bool own, float* s) {
+  constexpr int P = K / 2, WN = PPT + K - 1, CS = C + 4;
+  auto at = [&](int j) {
+    int i = i0 - P + j;
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+  };
+  {
+    float win[WN];
+#pragma unroll
+    for (int j = 0; j < WN; ++j) win[j] = own ? us[at(j)] : 0.f;
+    const float* w = cn.w[0];
+#pragma unroll
+    for (int o4 = 0; o4 < C / 4; ++o4) {
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        float v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int o = 4 * o4 + e;
+          float a = cn.b[0] ? cn.b[0][o] : 0.f;
+#pragma unroll
+          for (int k = 0; k < K; ++k) a += w[o * K + k] * win[p + k];
+          v[e] = gelu_exact(a);
+        }
+        if (own && i0 + p < n)
+          *reinterpret_cast<v4f*>(act + (i0 + p) * CS + 4 * o4) = v4f{v[0], v[1], v[2], v[3]};
+      }
+    }
+  }
+  __syncthreads();
+  for (int l = 1; l < cn.layers - 1; ++l) {
+    cnn_hidden_mfma<C, K, PW>(ws + size_t(l - 1) * C * (C * K + 4), cn.b[l], act, n);
+    __syncthreads();
+  }
+  const float* w = cn.w[cn.layers - 1];
+  const float bl = cn.b[cn.layers - 1] ? cn.b[cn.layers - 1][0] : 0.f;
+  float v[PPT];
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) v[p] = bl;
+#pragma unroll
+  for (int j = 0; j < WN; ++j) {
+    const float* xr = act + (own ? at(j) : 0) * CS;
+#pragma unroll
+    for (int c4 = 0; c4 < C / 4; ++c4) {
+      const v4f x = *reinterpret_cast<const v4f*>(xr + 4 * c4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+          const int k = j - p;
+          if (k >= 0 && k < K) v[p] += w[(4 * c4 + e) * K + k] * x[e];
+        }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) s[p] = cn.scale * v[p];
+}
+
 // Block = one trajectory's team (blockDim.x threads, PPT points each).
 // Writes traj[b][step] and, when corr is given, corr[b][step] = the
 // correction of that step (simulate_rollout's second result).
-template <typename T, int PPT, int C, int K>
+// PW > 0 (float): hidden layers on the matrix cores with PW 16-position
+// tiles per wavefront, their weights staged in LDS once for all steps.
+template <typename T, int PPT, int C, int K, int PW>
 __global__ void __launch_bounds__(kBlock) burgers_inc_fwd_kernel(BArgs<T> a, CnnArgs<T> cn, T* corr) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ T red_v[2 * kMaxWaves];
@@ -735,11 +886,25 @@ __global__ void __launch_bounds__(kBlock) burgers_inc_fwd_kernel(BArgs<T> a, Cnn
   const int n = a.n;
   T* us = reinterpret_cast<T*>(smem_raw);
   T* Fs = us + n;
-  T* buf = Fs + n;
+  T* buf = Fs + n;  // PW == 0: [C][n]; PW > 0: staged hidden weights, then act [n][C + 4]
   const bool own = c.i0 < n;
   const int ns = a.scheme == INC_SSPRK3 ? 3 : 1;
   const T nu = a.nu_ptr ? a.nu_ptr[c.b] : a.nu;
   const T t0 = a.t_ptr ? a.t_ptr[c.b] : a.t0;
+  T* act = buf;
+  if constexpr (PW > 0) {
+    // hidden layers' weights, once for the whole rollout: ws[l-1][o][k*C + c]
+    constexpr int RS = C * K + 4;
+    const int nh = cn.layers - 2;
+    for (int e = threadIdx.x; e < nh * C * C; e += blockDim.x) {
+      const int l = e / (C * C), oc = e - l * C * C, o = oc / C, cc = oc - o * C;
+      const T* src = cn.w[l + 1] + size_t(oc) * K;
+      T* dst = buf + size_t(l) * C * RS + o * RS + cc;
+#pragma unroll
+      for (int k = 0; k < K; ++k) dst[k * C] = src[k];
+    }
+    act = buf + size_t(nh) * C * RS;
+  }
   const size_t row = size_t(c.b) * n;
   T u0[PPT], uc[PPT], sv[PPT], xv[PPT], R[PPT];
 #pragma unroll
@@ -755,8 +920,9 @@ __global__ void __launch_bounds__(kBlock) burgers_inc_fwd_kernel(BArgs<T> a, Cnn
     for (int p = 0; p < PPT; ++p)
       if (c.i0 + p < n) us[c.i0 + p] = u0[p];
     __syncthreads();
-    cnn_eval<T, C, K, PPT>(cn, us, buf, n, c.i0, own, sv);
-    __syncthreads();  // us and buf are free again
+    if constexpr (PW > 0) cnn_eval_mfma<C, K, PPT, PW>(cn, buf, us, act, n, c.i0, own, sv);
+    else cnn_eval<T, C, K, PPT>(cn, us, buf, n, c.i0, own, sv);
+    __syncthreads();  // us and the activations are free again
     const size_t at = (size_t(c.b) * a.steps + step) * n;
     if (corr && own) {
 #pragma unroll
@@ -1505,8 +1671,30 @@ int dispatch(Kind kind, const inc_burgers_desc* d, int parts, const void* u, con
 
 template <typename T, int PPT, int C>
 void* inc_fwd_kernel_k(int k) {
-  return k == 5 ? reinterpret_cast<void*>(&burgers_inc_fwd_kernel<T, PPT, C, 5>)
-                : reinterpret_cast<void*>(&burgers_inc_fwd_kernel<T, PPT, C, 7>);
+  return k == 5 ? reinterpret_cast<void*>(&burgers_inc_fwd_kernel<T, PPT, C, 5, 0>)
+                : reinterpret_cast<void*>(&burgers_inc_fwd_kernel<T, PPT, C, 7, 0>);
+}
+
+// matrix-core hidden layers: PW 16-position tiles per wavefront (1, 2, 4 or 8)
+template <int PPT, int C, int K>
+void* inc_fwd_mfma_kernel_pw(int pw) {
+  switch (pw) {
+    case 1: return reinterpret_cast<void*>(&burgers_inc_fwd_kernel<float, PPT, C, K, 1>);
+    case 2: return reinterpret_cast<void*>(&burgers_inc_fwd_kernel<float, PPT, C, K, 2>);
+    case 4: return reinterpret_cast<void*>(&burgers_inc_fwd_kernel<float, PPT, C, K, 4>);
+    default: return reinterpret_cast<void*>(&burgers_inc_fwd_kernel<float, PPT, C, K, 8>);
+  }
+}
+
+// The instances that keep every value in registers (the others spill, and
+// their shapes take the vector-ALU network): one point per thread, any
+// width and kernel size except 48 x 7 at 8 tiles; two points per thread,
+// width 32 and kernel size 5.  nullptr otherwise.
+void* inc_fwd_mfma_kernel(int ppt, int c, int k, int pw) {
+  if (ppt == 2) return (c == 32 && k == 5) ? inc_fwd_mfma_kernel_pw<2, 32, 5>(pw) : nullptr;
+  if (c == 32) return k == 5 ? inc_fwd_mfma_kernel_pw<1, 32, 5>(pw) : inc_fwd_mfma_kernel_pw<1, 32, 7>(pw);
+  if (k == 5) return inc_fwd_mfma_kernel_pw<1, 48, 5>(pw);
+  return pw == 8 ? nullptr : inc_fwd_mfma_kernel_pw<1, 48, 7>(pw);
 }
 
 template <typename T>
@@ -1530,7 +1718,22 @@ int inc_fwd_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void
               "in-kernel correction network: n=%d needs at most %d point(s) per thread of a 256-thread team", d->n,
               kPptMax);
   if (team < kWave) team = kWave;
-  const size_t lds = size_t(2 + cd->channels) * d->n * sizeof(T);
+  size_t lds = size_t(2 + cd->channels) * d->n * sizeof(T);
+  // float: hidden layers on the matrix cores when their weights and the
+  // position-major activations fit in LDS
+  int pw = 0;
+  if (std::is_same<T, float>::value && cd->layers > 2) {
+    const int C = cd->channels, K = cd->kernel;
+    const int tiles = (d->n + 15) / 16, waves = team / kWave;
+    int need = (tiles + waves - 1) / waves;
+    need = need <= 1 ? 1 : (need <= 2 ? 2 : (need <= 4 ? 4 : 8));
+    const size_t lm = (size_t(2) * d->n + size_t(cd->layers - 2) * C * (C * K + 4) + size_t(d->n) * (C + 4)) *
+                      sizeof(float);
+    if (need <= 8 && lm <= 150 * 1024 && inc_fwd_mfma_kernel(ppt, C, K, need) != nullptr) {
+      pw = need;
+      lds = lm;
+    }
+  }
   INC_REQUIRE(lds <= 150 * 1024, INC_EUNSUPPORTED, "in-kernel correction network: n=%d, width %d need %zu bytes of LDS",
               d->n, cd->channels, lds);
   CnnArgs<T> cn{};
@@ -1551,7 +1754,9 @@ int inc_fwd_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void
   if (a.batch == 0) return INC_OK;
   void* fn;
   if constexpr (kPptMax == 2) {
-    if (ppt == 2)
+    if (pw > 0)
+      fn = inc_fwd_mfma_kernel(ppt, cd->channels, cd->kernel, pw);
+    else if (ppt == 2)
       fn = cd->channels == 32 ? inc_fwd_kernel_k<T, 2, 32>(cd->kernel) : inc_fwd_kernel_k<T, 2, 48>(cd->kernel);
     else
       fn = cd->channels == 32 ? inc_fwd_kernel_k<T, 1, 32>(cd->kernel) : inc_fwd_kernel_k<T, 1, 48>(cd->kernel);
