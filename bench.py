@@ -230,7 +230,7 @@ def extra_burgers_inc_eval(b=512, n=128, h=1000):
             for _ in range(h):
                 sv.correction_s = cnn(u)
                 u = sv.step(u)
-    tf, tp = _timed(fused, reps=3), _timed(per_step, reps=1)
+    tf, tp = _timed(fused, reps=3), _timed(per_step, reps=3)
     return {"workload": f"Burgers INC evaluation rollout B={b} N={n} {h} steps SSP-RK3/WENO5 + CNN(4x32,k5), "
                         "no gradient", "in_kernel_ms": tf * 1e3, "per_step_ms": tp * 1e3,
             "in_kernel_trajectory_steps_per_s": b * h / tf, "per_step_trajectory_steps_per_s": b * h / tp}
