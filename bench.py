@@ -179,7 +179,10 @@ def extra_burgers_saturation(b=8192, n=256, h=200):
 
 def extra_burgers_inc():
     """Burgers INC fwd+adj: B=512, N=128, nu=0.01, dt=0.01, 16 steps, SSP-RK3 + WENO5,
-    correction from a 4-layer 32-channel k=5 circular CNN, MSE to a nu=0.012 target."""
+    correction from a 4-layer 32-channel k=5 circular CNN, MSE to a nu=0.012 target.
+    The network runs inside the rollout and adjoint kernels (the default path);
+    the per-step path (convolution kernels and one solver launch per step) is
+    timed beside it."""
     import torch
     import inc_amd
     from inc_amd.correction import CorrectionCNN
@@ -192,16 +195,18 @@ def extra_burgers_inc():
                                                                     time_scheme="SSPRK3", device="cuda"))
         target = tgt_solver.rollout(u0, h)
 
-    def train_step():
+    def train_step(fused):
         sv = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=0.01, resolution=n, nu=0.01, batch_size=b,
                                                             time_scheme="SSPRK3", device="cuda"))
-        res, _ = inc_amd.simulate_rollout(cnn, sv, u0, h, "INC")
+        res, _ = inc_amd.simulate_rollout(cnn, sv, u0, h, "INC", fused=fused)
         loss = ((res - target) ** 2).mean()
         loss.backward()
         cnn.zero_grad(set_to_none=True)
-    t = _timed(train_step)
+    t = _timed(lambda: train_step(True))
+    tp = _timed(lambda: train_step(False))
     return {"workload": "Burgers INC fwd+adj B=512 N=128 16 steps SSP-RK3/WENO5 + CNN(4x32,k5)",
-            "trajectory_steps_per_s": b * h / t, "ms_per_train_step": t * 1e3}
+            "trajectory_steps_per_s": b * h / t, "ms_per_train_step": t * 1e3,
+            "per_step_path_ms_per_train_step": tp * 1e3, "fused_over_per_step": tp / t}
 
 
 def extra_burgers_inc_eval(b=512, n=128, h=1000):

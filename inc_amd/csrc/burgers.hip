@@ -934,6 +934,12 @@ __global__ void __launch_bounds__(kBlock) burgers_inc_fwd_kernel(BArgs<T> a, Cnn
 #pragma unroll
     for (int p = 0; p < PPT; ++p) uc[p] = u0[p];
     for (int st = 0; st < ns; ++st) {
+      if (a.stages && own) {  // stage states for the training adjoint: [b][step][stage][n]
+        T* dst = a.stages + ((size_t(c.b) * a.steps + step) * ns + st) * n;
+#pragma unroll
+        for (int p = 0; p < PPT; ++p)
+          if (c.i0 + p < n) dst[c.i0 + p] = uc[p];
+      }
       const T ts = st == 0 ? ts0 : (st == 1 ? ts0 + a.dt : ts0 + T(0.5) * a.dt);
       rhs_eval<T, PPT, INC_WENO>(a, c, uc, sv, xv, ts, nu, a.parts, us, Fs, red_v, red_i, R);
       __syncthreads();
@@ -957,6 +963,435 @@ __global__ void __launch_bounds__(kBlock) burgers_inc_fwd_kernel(BArgs<T> a, Cnn
 #pragma unroll
     for (int p = 0; p < PPT; ++p)
       if (c.i0 + p < n) a.out[row + c.i0 + p] = u0[p];
+  }
+}
+
+// ---- INC training rollout: the network's transpose inside the adjoint kernel ----------
+// Gradient of simulate_rollout(model, solver, u0, steps, 'INC')
+// (trainer/trainer_1d.py:154-183) with respect to u0 and the correction
+// network's weights, for a circular Conv1d + GELU network of width C
+// (float).  One block of 256 threads per trajectory runs backwards in time.
+// Per step: the solver step's transpose from its stored stage states gives
+// the cotangents of the step's input state and of its correction s; the
+// network is evaluated again at that input state (stage 0) with every
+// layer's pre-activation kept in LDS; its transpose adds the state's
+// cotangent through s and the weight gradients.  The hidden layers run on
+// the matrix cores; their weight gradients accumulate over all steps in MFMA
+// tiles held in registers, the first and last layers' and the biases' in
+// per-thread partial sums, and each trajectory writes its own gradient row.
+constexpr int kTrainBlock = 256;
+
+// d/dv of the exact (erf) GELU: Phi(v) + v phi(v).
+__device__ __forceinline__ float gelu_grad(float v) {
+  return 0.5f * (1.f + erff(v * 0.70710678118654752440f)) + v * 0.39894228040143267794f * expf(-0.5f * v * v);
+}
+
+// One wavefront's part of a C -> C circular convolution of `in` (position-
+// major, row stride C + 4) on the matrix cores: acc[m][t] holds
+// D[16m + 4q + r][16 (wave*PW + t) + lr] (lane (lr, q), r = 0..3).
+// TRANS == false: D[o][i] = sum_{k,c} W[o][c][k] in[i + k - P][c];
+// TRANS == true (the input gradient): D[c][i] = sum_{k,o} W[o][c][k]
+// in[i - k + P][o], i.e. the flipped transposed kernel, whose A operand is
+// read from the same staged weights ws[o][k*C + c] (row stride C*K + 4)
+// with four scalar reads per chunk.  Positions past n are computed from
+// clamped rows and dropped by the caller.
+template <int C, int K, int PW, bool TRANS>
+__device__ __forceinline__ void conv_tiles(const float* ws, const float* in, int n, v4f (&acc)[C / 16][PW]) {
+  constexpr int CT = C / 16, P = K / 2, CS = C + 4, RS = C * K + 4, NCH = C * K / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, q = lane >> 4;
+  int rb[PW];
+#pragma unroll
+  for (int t = 0; t < PW; ++t) rb[t] = min((wave * PW + t) * 16 + lr, n - 1) - P;
+  auto row = [&](int i) { return i < 0 ? i + n : (i >= n ? i - n : i); };  // i in (-n, 2n)
+#pragma unroll
+  for (int m = 0; m < CT; ++m)
+#pragma unroll
+    for (int t = 0; t < PW; ++t) acc[m][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  int k = 0, c0 = 0;
+  auto load = [&](v4f* fa, v4f* fb) __attribute__((always_inline)) {
+    const int kw = min(k, K - 1);  // the read after the last chunk stays inside the weights (unused)
+    if constexpr (!TRANS) {
+      const float* wl = ws + lr * RS + kw * C + c0 + 4 * q;
+#pragma unroll
+      for (int m = 0; m < CT; ++m) fa[m] = *reinterpret_cast<const v4f*>(wl + m * 16 * RS);
+    } else {
+      const float* wl = ws + (c0 + 4 * q) * RS + (K - 1 - kw) * C + lr;
+#pragma unroll
+      for (int m = 0; m < CT; ++m)
+        fa[m] = v4f{wl[m * 16], wl[RS + m * 16], wl[2 * RS + m * 16], wl[3 * RS + m * 16]};
+    }
+#pragma unroll
+    for (int t = 0; t < PW; ++t) fb[t] = *reinterpret_cast<const v4f*>(in + row(rb[t] + k) * CS + c0 + 4 * q);
+  };
+  auto next = [&]() __attribute__((always_inline)) {
+    c0 += 16;
+    if (c0 == C) {
+      c0 = 0;
+      ++k;
+    }
+  };
+  auto mma = [&](const v4f* fa, const v4f* fb) __attribute__((always_inline)) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int m = 0; m < CT; ++m)
+#pragma unroll
+        for (int t = 0; t < PW; ++t)
+          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[m][j], fb[t][j], acc[m][t], 0, 0, 0);
+  };
+  // two register sets in turn, the next chunk's reads issued before the
+  // current chunk's MFMAs; the reads are unconditional (a conditional read
+  // makes every MFMA wait for the reads in flight); the one after the last
+  // chunk reads valid rows and weights and is unused
+  v4f a0[CT], b0[PW], a1[CT], b1[PW];
+  load(a0, b0);
+  for (int s = 0; s < NCH; s += 2) {
+    next();
+    load(a1, b1);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(a0, b0);
+    __builtin_amdgcn_sched_barrier(0);
+    if (s + 1 >= NCH) break;
+    next();
+    load(a0, b0);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(a1, b1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// This wavefront's tiles of one hidden layer's weight gradient, added to
+// acc: dW[o][c][k] += sum_j gy[j][o] x[j + k - P][c] (gy, x position-major,
+// row stride C + 4).  The layer's (C/16) x (C*K/16) tiles of D[o][k*C + c]
+// go round the four wavefronts (tile wave + 4t); lane (lr, q) holds
+// D[16m + 4q + r][16ct + lr].  The reduction runs over positions, four per
+// MFMA, all tiles of the wavefront interleaved.
+template <int C, int K, int TPW>
+__device__ __forceinline__ void wgrad_tiles(const float* gy, const float* x, int n, v4f (&acc)[TPW]) {
+  constexpr int P = K / 2, CS = C + 4, NCOL = C * K / 16, TT = (C / 16) * NCOL;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, q = lane >> 4;
+  auto row = [&](int i) { return i < 0 ? i + n : (i >= n ? i - n : i); };
+  for (int j0 = 0; j0 < n; j0 += 4) {
+    const int j = j0 + q, jc = min(j, n - 1);
+    const float mask = j < n ? 1.f : 0.f;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      if (wave + 4 * t >= TT) continue;  // wave-uniform
+      const int id = wave + 4 * t, m = id / NCOL, col = (id - m * NCOL) * 16, k = col / C;
+      const float av = gy[jc * CS + m * 16 + lr] * mask;
+      const int jr = row(jc + k - P);
+      const float bv = x[jr * CS + (col - k * C) + lr];
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t], 0, 0, 0);
+    }
+  }
+}
+
+template <int C, int K, int NH, int PW>
+__global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<float> a, CnnArgs<float> cn,
+                                                                      const float* __restrict__ g_corr,
+                                                                      float* __restrict__ g_w) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ float red_v[2 * kMaxWaves];
+  __shared__ int red_i[kMaxWaves];
+  constexpr int P = K / 2, CS = C + 4, RS = C * K + 4, Q = C / 4;
+  constexpr int NCOL = C * K / 16, TT = (C / 16) * NCOL, TPW = (TT + 3) / 4;
+  static_assert(kTrainBlock % Q == 0, "a thread's channel quad must not change between items");
+  // per-thread partial sums (channel quad fixed per thread): first layer
+  // dW0[4][K], db0[4]; last layer dWL[4][K], dbL; hidden biases db[NH][4]
+  constexpr int V0 = 4 * K + 4, VL = 4 * K + 1, V = V0 + VL + 4 * NH;
+  const int n = a.n, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  TeamCtx<float> c;
+  c.team = kTrainBlock;
+  c.lt = tid;
+  c.b = blockIdx.x;
+  c.active = true;
+  c.wave_in_block = wave;
+  c.first_wave = 0;
+  c.i0 = tid;
+  c.n = n;
+  const bool own = tid < n;
+  float* ws = reinterpret_cast<float*>(smem_raw);  // [NH][C][RS]
+  float* Y = ws + NH * C * RS;                      // [NH + 1][n][CS]: pre-activations, then their cotangents
+  float* X = Y + (NH + 1) * n * CS;                 // [n][CS]: activations (the solver's scratch before them)
+  float* G = X + n * CS;                            // [n][CS]: cotangent of the activations
+  float* un = G + n * CS;                           // [n]: the step's input state
+  float* gb = un + n;                               // [n]: cotangent of the network's output
+  float* us = X;
+  float* cs = X + n;
+  float* Cw = X + 2 * n;
+  for (int e = tid; e < NH * C * C; e += kTrainBlock) {
+    const int l = e / (C * C), oc = e - l * C * C, o = oc / C, cc = oc - o * C;
+    const float* src = cn.w[l + 1] + size_t(oc) * K;
+    float* dst = ws + size_t(l) * C * RS + o * RS + cc;
+#pragma unroll
+    for (int k = 0; k < K; ++k) dst[k * C] = src[k];
+  }
+  auto row = [&](int i) { return i < 0 ? i + n : (i >= n ? i - n : i); };
+  const int quad = tid % Q;
+  const int nitems = n * Q;
+  float pw0[4][K], pb0[4], pwL[4][K], pbL = 0.f, pbh[NH > 0 ? NH : 1][4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    pb0[e] = 0.f;
+#pragma unroll
+    for (int k = 0; k < K; ++k) pw0[e][k] = pwL[e][k] = 0.f;
+#pragma unroll
+    for (int l = 0; l < (NH > 0 ? NH : 1); ++l) pbh[l][e] = 0.f;
+  }
+  v4f wacc[NH > 0 ? NH : 1][TPW];
+#pragma unroll
+  for (int l = 0; l < (NH > 0 ? NH : 1); ++l)
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) wacc[l][t] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  const int ns = a.scheme == INC_SSPRK3 ? 3 : 1;
+  const float nu = a.nu_ptr ? a.nu_ptr[c.b] : a.nu;
+  const float dt = a.dt;
+  const int parts = INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE;
+  float lam, gs, st[3], cb, r, b1, b2;
+  auto gload = [&](int step) {
+    return (own && a.g_traj) ? a.g_traj[(size_t(c.b) * a.steps + step) * n + tid] : 0.f;
+  };
+  lam = gload(a.steps - 1);
+  for (int step = a.steps - 1; step >= 0; --step) {
+    // ---- the solver step's transpose (its stage states from the forward)
+    const float* src = a.stages + (size_t(c.b) * a.steps + step) * ns * n;
+    st[0] = own ? src[tid] : 0.f;
+    st[1] = (own && ns == 3) ? src[n + tid] : 0.f;
+    st[2] = (own && ns == 3) ? src[2 * n + tid] : 0.f;
+    gs = 0.f;
+    if (ns == 1) {
+      cb = dt * lam;
+      rhs_vjp<float, 1, INC_WENO>(a, c, &st[0], &cb, nu, parts, us, cs, Cw, red_v, red_i, &r);
+      __syncthreads();
+      gs += cb;
+      lam += r;
+    } else {
+      cb = (2.f / 3.f) * dt * lam;
+      rhs_vjp<float, 1, INC_WENO>(a, c, &st[2], &cb, nu, parts, us, cs, Cw, red_v, red_i, &r);
+      __syncthreads();
+      gs += cb;
+      b2 = (2.f / 3.f) * lam + r;
+      lam = (1.f / 3.f) * lam;
+      cb = 0.25f * dt * b2;
+      rhs_vjp<float, 1, INC_WENO>(a, c, &st[1], &cb, nu, parts, us, cs, Cw, red_v, red_i, &r);
+      __syncthreads();
+      gs += cb;
+      b1 = 0.25f * b2 + r;
+      lam += 0.75f * b2;
+      cb = dt * b1;
+      rhs_vjp<float, 1, INC_WENO>(a, c, &st[0], &cb, nu, parts, us, cs, Cw, red_v, red_i, &r);
+      __syncthreads();
+      gs += cb;
+      lam += b1 + r;
+    }
+    // ---- the network at the step's input state, pre-activations kept
+    if (own) {
+      un[tid] = st[0];
+      const float gc = g_corr ? g_corr[(size_t(c.b) * a.steps + step) * n + tid] : 0.f;
+      gb[tid] = cn.scale * (gs + gc);
+    }
+    __syncthreads();
+    {
+      const float* w0 = cn.w[0];
+      for (int e = tid; e < nitems; e += kTrainBlock) {
+        const int j = e / Q;
+        float y[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) y[u] = cn.b[0] ? cn.b[0][4 * quad + u] : 0.f;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const float uv = un[row(j + k - P)];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) y[u] += w0[(4 * quad + u) * K + k] * uv;
+        }
+        *reinterpret_cast<v4f*>(Y + j * CS + 4 * quad) = v4f{y[0], y[1], y[2], y[3]};
+        *reinterpret_cast<v4f*>(X + j * CS + 4 * quad) =
+            v4f{gelu_exact(y[0]), gelu_exact(y[1]), gelu_exact(y[2]), gelu_exact(y[3])};
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 1; l <= NH; ++l) {
+      v4f acc[C / 16][PW];
+      conv_tiles<C, K, PW, false>(ws + size_t(l - 1) * C * RS, X, n, acc);
+      __syncthreads();  // every wavefront has read X
+      const int lr = lane & 15, q = lane >> 4;
+#pragma unroll
+      for (int m = 0; m < C / 16; ++m) {
+        const int o = m * 16 + 4 * q;
+        const v4f bo = cn.b[l] ? *reinterpret_cast<const v4f*>(cn.b[l] + o) : v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < PW; ++t) {
+          const int i = (wave * PW + t) * 16 + lr;
+          if (i < n) {
+            const v4f v = acc[m][t] + bo;
+            *reinterpret_cast<v4f*>(Y + (size_t(l) * n + i) * CS + o) = v;
+            *reinterpret_cast<v4f*>(X + i * CS + o) =
+                v4f{gelu_exact(v[0]), gelu_exact(v[1]), gelu_exact(v[2]), gelu_exact(v[3])};
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // ---- transpose of the last layer (C -> 1): X holds its input
+    {
+      const float* wL = cn.w[NH + 1];
+      for (int e = tid; e < nitems; e += kTrainBlock) {
+        const int j = e / Q;
+        const v4f xv = *reinterpret_cast<const v4f*>(X + j * CS + 4 * quad);
+        float g[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const float gv = gb[row(j - k + P)];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            pwL[u][k] += xv[u] * gv;
+            g[u] += wL[(4 * quad + u) * K + k] * gv;
+          }
+        }
+        pbL += gb[j];
+        *reinterpret_cast<v4f*>(G + j * CS + 4 * quad) = v4f{g[0], g[1], g[2], g[3]};
+      }
+    }
+    __syncthreads();
+    // ---- hidden layers, last to first
+#pragma unroll
+    for (int l = NH; l >= 1; --l) {
+      float* Yl = Y + size_t(l) * n * CS;
+      const float* Yp = Y + size_t(l - 1) * n * CS;
+      for (int e = tid; e < nitems; e += kTrainBlock) {
+        const int j = e / Q;
+        const v4f gv = *reinterpret_cast<const v4f*>(G + j * CS + 4 * quad);
+        const v4f yv = *reinterpret_cast<const v4f*>(Yl + j * CS + 4 * quad);
+        const v4f yp = *reinterpret_cast<const v4f*>(Yp + j * CS + 4 * quad);
+        v4f gy;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          gy[u] = gv[u] * gelu_grad(yv[u]);
+          pbh[l - 1][u] += gy[u];
+        }
+        *reinterpret_cast<v4f*>(Yl + j * CS + 4 * quad) = gy;
+        *reinterpret_cast<v4f*>(X + j * CS + 4 * quad) =
+            v4f{gelu_exact(yp[0]), gelu_exact(yp[1]), gelu_exact(yp[2]), gelu_exact(yp[3])};
+      }
+      __syncthreads();
+      wgrad_tiles<C, K, TPW>(Yl, X, n, wacc[l - 1]);
+      v4f acc[C / 16][PW];
+      conv_tiles<C, K, PW, true>(ws + size_t(l - 1) * C * RS, Yl, n, acc);
+      const int lr = lane & 15, q = lane >> 4;
+#pragma unroll
+      for (int m = 0; m < C / 16; ++m)
+#pragma unroll
+        for (int t = 0; t < PW; ++t) {
+          const int i = (wave * PW + t) * 16 + lr;
+          if (i < n) *reinterpret_cast<v4f*>(G + i * CS + m * 16 + 4 * q) = acc[m][t];
+        }
+      __syncthreads();
+    }
+    // ---- first layer (1 -> C): its cotangent, weight gradient and the state's cotangent
+    for (int e = tid; e < nitems; e += kTrainBlock) {
+      const int j = e / Q;
+      const v4f gv = *reinterpret_cast<const v4f*>(G + j * CS + 4 * quad);
+      const v4f yv = *reinterpret_cast<const v4f*>(Y + j * CS + 4 * quad);
+      v4f gy;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        gy[u] = gv[u] * gelu_grad(yv[u]);
+        pb0[u] += gy[u];
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float uv = un[row(j + k - P)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pw0[u][k] += gy[u] * uv;
+      }
+      *reinterpret_cast<v4f*>(Y + j * CS + 4 * quad) = gy;
+    }
+    __syncthreads();
+    if (own) {
+      const float* w0 = cn.w[0];
+      float g = 0.f;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float* yr = Y + row(tid - k + P) * CS;
+#pragma unroll
+        for (int q4 = 0; q4 < Q; ++q4) {
+          const v4f gy = *reinterpret_cast<const v4f*>(yr + 4 * q4);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) g += w0[(4 * q4 + u) * K + k] * gy[u];
+        }
+      }
+      lam += g;
+    }
+    if (step > 0) lam += gload(step - 1);
+    __syncthreads();  // Y and un are read above; the next step's transpose writes X
+  }
+  if (own) a.out[size_t(c.b) * n + tid] = lam;
+
+  // ---- gradient row of this trajectory: [w0, b0, w1, b1, ..., wL, bL] as
+  // the Conv1d tensors are laid out
+  float* gw = g_w + size_t(c.b) * (size_t(C) * K + C + size_t(NH) * (size_t(C) * C * K + C) + C * K + 1);
+  constexpr int OFF0 = 0, OFFH = C * K + C, HSZ = C * C * K + C, OFFL = OFFH + NH * HSZ;
+  {
+    const int lr = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int l = 0; l < NH; ++l)
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+        const int id = wave + 4 * t;
+        if (id >= TT) continue;
+        const int m = id / NCOL, col = (id - m * NCOL) * 16 + lr, k = col / C, cc = col - k * C;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int o = m * 16 + 4 * q + rr;
+          gw[OFFH + l * HSZ + (o * C + cc) * K + k] = wacc[l][t][rr];
+        }
+      }
+  }
+  // per-thread partials: sum over the threads of one channel quad (lanes
+  // l, l + Q, ... of each wavefront by shuffles, then the four wavefronts)
+  float v[V];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      v[u * K + k] = pw0[u][k];
+      v[V0 + u * K + k] = pwL[u][k];
+    }
+    v[4 * K + u] = pb0[u];
+#pragma unroll
+    for (int l = 0; l < NH; ++l) v[V0 + VL + 4 * l + u] = pbh[l][u];
+  }
+  v[V0 + 4 * K] = pbL;
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    for (int off = Q; off < kWave; off <<= 1) v[i] += shfl_xor(v[i], off);
+  float* red = Y;  // [4 waves][Q][V]
+  if (lane < Q) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) red[(wave * Q + lane) * V + i] = v[i];
+  }
+  __syncthreads();
+  for (int e = tid; e < Q * V; e += kTrainBlock) {
+    const int qd = e / V, i = e - qd * V;
+    const float s = red[qd * V + i] + red[(Q + qd) * V + i] + red[(2 * Q + qd) * V + i] + red[(3 * Q + qd) * V + i];
+    if (i < 4 * K) {
+      const int u = i / K, k = i - u * K;
+      gw[OFF0 + (4 * qd + u) * K + k] = s;
+    } else if (i < V0) {
+      gw[OFF0 + C * K + 4 * qd + (i - 4 * K)] = s;
+    } else if (i < V0 + 4 * K) {
+      const int u = (i - V0) / K, k = (i - V0) - u * K;
+      gw[OFFL + (4 * qd + u) * K + k] = s;
+    } else if (i == V0 + 4 * K) {
+      if (qd == 0) gw[OFFL + C * K] = s;
+    } else {
+      const int l = (i - V0 - VL) / 4, u = (i - V0 - VL) - 4 * l;
+      gw[OFFH + l * HSZ + C * C * K + 4 * qd + u] = s;
+    }
   }
 }
 
@@ -1699,7 +2134,7 @@ void* inc_fwd_mfma_kernel(int ppt, int c, int k, int pw) {
 
 template <typename T>
 int inc_fwd_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void* u, void* out, int steps,
-                   void* traj, void* corr, void* stream) {
+                   void* traj, void* corr, void* stream, void* stages = nullptr) {
   BArgs<T> a;
   int rc = fill_args<T>(d, a);
   if (rc) return rc;
@@ -1751,6 +2186,7 @@ int inc_fwd_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void
   a.s = nullptr;
   a.out = static_cast<T*>(out);
   a.traj = static_cast<T*>(traj);
+  a.stages = static_cast<T*>(stages);
   if (a.batch == 0) return INC_OK;
   void* fn;
   if constexpr (kPptMax == 2) {
@@ -1766,6 +2202,76 @@ int inc_fwd_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void
   T* cp = static_cast<T*>(corr);
   void* args[] = {&a, &cn, &cp};
   INC_HIP(hipLaunchKernel(fn, dim3(a.batch), dim3(team), args, lds, static_cast<hipStream_t>(stream)));
+  return INC_OK;
+}
+
+// Training adjoint instances: width 32, K 5 or 7, 0..2 hidden layers, 1 or 2
+// position tiles per wavefront (n <= 64, n <= 128).
+template <int K, int NH>
+void* inc_adj_kernel_pw(int pw) {
+  return pw == 1 ? reinterpret_cast<void*>(&burgers_inc_adj_kernel<32, K, NH, 1>)
+                 : reinterpret_cast<void*>(&burgers_inc_adj_kernel<32, K, NH, 2>);
+}
+
+template <int K>
+void* inc_adj_kernel_k(int nh, int pw) {
+  return nh == 0 ? inc_adj_kernel_pw<K, 0>(pw) : (nh == 1 ? inc_adj_kernel_pw<K, 1>(pw) : inc_adj_kernel_pw<K, 2>(pw));
+}
+
+// LDS of burgers_inc_adj_kernel: staged hidden weights, nh + 1 pre-activation
+// arrays, activations and their cotangent ([n][C + 4] each), the input state
+// and the output's cotangent; the final reduction reuses it from the
+// pre-activations on.
+size_t inc_adj_lds(int n, int C, int K, int nh) {
+  const size_t cs = C + 4, rs = size_t(C) * K + 4, ws = size_t(nh) * C * rs;
+  const size_t v = (4 * K + 4) + (4 * K + 1) + 4 * nh;
+  const size_t body = ws + size_t(nh + 3) * n * cs + 2 * size_t(n);
+  const size_t red = ws + 4 * size_t(C / 4) * v;
+  return (body > red ? body : red) * sizeof(float);
+}
+
+int inc_adj_params(int C, int K, int layers) {
+  return 2 * C * K + C + 1 + (layers - 2) * (C * C * K + C);
+}
+
+int inc_train_adj_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void* stages, const void* g_traj,
+                         const void* g_corr, void* g_u, void* g_w, int steps, void* stream) {
+  INC_REQUIRE(d->dtype == INC_F32, INC_EUNSUPPORTED, "the training adjoint with the network inside runs float32");
+  BArgs<float> a;
+  int rc = fill_args<float>(d, a);
+  if (rc) return rc;
+  INC_REQUIRE(d->convection == INC_WENO, INC_EUNSUPPORTED, "the in-kernel correction network runs WENO convection");
+  INC_REQUIRE(cd->channels == 32 && (cd->kernel == 5 || cd->kernel == 7) && cd->layers >= 2 && cd->layers <= 4,
+              INC_EUNSUPPORTED,
+              "training adjoint with the network inside: width 32, kernel size 5 or 7, 2..4 layers (got %d, %d, %d)",
+              cd->channels, cd->kernel, cd->layers);
+  INC_REQUIRE(d->n <= 128, INC_EUNSUPPORTED, "training adjoint with the network inside: n <= 128 (got %d)", d->n);
+  INC_REQUIRE(stages != nullptr && g_u != nullptr && g_w != nullptr, INC_EINVAL,
+              "the training adjoint needs stages, g_u and g_w");
+  const int nh = cd->layers - 2, pw = d->n <= 64 ? 1 : 2;
+  const size_t lds = inc_adj_lds(d->n, 32, cd->kernel, nh);
+  INC_REQUIRE(lds <= 160 * 1024 - 256, INC_EUNSUPPORTED,
+              "training adjoint with the network inside: n=%d, %d layers, K=%d need %zu bytes of LDS", d->n,
+              cd->layers, cd->kernel, lds);
+  CnnArgs<float> cn{};
+  cn.layers = cd->layers;
+  cn.scale = float(cd->scale);
+  for (int l = 0; l < cd->layers; ++l) {
+    INC_REQUIRE(cd->weight[l] != nullptr, INC_EINVAL, "correction network layer %d has no weight", l);
+    cn.w[l] = static_cast<const float*>(cd->weight[l]);
+    cn.b[l] = static_cast<const float*>(cd->bias[l]);
+  }
+  a.team = kTrainBlock;
+  a.steps = steps;
+  a.stages = static_cast<float*>(const_cast<void*>(stages));
+  a.g_traj = static_cast<const float*>(g_traj);
+  a.out = static_cast<float*>(g_u);
+  if (a.batch == 0) return INC_OK;
+  void* fn = cd->kernel == 5 ? inc_adj_kernel_k<5>(nh, pw) : inc_adj_kernel_k<7>(nh, pw);
+  const float* gc = static_cast<const float*>(g_corr);
+  float* gw = static_cast<float*>(g_w);
+  void* args[] = {&a, &cn, &gc, &gw};
+  INC_HIP(hipLaunchKernel(fn, dim3(a.batch), dim3(kTrainBlock), args, lds, static_cast<hipStream_t>(stream)));
   return INC_OK;
 }
 
@@ -1871,6 +2377,27 @@ int inc_burgers_inc_fwd(const inc_burgers_desc* d, const inc_cnn_desc* cnn, cons
   if (d->dtype == INC_F64) return inc::inc_fwd_launch<double>(d, cnn, u, u_out, steps, traj, corr, stream);
   inc::set_error("unsupported dtype %d", d->dtype);
   return INC_EINVAL;
+}
+
+int inc_burgers_inc_train_fwd(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* u, void* u_out,
+                              int steps, void* traj, void* corr, void* stages, void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(d != nullptr && cnn != nullptr, INC_EINVAL, "null descriptor");
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(stages != nullptr, INC_EINVAL, "the training forward needs stages");
+  if (d->dtype == INC_F32) return inc::inc_fwd_launch<float>(d, cnn, u, u_out, steps, traj, corr, stream, stages);
+  if (d->dtype == INC_F64) return inc::inc_fwd_launch<double>(d, cnn, u, u_out, steps, traj, corr, stream, stages);
+  inc::set_error("unsupported dtype %d", d->dtype);
+  return INC_EINVAL;
+}
+
+int inc_burgers_inc_train_adj(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* stages,
+                              const void* g_traj, const void* g_corr, void* g_u, void* g_w, int steps,
+                              void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(d != nullptr && cnn != nullptr, INC_EINVAL, "null descriptor");
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  return inc::inc_train_adj_launch(d, cnn, stages, g_traj, g_corr, g_u, g_w, steps, stream);
 }
 
 int inc_burgers_adaptive_fwd(const inc_burgers_desc* d, double cfl, int max_sub, const void* u, const void* s,

@@ -7,9 +7,10 @@ as the reference does.  Two MI355X additions:
 * with ``correction_term=None`` and an inc_amd solver on a fixed time step,
   the whole rollout is ONE kernel launch (state resident in registers/LDS,
   every step's output written once) and its gradient ONE adjoint launch;
-* with ``correction_term='INC'``, no gradient wanted (validation / test
-  rollouts), a Burgers solver and a CorrectionCNN the kernel runs, the
-  correction network is evaluated inside that one launch as well;
+* with ``correction_term='INC'``, a Burgers solver and a CorrectionCNN the
+  kernel runs, the correction network is evaluated inside that one launch
+  as well, and with a gradient wanted (training) its transpose inside the
+  one adjoint launch (float32, width 32, up to 4 layers, N <= 128);
 * ``checkpoint_every=k`` recomputes the INC rollout in segments of k steps
   during the backward pass (segment-start states are the only activations
   kept), for long unrolled horizons.
@@ -22,7 +23,7 @@ import torch
 from torch.utils.checkpoint import checkpoint
 
 from .correction import CorrectionCNN
-from .solver_1d import BurgersSolverTorch, KSSolverTorch
+from .solver_1d import BurgersSolverTorch, KSSolverTorch, inc_train_supported
 
 
 def _clone_t(t):
@@ -41,23 +42,26 @@ def _fusable(solver, correction_term):
 
 def _fusable_inc(model, solver, u):
     """An INC rollout the Burgers kernel runs with the correction network
-    inside (no gradient wanted, fixed time step, WENO, a CorrectionCNN of a
-    supported shape, within the kernel's size limits)."""
+    inside (fixed time step, WENO, a CorrectionCNN of a supported shape,
+    within the kernel's size limits; with a gradient wanted, within the
+    training adjoint's)."""
     if not isinstance(solver, BurgersSolverTorch) or not isinstance(model, CorrectionCNN):
-        return False
-    if torch.is_grad_enabled() and (u.requires_grad or any(p.requires_grad for p in model.parameters())):
         return False
     if solver.adaptive_CFL is not None or solver.param.convection_term != "WENO" or not u.is_cuda:
         return False
     n = u.shape[-1]
+    if torch.is_grad_enabled() and (u.requires_grad or any(p.requires_grad for p in model.parameters())):
+        return model.fused_desc(u.dtype) is not None and inc_train_supported(model, n, u.dtype)
     if u.dtype not in (torch.float32, torch.float64) or n > (512 if u.dtype == torch.float32 else 256):
         return False
     fd = model.fused_desc(u.dtype)
     return fd is not None and (fd[0].channels + 2) * n * u.element_size() <= 150 * 1024 and fd[0].kernel <= n
 
 
-def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, checkpoint_every=None):
-    """Unroll ``mstep`` solver steps (trainer_1d.py:154-183)."""
+def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, checkpoint_every=None, fused=True):
+    """Unroll ``mstep`` solver steps (trainer_1d.py:154-183).  ``fused=False``
+    keeps the step-by-step loop (one network evaluation and one solver launch
+    per step) where a fused rollout kernel would otherwise run."""
     b, d = u_initial.shape
     if correction_term not in (None, "INC"):
         raise ValueError(f"Invalid correction term: {correction_term}")
@@ -65,9 +69,9 @@ def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, c
     if mstep <= 0:
         empty = u_initial.new_zeros(b, 0, d)
         return empty, (empty.clone() if correction_term else None)
-    if correction_term == "INC" and _fusable_inc(model, solver, u_initial):
+    if fused and correction_term == "INC" and _fusable_inc(model, solver, u_initial):
         return solver.rollout_inc(model, u_initial, mstep)
-    if _fusable(solver, correction_term):
+    if fused and _fusable(solver, correction_term):
         if isinstance(solver, KSSolverTorch):
             return solver.rollout(u_initial, mstep, L), None
         return solver.rollout(u_initial, mstep), None

@@ -168,6 +168,68 @@ class _BurgersRolloutMSE(torch.autograd.Function):
         return g_u, g_s, g_target, None, None, None, None, None
 
 
+class _BurgersIncRollout(torch.autograd.Function):
+    """simulate_rollout(model, solver, u, steps, 'INC') (trainer_1d.py:154-183)
+    for a CorrectionCNN with the network evaluated inside the rollout kernel
+    and its transpose inside the adjoint kernel (inc_burgers_inc_train_fwd /
+    _adj); returns (results, corrections), both [B, steps, N],
+    differentiable in u and in the network's parameters (``params``, in the
+    order [w0, b0, w1, b1, ...] of its Conv1d layers)."""
+
+    @staticmethod
+    def forward(ctx, u, solver, model, steps, t, *params):
+        ctx.set_materialize_grads(False)
+        cdesc, keep_w = model.fused_desc(u.dtype)
+        u = u.contiguous()
+        b, n = u.shape
+        desc, keep = solver._desc(u, solver.dt, t)
+        out = torch.empty_like(u)
+        traj = u.new_empty(b, steps, n)
+        corr = u.new_empty(b, steps, n)
+        ns = 3 if solver.time_scheme == "SSPRK3" else 1
+        stages = u.new_empty(b, steps, ns, n)
+        rc = L.load().inc_burgers_inc_train_fwd(ctypes.byref(desc), ctypes.byref(cdesc), L.ptr(u), L.ptr(out),
+                                                steps, L.ptr(traj), L.ptr(corr), L.ptr(stages), L.stream())
+        L.check(rc, "BurgersSolverTorch.rollout_inc")
+        ctx.save_for_backward(stages)
+        ctx.desc, ctx.keep, ctx.cdesc, ctx.keep_w = desc, keep, cdesc, keep_w
+        ctx.steps, ctx.shape, ctx.param_shapes = steps, (b, n), [p.shape for p in params]
+        return traj, corr
+
+    @staticmethod
+    def backward(ctx, g_traj, g_corr):
+        (stages,) = ctx.saved_tensors
+        b, n = ctx.shape
+        g_traj = g_traj.contiguous() if g_traj is not None else None
+        g_corr = g_corr.contiguous() if g_corr is not None else None
+        g_u = stages.new_empty(b, n)
+        sizes = [math.prod(sh) for sh in ctx.param_shapes]
+        g_w = stages.new_empty(b, sum(sizes))
+        rc = L.load().inc_burgers_inc_train_adj(ctypes.byref(ctx.desc), ctypes.byref(ctx.cdesc), L.ptr(stages),
+                                                L.ptr(g_traj), L.ptr(g_corr), L.ptr(g_u), L.ptr(g_w), ctx.steps,
+                                                L.stream())
+        L.check(rc, "BurgersSolverTorch.rollout_inc backward")
+        g_rows = g_w.sum(0).split(sizes)
+        grads = [g.view(sh) for g, sh in zip(g_rows, ctx.param_shapes)]
+        return (g_u, None, None, None, None, *grads)
+
+
+def inc_train_supported(model, n, dtype):
+    """Whether inc_burgers_inc_train_adj runs this network at resolution n:
+    float32, width 32, kernel size 5 or 7, 2..4 layers, n <= 128 and the
+    block's LDS within 160 KiB (the library's own check, restated so that a
+    rollout can choose its path before launching)."""
+    convs = list(model.convs)
+    c, k, layers = convs[0].weight.shape[0], convs[0].k, len(convs)
+    if dtype != torch.float32 or c != 32 or k not in (5, 7) or not 2 <= layers <= 4 or not k <= n <= 128:
+        return False
+    nh = layers - 2
+    ws = nh * c * (c * k + 4)
+    body = ws + (nh + 3) * n * (c + 4) + 2 * n
+    red = ws + 4 * (c // 4) * ((4 * k + 4) + (4 * k + 1) + 4 * nh)
+    return 4 * max(body, red) <= 160 * 1024 - 256
+
+
 class _BurgersAdaptive(torch.autograd.Function):
     """`steps` steps of dt, each sub-stepped per trajectory to the CFL limit
     on the GPU (inc_burgers_adaptive_fwd); returns the final state or the
@@ -498,15 +560,26 @@ class BurgersSolverTorch:
         """simulate_rollout(model, self, u, mstep, 'INC') (trainer_1d.py:154-183)
         with the correction network evaluated inside the rollout kernel
         (inc_burgers_inc_fwd): returns (results, corrections), both
-        [B, mstep, N].  No gradient (validation / test rollouts); ``model``
-        must be a CorrectionCNN whose shape the kernel runs
-        (CorrectionCNN.fused_desc)."""
+        [B, mstep, N].  ``model`` must be a CorrectionCNN whose shape the
+        kernel runs (CorrectionCNN.fused_desc).  With a gradient wanted (u or
+        the network's parameters require it) the rollout is differentiable:
+        the adjoint kernel runs the network's transpose
+        (inc_burgers_inc_train_fwd / _adj, shapes per inc_train_supported)."""
         L.require_gpu(u)
         if self.adaptive_CFL is not None:
             raise ValueError("rollout_inc runs a fixed time step; adaptive_CFL is set")
         cnn = model.fused_desc(u.dtype)
         if cnn is None:
             raise ValueError("the correction network's shape is not one the in-kernel path runs")
+        params = [t for cv in model.convs for t in (cv.weight, cv.bias)]
+        if torch.is_grad_enabled() and (u.requires_grad or any(p.requires_grad for p in params)):
+            if not inc_train_supported(model, u.shape[-1], u.dtype):
+                raise ValueError("the correction network's shape is not one the in-kernel training adjoint runs")
+            traj, corr = _BurgersIncRollout.apply(u, self, model, mstep, self.t, *params)
+            for _ in range(mstep):
+                self.t += self.dt
+            self.correction_s = corr[:, -1] if mstep > 0 else None
+            return traj, corr
         cdesc, keep_w = cnn
         u = u.detach().contiguous()
         b, n = u.shape

@@ -175,6 +175,33 @@ int inc_burgers_inc_fwd(const inc_burgers_desc* d, const inc_cnn_desc* cnn, cons
                         void* u_out, int steps, void* traj, void* corr, void* stream);
 
 /*
+ * Training form of the same rollout: the simulate_rollout loop of
+ * CorrectionTrainer.process_batch (trainer/trainer_1d.py:699-727, rollout at
+ * :154-183) differentiated through the solver and the network.
+ * inc_burgers_inc_train_fwd is inc_burgers_inc_fwd that also writes every
+ * step's stage states:
+ *   stages [batch, steps, ns, n] (ns = 3 for SSP-RK3, 1 for Euler)
+ * inc_burgers_inc_train_adj is its adjoint.  From the stages, the cotangents
+ * of the trajectory (g_traj [batch, steps, n] or NULL) and of the
+ * corrections (g_corr [batch, steps, n] or NULL) it writes the cotangent of
+ * the initial state (g_u [batch, n]) and each trajectory's gradient of the
+ * network's parameters:
+ *   g_w [batch, P], row b = [w0, b0, w1, b1, ..., w_{L-1}, b_{L-1}] in the
+ *   layout of the weight and bias tensors, P = 2CK + C + 1 + (L-2)(C C K + C);
+ * the batch gradient is the sum of the rows.  The network is evaluated again
+ * at every step's input state inside the kernel; no activation is stored.
+ * Supported by the adjoint: float, WENO convection, fixed time step, C = 32,
+ * K = 5 or 7, 2..4 layers, n <= 128 and the block's LDS (the hidden layers'
+ * weights and (L + 1) * n * (C + 4) activations) within 160 KiB.
+ */
+int inc_burgers_inc_train_fwd(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* u,
+                              void* u_out, int steps, void* traj, void* corr, void* stages,
+                              void* stream);
+int inc_burgers_inc_train_adj(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* stages,
+                              const void* g_traj, const void* g_corr, void* g_u, void* g_w, int steps,
+                              void* stream);
+
+/*
  * Adaptive sub-stepping per trajectory (BurgersSolverTorch.step_adaptive,
  * solver_1d.py:148-179, which sub-steps the whole batch to
  * dt_j = min(remaining, CFL * dx / max|u|) with max|u| read back to the host

@@ -89,6 +89,23 @@ def test_argument_errors_map_to_codes_without_launch():
     assert rc == L.INC_EINVAL and b"loss_parts" in lib.inc_last_error()
     rc = lib.inc_ks_adj_mse(ctypes.byref(k), 1, None, 1, 1, None, None, None, 1, None)
     assert rc == L.INC_EINVAL and b"traj" in lib.inc_last_error()
+    # training rollout with the network inside: the forward needs stages; the
+    # adjoint runs float32, width 32, up to 4 layers and n <= 128
+    d = L.BurgersDesc(dtype=L.INC_F32, batch=1, n=64, dx=0.1, dt=0.01)
+    cn = L.CnnDesc(layers=4, channels=32, kernel=5, scale=1.0)
+    rc = lib.inc_burgers_inc_train_fwd(ctypes.byref(d), ctypes.byref(cn), None, None, 1, None, None, None, None)
+    assert rc == L.INC_EINVAL and b"stages" in lib.inc_last_error()
+    rc = lib.inc_burgers_inc_train_adj(ctypes.byref(d), ctypes.byref(cn), None, None, None, None, None, 1, None)
+    assert rc == L.INC_EINVAL and b"stages" in lib.inc_last_error()
+    cn48 = L.CnnDesc(layers=4, channels=48, kernel=5, scale=1.0)
+    rc = lib.inc_burgers_inc_train_adj(ctypes.byref(d), ctypes.byref(cn48), 1, None, None, 1, 1, 1, None)
+    assert rc == L.INC_EUNSUPPORTED and b"width 32" in lib.inc_last_error()
+    d64 = L.BurgersDesc(dtype=L.INC_F64, batch=1, n=64, dx=0.1, dt=0.01)
+    rc = lib.inc_burgers_inc_train_adj(ctypes.byref(d64), ctypes.byref(cn), 1, None, None, 1, 1, 1, None)
+    assert rc == L.INC_EUNSUPPORTED and b"float32" in lib.inc_last_error()
+    d256 = L.BurgersDesc(dtype=L.INC_F32, batch=1, n=256, dx=0.1, dt=0.01)
+    rc = lib.inc_burgers_inc_train_adj(ctypes.byref(d256), ctypes.byref(cn), 1, None, None, 1, 1, 1, None)
+    assert rc == L.INC_EUNSUPPORTED and b"n <= 128" in lib.inc_last_error()
     # successful-path state is reset by the next call
     assert lib.inc_weno_fwd(L.INC_F32, 0, None, None, 0, 16, None) == L.INC_OK
     assert lib.inc_last_error() == b""
