@@ -13,8 +13,9 @@ corrections, so both cotangents reach the kernel.  Shapes cover 2, 3 and 4
 layers, kernel sizes 5 and 7, one and two position tiles per wavefront,
 resolutions that are not multiples of 16, Euler and SSP-RK3, and the
 training workload's batch (512 x 128, 16 steps).  The kernel runs float32;
-tolerances (relative L2): trajectory and corrections 1e-5, gradients 1e-4
-against the per-step float32 path and 2e-4 against the FP64 oracle.
+tolerances (relative L2): trajectory and corrections 1e-5; gradients 2e-5
+against the per-step float32 path (measured on MI355X: 9e-8 .. 2.1e-6) and
+5e-5 against the FP64 oracle.
 """
 import copy
 import math
@@ -81,9 +82,9 @@ def test_inc_training_rollout_matches_per_step_path(cuda, case):
     r2, c2, gu2, gw2, sv2 = _run(net, u0, n, b, H, scheme, target, wc, fused=False)
     assert rel_l2(r1, r2) < 1e-5
     assert rel_l2(c1, c2) < 1e-5
-    assert rel_l2(gu1, gu2) < 1e-4
+    assert rel_l2(gu1, gu2) < 2e-5
     for i, (g1, g2) in enumerate(zip(gw1, gw2)):
-        assert rel_l2(g1, g2) < 1e-4, f"parameter {i} ({tuple(g1.shape)})"
+        assert rel_l2(g1, g2) < 2e-5, f"parameter {i} ({tuple(g1.shape)})"
     assert abs(sv1.t - sv2.t) < 1e-12
     assert rel_l2(sv1.correction_s.detach(), sv2.correction_s.detach()) < 1e-5
 
@@ -107,9 +108,9 @@ def test_inc_training_rollout_matches_fp64_oracle(cuda):
                            wc.to(cuda, torch.float32), fused=True)
     assert rel_l2(r.double().cpu(), res.detach()) < 1e-5
     assert rel_l2(c.double().cpu(), corr.detach()) < 1e-5
-    assert rel_l2(gu.double().cpu(), uo.grad) < 2e-4
+    assert rel_l2(gu.double().cpu(), uo.grad) < 5e-5
     for i, (g, pr) in enumerate(zip(gw, cpu_net.parameters())):
-        assert rel_l2(g.double().cpu(), pr.grad) < 2e-4, f"parameter {i} ({tuple(pr.shape)})"
+        assert rel_l2(g.double().cpu(), pr.grad) < 5e-5, f"parameter {i} ({tuple(pr.shape)})"
 
 
 def test_inc_training_rollout_gradient_of_corrections_only(cuda):
@@ -127,7 +128,7 @@ def test_inc_training_rollout_gradient_of_corrections_only(cuda):
         (corr ** 2).sum().backward()
         out.append([u.grad.clone()] + [p.grad.clone() for p in net.parameters()])
     for g1, g2 in zip(*out):
-        assert rel_l2(g1, g2) < 1e-4
+        assert rel_l2(g1, g2) < 2e-5
 
 
 def test_inc_training_shapes_outside_the_kernel_take_the_per_step_path(cuda):
