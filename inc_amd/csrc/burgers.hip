@@ -981,9 +981,12 @@ __global__ void __launch_bounds__(kBlock) burgers_inc_fwd_kernel(BArgs<T> a, Cnn
 // per-thread partial sums, and each trajectory writes its own gradient row.
 constexpr int kTrainBlock = 256;
 
-// d/dv of the exact (erf) GELU: Phi(v) + v phi(v).
-__device__ __forceinline__ float gelu_grad(float v) {
-  return 0.5f * (1.f + erff(v * 0.70710678118654752440f)) + v * 0.39894228040143267794f * expf(-0.5f * v * v);
+// The exact (erf) GELU and its derivative from one erf: gelu(v) = v Phi(v),
+// gelu'(v) = Phi(v) + v phi(v); returns gelu(v), writes gelu'(v) to d.
+__device__ __forceinline__ float gelu_with_grad(float v, float& d) {
+  const float ph = 0.5f * (1.f + erff(v * 0.70710678118654752440f));
+  d = ph + v * 0.39894228040143267794f * expf(-0.5f * v * v);
+  return v * ph;
 }
 
 // One wavefront's part of a C -> C circular convolution of `in` (position-
@@ -1064,27 +1067,46 @@ __device__ __forceinline__ void conv_tiles(const float* ws, const float* in, int
 // This wavefront's tiles of one hidden layer's weight gradient, added to
 // acc: dW[o][c][k] += sum_j gy[j][o] x[j + k - P][c] (gy, x position-major,
 // row stride C + 4).  The layer's (C/16) x (C*K/16) tiles of D[o][k*C + c]
-// go round the four wavefronts (tile wave + 4t); lane (lr, q) holds
-// D[16m + 4q + r][16ct + lr].  The reduction runs over positions, four per
-// MFMA, all tiles of the wavefront interleaved.
-template <int C, int K, int TPW>
-__device__ __forceinline__ void wgrad_tiles(const float* gy, const float* x, int n, v4f (&acc)[TPW]) {
+// go round the four wavefronts (tile W + 4t of wavefront W); lane (lr, q)
+// holds D[16m + 4q + r][16ct + lr].  The reduction runs over positions, four
+// per MFMA, all tiles of the wavefront interleaved.  W is a template
+// argument so that every tile's row, column and tap are constants: per four
+// positions the lane computes one row offset per tap and each operand is
+// one LDS read at a constant offset from it.
+template <int C, int K, int TPW, int W>
+__device__ __forceinline__ void wgrad_wave(const float* gy, const float* x, int n, v4f (&acc)[TPW]) {
   constexpr int P = K / 2, CS = C + 4, NCOL = C * K / 16, TT = (C / 16) * NCOL;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int lr = lane & 15, q = lane >> 4;
+  const int lane = threadIdx.x & 63, lr = lane & 15, q = lane >> 4;
   auto row = [&](int i) { return i < 0 ? i + n : (i >= n ? i - n : i); };
-  for (int j0 = 0; j0 < n; j0 += 4) {
-    const int j = j0 + q, jc = min(j, n - 1);
-    const float mask = j < n ? 1.f : 0.f;
+  auto quad = [&](int j0, bool tail) __attribute__((always_inline)) {
+    const int j = j0 + q, jc = tail ? min(j, n - 1) : j;
+    const float* ga = gy + jc * CS + lr;
+    const float* xb[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) xb[k] = x + row(jc + k - P) * CS + lr;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
-      if (wave + 4 * t >= TT) continue;  // wave-uniform
-      const int id = wave + 4 * t, m = id / NCOL, col = (id - m * NCOL) * 16, k = col / C;
-      const float av = gy[jc * CS + m * 16 + lr] * mask;
-      const int jr = row(jc + k - P);
-      const float bv = x[jr * CS + (col - k * C) + lr];
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t], 0, 0, 0);
+      const int id = W + 4 * t;
+      if (id >= TT) continue;
+      const int m = id / NCOL, col = (id - m * NCOL) * 16, k = col / C, c0 = col - k * C;
+      float av = ga[m * 16];
+      if (tail && j >= n) av = 0.f;
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, xb[k][c0], acc[t], 0, 0, 0);
     }
+  };
+  const int n4 = n & ~3;
+#pragma unroll 2
+  for (int j0 = 0; j0 < n4; j0 += 4) quad(j0, false);
+  if (n4 < n) quad(n4, true);
+}
+
+template <int C, int K, int TPW>
+__device__ __forceinline__ void wgrad_tiles(const float* gy, const float* x, int n, v4f (&acc)[TPW]) {
+  switch (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) {
+    case 0: wgrad_wave<C, K, TPW, 0>(gy, x, n, acc); break;
+    case 1: wgrad_wave<C, K, TPW, 1>(gy, x, n, acc); break;
+    case 2: wgrad_wave<C, K, TPW, 2>(gy, x, n, acc); break;
+    default: wgrad_wave<C, K, TPW, 3>(gy, x, n, acc); break;
   }
 }
 
@@ -1236,12 +1258,22 @@ __global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<floa
       }
       __syncthreads();
     }
-    // ---- transpose of the last layer (C -> 1): X holds its input
+    // ---- transpose of the last layer (C -> 1): X holds its input.  From
+    // here on a pre-activation slot is overwritten by the GELU derivative at
+    // it (its one later use), one layer ahead of the pass that reads it.
     {
       const float* wL = cn.w[NH + 1];
+      float* Yt = Y + size_t(NH) * n * CS;
       for (int e = tid; e < nitems; e += kTrainBlock) {
         const int j = e / Q;
         const v4f xv = *reinterpret_cast<const v4f*>(X + j * CS + 4 * quad);
+        {
+          const v4f yv = *reinterpret_cast<const v4f*>(Yt + j * CS + 4 * quad);
+          float dv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) gelu_with_grad(yv[u], dv[u]);
+          *reinterpret_cast<v4f*>(Yt + j * CS + 4 * quad) = v4f{dv[0], dv[1], dv[2], dv[3]};
+        }
         float g[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -1261,21 +1293,23 @@ __global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<floa
 #pragma unroll
     for (int l = NH; l >= 1; --l) {
       float* Yl = Y + size_t(l) * n * CS;
-      const float* Yp = Y + size_t(l - 1) * n * CS;
+      float* Yp = Y + size_t(l - 1) * n * CS;
       for (int e = tid; e < nitems; e += kTrainBlock) {
         const int j = e / Q;
         const v4f gv = *reinterpret_cast<const v4f*>(G + j * CS + 4 * quad);
-        const v4f yv = *reinterpret_cast<const v4f*>(Yl + j * CS + 4 * quad);
+        const v4f dl = *reinterpret_cast<const v4f*>(Yl + j * CS + 4 * quad);  // gelu'(Y_l)
         const v4f yp = *reinterpret_cast<const v4f*>(Yp + j * CS + 4 * quad);
         v4f gy;
+        float ap[4], dp[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          gy[u] = gv[u] * gelu_grad(yv[u]);
+          gy[u] = gv[u] * dl[u];
           pbh[l - 1][u] += gy[u];
+          ap[u] = gelu_with_grad(yp[u], dp[u]);
         }
         *reinterpret_cast<v4f*>(Yl + j * CS + 4 * quad) = gy;
-        *reinterpret_cast<v4f*>(X + j * CS + 4 * quad) =
-            v4f{gelu_exact(yp[0]), gelu_exact(yp[1]), gelu_exact(yp[2]), gelu_exact(yp[3])};
+        *reinterpret_cast<v4f*>(Yp + j * CS + 4 * quad) = v4f{dp[0], dp[1], dp[2], dp[3]};
+        *reinterpret_cast<v4f*>(X + j * CS + 4 * quad) = v4f{ap[0], ap[1], ap[2], ap[3]};
       }
       __syncthreads();
       wgrad_tiles<C, K, TPW>(Yl, X, n, wacc[l - 1]);
@@ -1295,11 +1329,11 @@ __global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<floa
     for (int e = tid; e < nitems; e += kTrainBlock) {
       const int j = e / Q;
       const v4f gv = *reinterpret_cast<const v4f*>(G + j * CS + 4 * quad);
-      const v4f yv = *reinterpret_cast<const v4f*>(Y + j * CS + 4 * quad);
+      const v4f dv = *reinterpret_cast<const v4f*>(Y + j * CS + 4 * quad);  // gelu'(Y_0)
       v4f gy;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        gy[u] = gv[u] * gelu_grad(yv[u]);
+        gy[u] = gv[u] * dv[u];
         pb0[u] += gy[u];
       }
 #pragma unroll
