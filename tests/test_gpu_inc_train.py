@@ -29,7 +29,7 @@ from inc_amd.solver_1d import inc_train_supported
 from oracle import BurgersOracle, burgers_params
 from oracle.rollout import simulate_rollout as oracle_rollout
 from tests.conftest import rel_l2
-from tests.test_gpu_burgers import smooth
+from tests.test_gpu_burgers import _forcing, make_pair, smooth
 
 pytestmark = pytest.mark.gpu
 
@@ -110,6 +110,44 @@ def test_inc_training_rollout_matches_fp64_oracle(cuda):
     assert rel_l2(c.double().cpu(), corr.detach()) < 1e-5
     assert rel_l2(gu.double().cpu(), uo.grad) < 5e-5
     for i, (g, pr) in enumerate(zip(gw, cpu_net.parameters())):
+        assert rel_l2(g.double().cpu(), pr.grad) < 5e-5, f"parameter {i} ({tuple(pr.shape)})"
+
+
+@pytest.mark.parametrize("H", [1, 5])
+def test_inc_training_rollout_learning_forcing_per_trajectory_nu(cuda, H):
+    # the trainer's solver (trainer_1d.py:338-358): 'learning' forcing and one
+    # viscosity per trajectory, against the per-step path and the FP64 oracle
+    n, b = 128, 4
+    gen = torch.Generator().manual_seed(40 + H)
+    nu = torch.exp(torch.empty(b, 1, dtype=torch.float64).uniform_(math.log(0.003), math.log(0.03), generator=gen))
+    forcing = _forcing(b, gen)
+    dx = 2 * math.pi / n
+    oracle, gpu = make_pair(n, b, dt=0.3 * min(dx, dx * dx / 0.03), nu=nu, scheme="SSPRK3", eq="learning",
+                            forcing=forcing)
+    u0 = smooth(b, n, gen)
+    target = (0.9 * smooth(b * H, n, gen)).reshape(b, H, n)
+    wc = torch.randn(b, H, n, generator=gen, dtype=torch.float64)
+    net = _net(4, 5, 9, cuda)
+    outs = []
+    for fused in (True, False):
+        sv = gpu(torch.float32)
+        u = u0.to(cuda, torch.float32).requires_grad_(True)
+        net.zero_grad(set_to_none=True)
+        res, corr = simulate_rollout(net, sv, u, H, "INC", fused=fused)
+        _loss(res, corr, target.to(cuda, torch.float32), wc.to(cuda, torch.float32)).backward()
+        outs.append([res.detach(), corr.detach(), u.grad.detach()] + [p.grad.detach().clone() for p in net.parameters()])
+    for i, (g1, g2) in enumerate(zip(*outs)):
+        assert rel_l2(g1, g2) < 2e-5, f"output {i}"
+    cpu_net = copy.deepcopy(net).cpu().double()
+    for cv in cpu_net.convs:
+        cv.backend = "torch"
+    uo = u0.clone().requires_grad_(True)
+    res, corr = oracle_rollout(cpu_net, oracle, uo, H, "INC")
+    _loss(res, corr, target, wc).backward()
+    assert rel_l2(outs[0][0].double().cpu(), res.detach()) < 1e-5
+    assert rel_l2(outs[0][1].double().cpu(), corr.detach()) < 1e-5
+    assert rel_l2(outs[0][2].double().cpu(), uo.grad) < 5e-5
+    for i, (g, pr) in enumerate(zip(outs[0][3:], cpu_net.parameters())):
         assert rel_l2(g.double().cpu(), pr.grad) < 5e-5, f"parameter {i} ({tuple(pr.shape)})"
 
 
