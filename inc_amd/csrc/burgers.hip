@@ -981,12 +981,31 @@ __global__ void __launch_bounds__(kBlock) burgers_inc_fwd_kernel(BArgs<T> a, Cnn
 // per-thread partial sums, and each trajectory writes its own gradient row.
 constexpr int kTrainBlock = 256;
 
-// The exact (erf) GELU and its derivative from one erf: gelu(v) = v Phi(v),
-// gelu'(v) = Phi(v) + v phi(v); returns gelu(v), writes gelu'(v) to d.
+// The GELU and its derivative as the adjoint re-evaluates them: gelu(v) =
+// v Phi(v), gelu'(v) = Phi(v) + v phi(v), Phi(v) = (1 + erf(v / sqrt 2)) / 2,
+// phi(v) = exp(-v^2 / 2) / sqrt(2 pi).  erf by Abramowitz & Stegun 7.1.26
+// (Hastings): erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2),
+// t = 1 / (1 + p x), x >= 0, |error| <= 1.5e-7, odd extension for x < 0.
+// exp(-x^2) at x = |v| / sqrt 2 is phi's exponential as well, so both come
+// from one exp and one reciprocal, with no branch (the library erff takes a
+// divergent branch and ~30 instructions).  The forward kernels keep erff;
+// the recomputed activations differ from theirs by ~1e-7 relative, well
+// inside the gradients' float32 rounding.  Returns gelu(v), writes gelu'(v).
 __device__ __forceinline__ float gelu_with_grad(float v, float& d) {
-  const float ph = 0.5f * (1.f + erff(v * 0.70710678118654752440f));
-  d = ph + v * 0.39894228040143267794f * expf(-0.5f * v * v);
+  const float x = fabsf(v) * 0.70710678118654752440f;
+  const float t = __builtin_amdgcn_rcpf(1.f + 0.3275911f * x);
+  const float e = __expf(-x * x);
+  const float poly =
+      t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+  const float ea = 1.f - poly * e;  // erf(x)
+  const float ph = 0.5f * (1.f + (v < 0.f ? -ea : ea));
+  d = ph + v * 0.39894228040143267794f * e;
   return v * ph;
+}
+
+__device__ __forceinline__ float gelu_adj(float v) {
+  float d;
+  return gelu_with_grad(v, d);
 }
 
 // One wavefront's part of a C -> C circular convolution of `in` (position-
@@ -1231,7 +1250,7 @@ __global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<floa
         }
         *reinterpret_cast<v4f*>(Y + j * CS + 4 * quad) = v4f{y[0], y[1], y[2], y[3]};
         *reinterpret_cast<v4f*>(X + j * CS + 4 * quad) =
-            v4f{gelu_exact(y[0]), gelu_exact(y[1]), gelu_exact(y[2]), gelu_exact(y[3])};
+            v4f{gelu_adj(y[0]), gelu_adj(y[1]), gelu_adj(y[2]), gelu_adj(y[3])};
       }
     }
     __syncthreads();
@@ -1252,7 +1271,7 @@ __global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<floa
             const v4f v = acc[m][t] + bo;
             *reinterpret_cast<v4f*>(Y + (size_t(l) * n + i) * CS + o) = v;
             *reinterpret_cast<v4f*>(X + i * CS + o) =
-                v4f{gelu_exact(v[0]), gelu_exact(v[1]), gelu_exact(v[2]), gelu_exact(v[3])};
+                v4f{gelu_adj(v[0]), gelu_adj(v[1]), gelu_adj(v[2]), gelu_adj(v[3])};
           }
         }
       }
