@@ -216,13 +216,16 @@ class _BurgersIncRollout(torch.autograd.Function):
 
 def inc_train_supported(model, n, dtype):
     """Whether inc_burgers_inc_train_adj runs this network at resolution n:
-    float32, width 32, kernel size 5 or 7, 2..4 layers, n <= 128 and the
-    block's LDS within 160 KiB (the library's own check, restated so that a
-    rollout can choose its path before launching)."""
+    float32 state and parameters on the GPU, width 32, kernel size 5 or 7,
+    2..4 layers, n <= 128 and the block's LDS within 160 KiB (the library's
+    own check, restated so that a rollout can choose its path before
+    launching)."""
     convs = list(model.convs)
     c, k, layers = convs[0].weight.shape[0], convs[0].k, len(convs)
     if dtype != torch.float32 or c != 32 or k not in (5, 7) or not 2 <= layers <= 4 or not k <= n <= 128:
         return False
+    if any(p.dtype != dtype or not p.is_cuda for p in model.parameters()):
+        return False  # the gradients are returned in the parameters' own dtype
     nh = layers - 2
     ws = nh * c * (c * k + 4)
     body = ws + (nh + 3) * n * (c + 4) + 2 * n
