@@ -214,23 +214,31 @@ class _BurgersIncRollout(torch.autograd.Function):
         return (g_u, None, None, None, None, *grads)
 
 
-def inc_train_supported(model, n, dtype):
-    """Whether inc_burgers_inc_train_adj runs this network at resolution n:
-    float32 state and parameters on the GPU, width 32, kernel size 5 or 7,
-    2..4 layers, n <= 128 and the block's LDS within 160 KiB (the library's
-    own check, restated so that a rollout can choose its path before
-    launching)."""
-    convs = list(model.convs)
-    c, k, layers = convs[0].weight.shape[0], convs[0].k, len(convs)
-    if dtype != torch.float32 or c != 32 or k not in (5, 7) or not 2 <= layers <= 4 or not k <= n <= 128:
+def inc_train_shape_supported(channels, kernel, layers, n):
+    """The shapes inc_burgers_inc_train_adj runs: width 32, kernel size 5 or
+    7, 2..4 layers, 8 <= n <= 128, and the block's LDS (hidden weights,
+    layers + 1 activation arrays of [n][C + 4], two [n] arrays; the final
+    reduction's [4][C/4][V] partials) within 160 KiB less the static part.
+    The library's own check, restated so that a rollout can choose its path
+    before launching."""
+    c, k = channels, kernel
+    if c != 32 or k not in (5, 7) or not 2 <= layers <= 4 or not max(k, 8) <= n <= 128:
         return False
-    if any(p.dtype != dtype or not p.is_cuda for p in model.parameters()):
-        return False  # the gradients are returned in the parameters' own dtype
     nh = layers - 2
     ws = nh * c * (c * k + 4)
     body = ws + (nh + 3) * n * (c + 4) + 2 * n
     red = ws + 4 * (c // 4) * ((4 * k + 4) + (4 * k + 1) + 4 * nh)
     return 4 * max(body, red) <= 160 * 1024 - 256
+
+
+def inc_train_supported(model, n, dtype):
+    """Whether the fused INC training rollout runs this CorrectionCNN at
+    resolution n: float32 state and parameters on the GPU and a shape
+    inc_train_shape_supported accepts."""
+    convs = list(model.convs)
+    if dtype != torch.float32 or any(p.dtype != dtype or not p.is_cuda for p in model.parameters()):
+        return False  # the gradients are returned in the parameters' own dtype
+    return inc_train_shape_supported(convs[0].weight.shape[0], convs[0].k, len(convs), n)
 
 
 class _BurgersAdaptive(torch.autograd.Function):

@@ -118,3 +118,26 @@ def test_product_refuses_cpu_tensors():
         inc_amd.weno_reconstruction(torch.zeros(2, 16), "left")
     with pytest.raises(ValueError):
         inc_amd.weno_reconstruction(torch.zeros(2, 16), "up")
+
+
+def test_fused_training_shape_rule_matches_the_library():
+    # The Python predicate that routes a training rollout to the fused kernels
+    # must accept exactly what inc_burgers_inc_train_adj accepts.  With batch 0
+    # the library validates every argument and returns before any launch.
+    from inc_amd.solver_1d import inc_train_shape_supported
+    lib = L.load()
+    seen = {True: 0, False: 0}
+    for c in (16, 32, 48):
+        for k in (3, 5, 7):
+            for layers in (2, 3, 4, 5):
+                for n in (8, 16, 37, 64, 65, 96, 100, 112, 120, 128, 129, 256):
+                    d = L.BurgersDesc(dtype=L.INC_F32, batch=0, n=n, dx=0.1, dt=0.01, scheme=L.INC_SSPRK3)
+                    cn = L.CnnDesc(layers=layers, channels=c, kernel=k, scale=1.0)
+                    for i in range(layers):
+                        cn.weight[i] = 1
+                    rc = lib.inc_burgers_inc_train_adj(ctypes.byref(d), ctypes.byref(cn), 1, None, None, 1, 1, 1,
+                                                       None)
+                    want = inc_train_shape_supported(c, k, layers, n)
+                    assert (rc == L.INC_OK) == want, (c, k, layers, n, rc, lib.inc_last_error())
+                    seen[want] += 1
+    assert seen[True] > 10 and seen[False] > 10
