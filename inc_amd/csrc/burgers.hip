@@ -1015,9 +1015,10 @@ __device__ __forceinline__ float gelu_adj(float v) {
 // TRANS == true (the input gradient): D[c][i] = sum_{k,o} W[o][c][k]
 // in[i - k + P][o], i.e. the flipped transposed kernel, whose A operand is
 // read from the same staged weights ws[o][k*C + c] (row stride C*K + 4)
-// with four scalar reads per chunk.  Positions past n are computed from
-// clamped rows and dropped by the caller.
-template <int C, int K, int PW, bool TRANS>
+// with four scalar reads per chunk.  HALO: `in` carries P wrapped rows on
+// either side (rows -P .. n + P - 1), so no row index wraps.  Positions past
+// n are computed from clamped rows and dropped by the caller.
+template <int C, int K, int PW, bool TRANS, bool HALO>
 __device__ __forceinline__ void conv_tiles(const float* ws, const float* in, int n, v4f (&acc)[C / 16][PW]) {
   constexpr int CT = C / 16, P = K / 2, CS = C + 4, RS = C * K + 4, NCH = C * K / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1044,7 +1045,8 @@ __device__ __forceinline__ void conv_tiles(const float* ws, const float* in, int
         fa[m] = v4f{wl[m * 16], wl[RS + m * 16], wl[2 * RS + m * 16], wl[3 * RS + m * 16]};
     }
 #pragma unroll
-    for (int t = 0; t < PW; ++t) fb[t] = *reinterpret_cast<const v4f*>(in + row(rb[t] + k) * CS + c0 + 4 * q);
+    for (int t = 0; t < PW; ++t)
+      fb[t] = *reinterpret_cast<const v4f*>(in + (HALO ? rb[t] + kw : row(rb[t] + k)) * CS + c0 + 4 * q);
   };
   auto next = [&]() __attribute__((always_inline)) {
     c0 += 16;
@@ -1089,20 +1091,19 @@ __device__ __forceinline__ void conv_tiles(const float* ws, const float* in, int
 // go round the four wavefronts (tile W + 4t of wavefront W); lane (lr, q)
 // holds D[16m + 4q + r][16ct + lr].  The reduction runs over positions, four
 // per MFMA, all tiles of the wavefront interleaved.  W is a template
-// argument so that every tile's row, column and tap are constants: per four
-// positions the lane computes one row offset per tap and each operand is
-// one LDS read at a constant offset from it.
+// argument so that every tile's row, column and tap are constants, and x
+// carries P wrapped rows on either side (rows -P .. n + P - 1): each operand
+// is one LDS read at a constant offset from the lane's row, with no wrap.
 template <int C, int K, int TPW, int W>
 __device__ __forceinline__ void wgrad_wave(const float* gy, const float* x, int n, v4f (&acc)[TPW]) {
   constexpr int P = K / 2, CS = C + 4, NCOL = C * K / 16, TT = (C / 16) * NCOL;
   const int lane = threadIdx.x & 63, lr = lane & 15, q = lane >> 4;
-  auto row = [&](int i) { return i < 0 ? i + n : (i >= n ? i - n : i); };
   auto quad = [&](int j0, bool tail) __attribute__((always_inline)) {
     const int j = j0 + q, jc = tail ? min(j, n - 1) : j;
     const float* ga = gy + jc * CS + lr;
     const float* xb[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) xb[k] = x + row(jc + k - P) * CS + lr;
+    for (int k = 0; k < K; ++k) xb[k] = x + (jc + k - P) * CS + lr;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
       const int id = W + 4 * t;
@@ -1155,13 +1156,21 @@ __global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<floa
   const bool own = tid < n;
   float* ws = reinterpret_cast<float*>(smem_raw);  // [NH][C][RS]
   float* Y = ws + NH * C * RS;                      // [NH + 1][n][CS]: pre-activations, then their cotangents
-  float* X = Y + (NH + 1) * n * CS;                 // [n][CS]: activations (the solver's scratch before them)
-  float* G = X + n * CS;                            // [n][CS]: cotangent of the activations
+  float* Xb = Y + (NH + 1) * n * CS;                // [n + 2P][CS]: activations (the solver's scratch before them)
+  float* X = Xb + P * CS;                           // row j of the activations, rows -P .. n + P - 1 (wrapped)
+  float* G = Xb + (n + 2 * P) * CS;                 // [n][CS]: cotangent of the activations
   float* un = G + n * CS;                           // [n]: the step's input state
   float* gb = un + n;                               // [n]: cotangent of the network's output
-  float* us = X;
-  float* cs = X + n;
-  float* Cw = X + 2 * n;
+  float* us = Xb;
+  float* cs = Xb + n;
+  float* Cw = Xb + 2 * n;
+  // an activation row goes to X with its wrapped copy when it lies within P
+  // of either end (n >= 8 > 2P: at most one copy)
+  auto xstore = [&](int j, int col, v4f v) __attribute__((always_inline)) {
+    *reinterpret_cast<v4f*>(X + j * CS + col) = v;
+    if (j < P) *reinterpret_cast<v4f*>(X + (j + n) * CS + col) = v;
+    if (j >= n - P) *reinterpret_cast<v4f*>(X + (j - n) * CS + col) = v;
+  };
   for (int e = tid; e < NH * C * C; e += kTrainBlock) {
     const int l = e / (C * C), oc = e - l * C * C, o = oc / C, cc = oc - o * C;
     const float* src = cn.w[l + 1] + size_t(oc) * K;
@@ -1249,15 +1258,14 @@ __global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<floa
           for (int u = 0; u < 4; ++u) y[u] += w0[(4 * quad + u) * K + k] * uv;
         }
         *reinterpret_cast<v4f*>(Y + j * CS + 4 * quad) = v4f{y[0], y[1], y[2], y[3]};
-        *reinterpret_cast<v4f*>(X + j * CS + 4 * quad) =
-            v4f{gelu_adj(y[0]), gelu_adj(y[1]), gelu_adj(y[2]), gelu_adj(y[3])};
+        xstore(j, 4 * quad, v4f{gelu_adj(y[0]), gelu_adj(y[1]), gelu_adj(y[2]), gelu_adj(y[3])});
       }
     }
     __syncthreads();
 #pragma unroll
     for (int l = 1; l <= NH; ++l) {
       v4f acc[C / 16][PW];
-      conv_tiles<C, K, PW, false>(ws + size_t(l - 1) * C * RS, X, n, acc);
+      conv_tiles<C, K, PW, false, true>(ws + size_t(l - 1) * C * RS, X, n, acc);
       __syncthreads();  // every wavefront has read X
       const int lr = lane & 15, q = lane >> 4;
 #pragma unroll
@@ -1270,8 +1278,7 @@ __global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<floa
           if (i < n) {
             const v4f v = acc[m][t] + bo;
             *reinterpret_cast<v4f*>(Y + (size_t(l) * n + i) * CS + o) = v;
-            *reinterpret_cast<v4f*>(X + i * CS + o) =
-                v4f{gelu_adj(v[0]), gelu_adj(v[1]), gelu_adj(v[2]), gelu_adj(v[3])};
+            xstore(i, o, v4f{gelu_adj(v[0]), gelu_adj(v[1]), gelu_adj(v[2]), gelu_adj(v[3])});
           }
         }
       }
@@ -1328,12 +1335,12 @@ __global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<floa
         }
         *reinterpret_cast<v4f*>(Yl + j * CS + 4 * quad) = gy;
         *reinterpret_cast<v4f*>(Yp + j * CS + 4 * quad) = v4f{dp[0], dp[1], dp[2], dp[3]};
-        *reinterpret_cast<v4f*>(X + j * CS + 4 * quad) = v4f{ap[0], ap[1], ap[2], ap[3]};
+        xstore(j, 4 * quad, v4f{ap[0], ap[1], ap[2], ap[3]});
       }
       __syncthreads();
       wgrad_tiles<C, K, TPW>(Yl, X, n, wacc[l - 1]);
       v4f acc[C / 16][PW];
-      conv_tiles<C, K, PW, true>(ws + size_t(l - 1) * C * RS, Yl, n, acc);
+      conv_tiles<C, K, PW, true, false>(ws + size_t(l - 1) * C * RS, Yl, n, acc);
       const int lr = lane & 15, q = lane >> 4;
 #pragma unroll
       for (int m = 0; m < C / 16; ++m)
@@ -2272,13 +2279,13 @@ void* inc_adj_kernel_k(int nh, int pw) {
 }
 
 // LDS of burgers_inc_adj_kernel: staged hidden weights, nh + 1 pre-activation
-// arrays, activations and their cotangent ([n][C + 4] each), the input state
-// and the output's cotangent; the final reduction reuses it from the
-// pre-activations on.
+// arrays, activations ([n + K - 1][C + 4], with wrapped rows) and their
+// cotangent ([n][C + 4]), the input state and the output's cotangent; the
+// final reduction reuses it from the pre-activations on.
 size_t inc_adj_lds(int n, int C, int K, int nh) {
   const size_t cs = C + 4, rs = size_t(C) * K + 4, ws = size_t(nh) * C * rs;
   const size_t v = (4 * K + 4) + (4 * K + 1) + 4 * nh;
-  const size_t body = ws + size_t(nh + 3) * n * cs + 2 * size_t(n);
+  const size_t body = ws + size_t(nh + 3) * n * cs + size_t(K / 2) * 2 * cs + 2 * size_t(n);
   const size_t red = ws + 4 * size_t(C / 4) * v;
   return (body > red ? body : red) * sizeof(float);
 }

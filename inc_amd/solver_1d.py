@@ -217,7 +217,8 @@ class _BurgersIncRollout(torch.autograd.Function):
 def inc_train_shape_supported(channels, kernel, layers, n):
     """The shapes inc_burgers_inc_train_adj runs: width 32, kernel size 5 or
     7, 2..4 layers, 8 <= n <= 128, and the block's LDS (hidden weights,
-    layers + 1 activation arrays of [n][C + 4], two [n] arrays; the final
+    layers + 1 activation arrays of [n][C + 4] and 2 (K // 2) wrapped rows,
+    two [n] arrays; the final
     reduction's [4][C/4][V] partials) within 160 KiB less the static part.
     The library's own check, restated so that a rollout can choose its path
     before launching."""
@@ -226,7 +227,7 @@ def inc_train_shape_supported(channels, kernel, layers, n):
         return False
     nh = layers - 2
     ws = nh * c * (c * k + 4)
-    body = ws + (nh + 3) * n * (c + 4) + 2 * n
+    body = ws + (nh + 3) * n * (c + 4) + (k // 2) * 2 * (c + 4) + 2 * n
     red = ws + 4 * (c // 4) * ((4 * k + 4) + (4 * k + 1) + 4 * nh)
     return 4 * max(body, red) <= 160 * 1024 - 256
 
