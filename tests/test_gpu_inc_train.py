@@ -11,11 +11,11 @@ FP64 oracle solver driven by a CPU float64 copy of the network, both
 differentiated by torch autograd.  The loss weights the trajectory and the
 corrections, so both cotangents reach the kernel.  Shapes cover 2, 3 and 4
 layers, kernel sizes 5 and 7, one and two position tiles per wavefront,
-resolutions that are not multiples of 16, Euler and SSP-RK3, and the
-training workload's batch (512 x 128, 16 steps).  The kernel runs float32;
-tolerances (relative L2): trajectory and corrections 1e-5; gradients 2e-5
-against the per-step float32 path (measured on MI355X: 9e-8 .. 2.1e-6) and
-5e-5 against the FP64 oracle.
+resolutions that are not multiples of 16, the smallest resolutions (8, 16),
+Euler and SSP-RK3, and the training workload's batch (512 x 128, 16 steps).
+The kernel runs float32; tolerances (relative L2): trajectory and
+corrections 1e-5; gradients 2e-5 against the per-step float32 path
+(measured on MI355X: 9e-8 .. 2.1e-6) and 5e-5 against the FP64 oracle.
 """
 import copy
 import math
@@ -65,8 +65,11 @@ def _run(net, u0, n, b, H, scheme, target, wc, fused):
 
 
 # (n, batch, kernel, depth, scheme, horizon)
+# n = 8 and 16 are the smallest resolutions (the wrapped halo rows of the
+# activations then cover a quarter or more of the grid)
 CASES = [(128, 6, 5, 4, "SSPRK3", 7), (100, 3, 7, 3, "SSPRK3", 6), (64, 5, 5, 2, "Euler", 5),
-         (48, 4, 7, 4, "Euler", 6), (37, 2, 5, 3, "SSPRK3", 4), (128, 512, 5, 4, "SSPRK3", 16)]
+         (48, 4, 7, 4, "Euler", 6), (37, 2, 5, 3, "SSPRK3", 4), (128, 512, 5, 4, "SSPRK3", 16),
+         (8, 1, 5, 3, "SSPRK3", 3), (16, 2, 7, 4, "Euler", 2)]
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "n{}_b{}_k{}_d{}_{}_h{}".format(*c))
