@@ -989,7 +989,7 @@ constexpr int kTrainBlock = 256;
 // exp(-x^2) at x = |v| / sqrt 2 is phi's exponential as well, so both come
 // from one exp and one reciprocal, with no branch (the library erff takes a
 // divergent branch and ~30 instructions).  The forward kernels keep erff;
-// the recomputed activations differ from theirs by ~1e-7 relative, well
+// the recomputed activations differ from theirs by a few 1e-7, well
 // inside the gradients' float32 rounding.  Returns gelu(v), writes gelu'(v).
 __device__ __forceinline__ float gelu_with_grad(float v, float& d) {
   const float x = fabsf(v) * 0.70710678118654752440f;
