@@ -139,8 +139,15 @@ class CircularConv1d(nn.Module):
 
 
 class CorrectionCNN(nn.Module):
-    def __init__(self, depth=4, width=32, k=5, generator=None, backend="hip"):
+    """s = scale * conv_{depth-1}(gelu(... gelu(conv_0(u)) ...)): circular
+    Conv1d layers 1 -> width -> ... -> width -> 1 with exact GELU between them,
+    He-normal weights and zero biases.  ``scale`` multiplies the output (a
+    correction kept small against the dynamics, as in ``0.01 * model(u)``);
+    the fused rollout kernels apply it themselves."""
+
+    def __init__(self, depth=4, width=32, k=5, generator=None, backend="hip", scale=1.0):
         super().__init__()
+        self.scale = float(scale)
         layers = []
         cin = 1
         for i in range(depth):
@@ -165,7 +172,7 @@ class CorrectionCNN(nn.Module):
         if (convs[0].backend != "hip" or not 2 <= len(convs) <= 8 or c not in (32, 48) or k not in (5, 7)
                 or any(cv.k != k for cv in convs)):
             return None
-        d = _lib.CnnDesc(layers=len(convs), channels=c, kernel=k, scale=1.0)
+        d = _lib.CnnDesc(layers=len(convs), channels=c, kernel=k, scale=self.scale)
         keep = []
         for i, cv in enumerate(convs):
             w = cv.weight.detach().to(dtype).contiguous()
@@ -179,10 +186,12 @@ class CorrectionCNN(nn.Module):
         if self.convs[0].backend == "hip":
             params = [t for conv in self.convs for t in (conv.weight, conv.bias)]
             _lib.require_gpu(u, *params)
-            return _CNN.apply(u, *params)
-        x = u.unsqueeze(1)
-        for i, conv in enumerate(self.convs):
-            x = conv(x)
-            if i < len(self.convs) - 1:
-                x = self.act(x)
-        return x.squeeze(1)
+            s = _CNN.apply(u, *params)
+        else:
+            x = u.unsqueeze(1)
+            for i, conv in enumerate(self.convs):
+                x = conv(x)
+                if i < len(self.convs) - 1:
+                    x = self.act(x)
+            s = x.squeeze(1)
+        return s if self.scale == 1.0 else self.scale * s

@@ -56,9 +56,11 @@ def cos_modes(b, n, L, seed, kmax, sd=0.5):
     return u
 
 
-def row(name, tf, horizons, f64, grad=None):
+def row(name, tf, horizons, f64, grad=None, wgrad=None):
     hz = ", ".join(f"{h}: {e:.1e}" for h, e in horizons)
     g = "" if grad is None else f"{grad:.1e}"
+    if wgrad is not None:
+        g += f" (weights: {wgrad:.1e})"
     return f"| {name} | {tf:.1e} | {hz} | {f64:.1e} | {g} |"
 
 
@@ -104,7 +106,10 @@ def ks_plain():
 
 
 def burgers_inc():
-    """Burgers INC: N=128, nu=0.01, dt=0.01, 16 steps, CNN(4x32,k5); oracle on 8 trajectories."""
+    """Burgers INC: N=128, nu=0.01, dt=0.01, 16 steps, CNN(4x32,k5) with output scale 0.1; oracle on
+    8 trajectories.  float32 runs the network inside the rollout and adjoint kernels (the training
+    path); its gradient column gives the initial state's and, in brackets, the weights' (worst
+    parameter tensor)."""
     n, b, H = 128, 8, 16
     u0 = sine_states(b, n, 2000)
     cnn = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(2), backend="torch").double()
@@ -122,16 +127,18 @@ def burgers_inc():
         ((traj - target) ** 2).mean().backward()
         return traj.detach(), u.grad
     ref, gref = oracle_run()
+    wref = [p.grad.clone() for p in cnn.parameters()]
     res = {}
     for dt_ in (torch.float32, torch.float64):
-        net = CorrectionCNN(4, 32, 5).to(DEV, dt_)
+        net = CorrectionCNN(4, 32, 5, scale=0.1).to(DEV, dt_)
         net.load_state_dict(cnn.state_dict())
         sv = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=0.01, resolution=n, nu=0.01, batch_size=b,
                                                             time_scheme="SSPRK3", device="cuda"))
         u = u0.to(DEV, dt_).requires_grad_(True)
-        traj, _ = inc_amd.simulate_rollout(lambda v: 0.1 * net(v), sv, u, H, "INC")
+        traj, _ = inc_amd.simulate_rollout(net, sv, u, H, "INC")
         ((traj - target.to(DEV, dt_)) ** 2).mean().backward()
-        res[dt_] = (traj.detach().cpu(), u.grad.cpu())
+        res[dt_] = (traj.detach().cpu(), u.grad.cpu(),
+                    max(rel(p.grad.cpu(), w) for p, w in zip(net.parameters(), wref)))
     sv = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=0.01, resolution=n, nu=0.01, batch_size=b,
                                                         time_scheme="SSPRK3", device="cuda"))
     net32 = CorrectionCNN(4, 32, 5).to(DEV)
@@ -148,7 +155,7 @@ def burgers_inc():
         prev = ref[:, k]
     hz = [(h, rel(res[torch.float32][0][:, h - 1], ref[:, h - 1])) for h in (1, 8, 16)]
     return row("Burgers INC N=128, 16 steps (fwd+adj)", tf, hz, rel(res[torch.float64][0], ref),
-               rel(res[torch.float32][1], gref))
+               rel(res[torch.float32][1], gref), res[torch.float32][2])
 
 
 def ks_inc():

@@ -74,11 +74,14 @@ def test_inc_rollout_in_kernel_matches_per_step_path(cuda, case, dtype):
     assert abs(sv1.t - sv2.t) < 1e-12
 
 
-def test_inc_rollout_in_kernel_fp64_matches_oracle(cuda):
+@pytest.mark.parametrize("scale", [1.0, 0.1])
+def test_inc_rollout_in_kernel_fp64_matches_oracle(cuda, scale):
+    # scale: the network's output scale (CorrectionCNN(scale=...)), applied in the kernel
     n, b, H = 96, 3, 6
     gen = torch.Generator().manual_seed(5)
     u0 = smooth(b, n, gen)
     net = _net(4, 32, 5, 17, torch.float64, cuda)
+    net.scale = scale
     cpu_net = copy.deepcopy(net).cpu()
     for cv in cpu_net.convs:
         cv.backend = "torch"

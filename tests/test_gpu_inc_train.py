@@ -34,9 +34,9 @@ from tests.test_gpu_burgers import _forcing, make_pair, smooth
 pytestmark = pytest.mark.gpu
 
 
-def _net(depth, k, seed, device):
+def _net(depth, k, seed, device, scale=1.0):
     gen = torch.Generator().manual_seed(seed)
-    net = CorrectionCNN(depth, 32, k, generator=gen)
+    net = CorrectionCNN(depth, 32, k, generator=gen, scale=scale)
     with torch.no_grad():
         for cv in net.convs:
             cv.bias.copy_(0.1 * torch.randn(cv.bias.shape, generator=gen))
@@ -92,13 +92,15 @@ def test_inc_training_rollout_matches_per_step_path(cuda, case):
     assert rel_l2(sv1.correction_s.detach(), sv2.correction_s.detach()) < 1e-5
 
 
-def test_inc_training_rollout_matches_fp64_oracle(cuda):
+@pytest.mark.parametrize("scale", [1.0, 0.1])
+def test_inc_training_rollout_matches_fp64_oracle(cuda, scale):
+    # scale: the network's output scale (CorrectionCNN(scale=...)), applied in the kernels
     n, b, H, k, depth = 96, 3, 6, 5, 4
     gen = torch.Generator().manual_seed(21)
     u0 = smooth(b, n, gen)
     target = (0.9 * smooth(b * H, n, gen)).reshape(b, H, n)
     wc = torch.randn(b, H, n, generator=gen)
-    net = _net(depth, k, 5, cuda)
+    net = _net(depth, k, 5, cuda, scale)
     cpu_net = copy.deepcopy(net).cpu().double()
     for cv in cpu_net.convs:
         cv.backend = "torch"
@@ -155,11 +157,11 @@ def test_inc_training_rollout_learning_forcing_per_trajectory_nu(cuda, H):
 
 
 def test_inc_training_rollout_gradient_of_corrections_only(cuda):
-    # a loss on the corrections alone: only g_corr reaches the kernel
+    # a loss on the corrections alone: only g_corr reaches the kernel (scaled output)
     n, b, H = 64, 3, 5
     gen = torch.Generator().manual_seed(4)
     u0 = smooth(b, n, gen).to(cuda, torch.float32)
-    net = _net(3, 5, 8, cuda)
+    net = _net(3, 5, 8, cuda, scale=0.05)
     out = []
     for fused in (True, False):
         sv, _ = _solver(n, b, "SSPRK3")
