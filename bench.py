@@ -19,8 +19,10 @@ its duration, against the 157.3 TFLOP/s vector peak.  cpu_baseline: the FP64
 oracle (PyTorch CPU restatement of the same scheme) on a bounded sample,
 fwd+adj.  other_workloads (not the metric): the same kernels at saturation,
 the rollout with its MSE loss fused in against torch's MSELoss, Burgers at
-saturation, and the INC training steps (Burgers, KS with checkpointing,
-ragged Burgers).
+saturation, the INC training steps (Burgers with the correction network inside
+the rollout and adjoint kernels, against the per-step path; KS with
+checkpointing; ragged Burgers), and the Burgers INC evaluation rollout with the
+network inside the kernel.
 """
 import argparse
 import json
