@@ -53,3 +53,20 @@ def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
     assert autotune.ks_radix_candidates(256, torch.float64) == [2, 4]
     assert autotune.ks_radix_candidates(1024, torch.float64) == [4]
     assert 16 not in autotune.burgers_team_candidates(4096)
+
+
+def test_correction_cnn_output_scale():
+    # CorrectionCNN(scale=s) is s * CorrectionCNN() on the same weights (torch backend on the CPU),
+    # and its gradients scale with it
+    from inc_amd.correction import CorrectionCNN
+    gen = torch.Generator().manual_seed(0)
+    a = CorrectionCNN(3, 32, 5, generator=gen, backend="torch", scale=0.05).double()
+    b = CorrectionCNN(3, 32, 5, backend="torch").double()
+    b.load_state_dict(a.state_dict())
+    u = torch.randn(2, 24, generator=gen, dtype=torch.float64)
+    ya, yb = a(u), b(u)
+    assert torch.allclose(ya, 0.05 * yb, rtol=1e-14, atol=0)
+    ya.sum().backward()
+    yb.sum().backward()
+    for pa, pb in zip(a.parameters(), b.parameters()):
+        assert torch.allclose(pa.grad, 0.05 * pb.grad, rtol=1e-12, atol=1e-300)
