@@ -1114,9 +1114,15 @@ __device__ __forceinline__ void wgrad_wave(const float* gy, const float* x, int 
       acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, xb[k][c0], acc[t], 0, 0, 0);
     }
   };
-  const int n4 = n & ~3;
-#pragma unroll 2
-  for (int j0 = 0; j0 < n4; j0 += 4) quad(j0, false);
+  // two quads per iteration, so the second's reads can issue under the
+  // first's MFMAs; then the remaining whole quad and the ragged tail
+  const int n4 = n & ~3, n8 = n & ~7;
+  int j0 = 0;
+  for (; j0 < n8; j0 += 8) {
+    quad(j0, false);
+    quad(j0 + 4, false);
+  }
+  if (j0 < n4) quad(j0, false);
   if (n4 < n) quad(n4, true);
 }
 
@@ -2288,10 +2294,6 @@ size_t inc_adj_lds(int n, int C, int K, int nh) {
   const size_t body = ws + size_t(nh + 3) * n * cs + size_t(K / 2) * 2 * cs + 2 * size_t(n);
   const size_t red = ws + 4 * size_t(C / 4) * v;
   return (body > red ? body : red) * sizeof(float);
-}
-
-int inc_adj_params(int C, int K, int layers) {
-  return 2 * C * K + C + 1 + (layers - 2) * (C * C * K + C);
 }
 
 int inc_train_adj_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void* stages, const void* g_traj,
