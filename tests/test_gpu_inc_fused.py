@@ -107,3 +107,28 @@ def test_inc_rollout_shapes_outside_the_kernel_take_the_per_step_path(cuda):
     sv2, _ = _solver(n, b, "SSPRK3", torch.float64)
     res2, _ = simulate_rollout(net, sv2, u0.clone().requires_grad_(True), H, "INC", fused=False)
     assert rel_l2(res1, res2.detach()) < 1e-12
+
+
+@pytest.mark.parametrize("team", [128, 256])
+def test_inc_rollout_in_kernel_explicit_team(cuda, team):
+    # an explicit team (sim_params.team) changes the block shape and the matrix-core tiling (one
+    # wavefront holds 16 * PW positions): same rollout as the automatic choice, with and without gradient
+    n, b, H = 128, 4, 5
+    gen = torch.Generator().manual_seed(31)
+    u0 = smooth(b, n, gen).to(cuda, torch.float32)
+    net = _net(4, 32, 5, 7, torch.float32, cuda)
+    outs = []
+    for t in (0, team):
+        sv, p = _solver(n, b, "SSPRK3", torch.float32)
+        sv.team = t
+        with torch.no_grad():
+            res, corr = simulate_rollout(net, sv, u0, H, "INC")
+        sv2, _ = _solver(n, b, "SSPRK3", torch.float32)
+        sv2.team = t
+        u = u0.clone().requires_grad_(True)
+        net.zero_grad(set_to_none=True)
+        r2, c2 = simulate_rollout(net, sv2, u, H, "INC")
+        (r2.square().mean() + c2.mean()).backward()
+        outs.append([res, corr, r2.detach(), u.grad] + [q.grad.clone() for q in net.parameters()])
+    for i, (a, c) in enumerate(zip(*outs)):
+        assert rel_l2(a, c) < 1e-6, i
