@@ -2211,10 +2211,14 @@ int inc_fwd_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void
               "in-kernel correction network: width 32 or 48, kernel size 5 or 7 (got %d, %d)", cd->channels,
               cd->kernel);
   INC_REQUIRE(cd->kernel <= d->n, INC_EINVAL, "kernel size exceeds the resolution");
-  // float: up to 2 points per thread; double: 1 (two would spill registers)
+  // float: up to 2 points per thread; double: 1 (two would spill registers).
+  // The automatic choice keeps one point per thread while a 256-thread team
+  // covers n: more points per thread shrink the team, and the network's
+  // tiles then crowd onto fewer wavefronts (B=4096, N=128: 23.4 ms for 100
+  // steps at one point per thread against 40.0 ms at two).
   constexpr int kPptMax = std::is_same<T, float>::value ? 2 : 1;
   int team = 0;
-  const int ppt = pick_ppt(d->n, d->team, &team, d->batch, kPptMax);
+  const int ppt = pick_ppt(d->n, d->team, &team, d->batch, d->n <= kBlock ? 1 : kPptMax);
   INC_REQUIRE(ppt >= 1 && ppt <= kPptMax, INC_EUNSUPPORTED,
               "in-kernel correction network: n=%d needs at most %d point(s) per thread of a 256-thread team", d->n,
               kPptMax);
@@ -2225,12 +2229,25 @@ int inc_fwd_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void
   int pw = 0;
   if (std::is_same<T, float>::value && cd->layers > 2) {
     const int C = cd->channels, K = cd->kernel;
-    const int tiles = (d->n + 15) / 16, waves = team / kWave;
-    int need = (tiles + waves - 1) / waves;
-    need = need <= 1 ? 1 : (need <= 2 ? 2 : (need <= 4 ? 4 : 8));
+    const int tiles = (d->n + 15) / 16;
+    auto pw_for = [&](int tm) {  // 16-position tiles per wavefront, rounded up to an instance's
+      const int waves = tm / kWave, need = (tiles + waves - 1) / waves;
+      return need <= 1 ? 1 : (need <= 2 ? 2 : (need <= 4 ? 4 : 8));
+    };
     const size_t lm = (size_t(2) * d->n + size_t(cd->layers - 2) * C * (C * K + 4) + size_t(d->n) * (C + 4)) *
                       sizeof(float);
-    if (need <= 8 && lm <= 150 * 1024 && inc_fwd_mfma_kernel(ppt, C, K, need) != nullptr) {
+    // An automatic team widens to 256 threads when the batch would leave
+    // SIMDs without a wavefront: the network's tiles then spread over four
+    // wavefronts (the solver's extra threads idle through its stages).
+    // Measured on MI355X, N=64: B=256 1.59x faster; at one wavefront per
+    // SIMD or more the narrower team is as fast or faster (B=4096, N=128:
+    // 2x), so it stays.
+    const long long waves_now = static_cast<long long>(d->batch) * (team / kWave);
+    if (d->team == 0 && team < kBlock && waves_now < simd_count() && lm <= 150 * 1024 &&
+        inc_fwd_mfma_kernel(ppt, C, K, pw_for(kBlock)))
+      team = kBlock;
+    const int need = pw_for(team);
+    if (lm <= 150 * 1024 && inc_fwd_mfma_kernel(ppt, C, K, need) != nullptr) {
       pw = need;
       lds = lm;
     }
