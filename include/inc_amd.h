@@ -13,7 +13,8 @@
  *    float32 (INC_F32) or float64 (INC_F64) for every floating array of a
  *    call, except the KS domain sizes, which are always float64.
  *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = the
- *    null stream), allocates nothing and is safe under hipGraph capture.
+ *    null stream), allocates nothing and only enqueues kernels on it (no
+ *    host synchronisation).  Capture into a hipGraph is not validated.
  *  - Return value: INC_OK (0) or an error code; inc_last_error() returns a
  *    thread-local message for the last failing call of the calling thread.
  *  - Calls that overlap on one stream are ordered; calls on different
@@ -191,8 +192,10 @@ int inc_burgers_inc_fwd(const inc_burgers_desc* d, const inc_cnn_desc* cnn, cons
  * the batch gradient is the sum of the rows.  The network is evaluated again
  * at every step's input state inside the kernel; no activation is stored.
  * Supported by the adjoint: float, WENO convection, fixed time step, C = 32,
- * K = 5 or 7, 2..4 layers, n <= 128 and the block's LDS (the hidden layers'
- * weights and (L + 1) * n * (C + 4) activations) within 160 KiB.
+ * K = 5 or 7, 2..4 layers, 8 <= n <= 128 and the block's LDS within 160 KiB:
+ * the hidden layers' weights (L - 2) * C * (C K + 4) and the activations
+ * ((L + 1) n + K - 1) * (C + 4) + 2 n floats.  Other shapes return
+ * INC_EUNSUPPORTED.
  */
 int inc_burgers_inc_train_fwd(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* u,
                               void* u_out, int steps, void* traj, void* corr, void* stages,
