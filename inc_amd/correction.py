@@ -18,12 +18,13 @@ from torch.nn import functional as F
 from . import _lib
 
 
-def _conv_fwd(x, w, b, act=_lib.INC_ACT_NONE):
-    """(act(conv(x) + b), act'(conv(x) + b) or None)."""
+def _conv_fwd(x, w, b, act=_lib.INC_ACT_NONE, need_dy=True):
+    """(act(conv(x) + b), act'(conv(x) + b) or None); need_dy=False skips the
+    derivative (no gradient wanted)."""
     bsz, ci, n = x.shape
     co, _, k = w.shape
     y = x.new_empty(bsz, co, n)
-    dy = x.new_empty(bsz, co, n) if act != _lib.INC_ACT_NONE else None
+    dy = x.new_empty(bsz, co, n) if act != _lib.INC_ACT_NONE and need_dy else None
     rc = _lib.load().inc_conv1d_fwd_act(_lib.dtype_code(x.dtype), act, bsz, ci, co, n, k, _lib.ptr(x), _lib.ptr(w),
                                         _lib.ptr(b), _lib.ptr(y), _lib.ptr(dy), _lib.stream())
     _lib.check(rc, "circular conv1d")
@@ -82,6 +83,17 @@ def circular_conv1d(x, weight, bias=None):
     (torch.nn.Conv1d(padding=K//2, padding_mode='circular') semantics)."""
     _lib.require_gpu(x, weight, bias)
     return _CircConv1d.apply(x, weight, bias)
+
+
+def _cnn_eval(u, ws, bs):
+    """The network without gradient: every layer's GELU fused into its
+    convolution, no derivative computed or stored."""
+    x = u.unsqueeze(1).contiguous()
+    depth = len(ws)
+    for i in range(depth):
+        act = _lib.INC_ACT_GELU if i < depth - 1 else _lib.INC_ACT_NONE
+        x, _ = _conv_fwd(x, ws[i].contiguous(), bs[i].contiguous(), act, need_dy=False)
+    return x.squeeze(1)
 
 
 class _CNN(torch.autograd.Function):
@@ -186,7 +198,10 @@ class CorrectionCNN(nn.Module):
         if self.convs[0].backend == "hip":
             params = [t for conv in self.convs for t in (conv.weight, conv.bias)]
             _lib.require_gpu(u, *params)
-            s = _CNN.apply(u, *params)
+            if torch.is_grad_enabled() and (u.requires_grad or any(p.requires_grad for p in params)):
+                s = _CNN.apply(u, *params)
+            else:
+                s = _cnn_eval(u, params[0::2], params[1::2])
         else:
             x = u.unsqueeze(1)
             for i, conv in enumerate(self.convs):

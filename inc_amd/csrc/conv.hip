@@ -43,6 +43,12 @@ __device__ __forceinline__ T gelu_and_grad(T v, T& g) {
   return h + v * T(0.39894228040143267794) * exp(T(-0.5) * v * v);
 }
 
+// gelu(v) alone (no gradient wanted): the same value as gelu_and_grad's g.
+template <typename T>
+__device__ __forceinline__ T gelu_value(T v) {
+  return v * (T(0.5) * (T(1) + erf(v * T(0.70710678118654752440))));
+}
+
 // One block: TI = 64 * kPPT positions of one row, kWaves * CPT output
 // channels.  The input tile (with its periodic halo) is staged in LDS and
 // shared by the block's wavefronts; every wavefront owns CPT output channels
@@ -54,7 +60,7 @@ __device__ __forceinline__ T gelu_and_grad(T v, T& g) {
 // (the input gradient of a layer whose input was an activation: mul holds
 // the activation's derivative); with act = GELU, y = gelu(v) and
 // dy = gelu'(v) (the next layer's input and the derivative its input
-// gradient needs).
+// gradient needs), or y alone when dy is NULL (no gradient wanted).
 template <typename T, int K, int CPT, bool FLIP>
 __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int N, const T* __restrict__ x,
                                                       const T* __restrict__ w, const T* __restrict__ bias,
@@ -112,9 +118,13 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int N, co
         T v = acc[a][p] + bo;
         if (mul) v *= mul[row + i];
         if (act == INC_ACT_GELU) {
-          T gv;
-          dy[row + i] = gelu_and_grad(v, gv);
-          v = gv;
+          if (dy) {
+            T gv;
+            dy[row + i] = gelu_and_grad(v, gv);
+            v = gv;
+          } else {
+            v = gelu_value(v);
+          }
         }
         y[row + i] = v;
       }
@@ -621,9 +631,13 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int C
         const size_t at = base + size_t(RPP * u) * N;
         float v = (os[o * OS + jj] + (bias ? bias[o] : 0.f)) * mv[u];
         if (act == INC_ACT_GELU) {
-          float gv;
-          dy[at] = gelu_and_grad(v, gv);
-          v = gv;
+          if (dy) {
+            float gv;
+            dy[at] = gelu_and_grad(v, gv);
+            v = gv;
+          } else {
+            v = gelu_value(v);
+          }
         }
         y[at] = v;
       }
@@ -811,7 +825,6 @@ int inc_conv1d_fwd_act(int dtype, int act, int B, int Ci, int Co, int N, int K, 
   inc::clear_error();
   int rc = inc::conv_check(dtype, act, B, Ci, Co, N, K);
   if (rc || B == 0) return rc;
-  INC_REQUIRE(act == INC_ACT_NONE || dy != nullptr, INC_EINVAL, "conv1d with an activation needs dy");
   hipStream_t st = static_cast<hipStream_t>(stream);
   return dtype == INC_F32
              ? inc::conv_fwd_launch<float, false>(B, Ci, Co, N, K, x, w, bias, y, nullptr, act, dy, st)

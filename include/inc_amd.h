@@ -347,7 +347,8 @@ int inc_conv1d_bwd_weight(int dtype, int B, int Ci, int Co, int n, int K, const 
  * next layer (and its weight gradient); dy is what the input gradient of that
  * next layer multiplies by (inc_conv1d_bwd_data_mul).  act: INC_ACT_NONE (dy
  * unused, the plain inc_conv1d_fwd) or INC_ACT_GELU (exact erf form, as
- * torch.nn.GELU()).
+ * torch.nn.GELU()).  dy may be NULL when no gradient is wanted: y alone is
+ * written, without the derivative's exp and store.
  */
 enum { INC_ACT_NONE = 0, INC_ACT_GELU = 1 };
 int inc_conv1d_fwd_act(int dtype, int act, int B, int Ci, int Co, int n, int K, const void* x,

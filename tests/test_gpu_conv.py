@@ -83,6 +83,9 @@ def test_conv_gelu_epilogue_and_scaled_input_gradient(cuda, shape, dtype):
     assert rel_l2(y, h.detach()) < tol
     assert rel_l2(dy, dv) < tol
     assert rel_l2(gx, gxr * m) < tol
+    # without the derivative (no gradient wanted): the same y, bit for bit
+    y2, dy2 = _conv_fwd(c(x), c(w), c(b), act=INC_ACT_GELU, need_dy=False)
+    assert dy2 is None and torch.equal(y2, y)
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-11), (torch.float32, 1e-5)])
@@ -92,8 +95,11 @@ def test_cnn_backends_agree(cuda, dtype, tol):
     ref = CorrectionCNN(4, 32, 5, backend="torch").cuda().to(dtype)
     ref.load_state_dict(hip.state_dict())
     u = torch.randn(6, 128, dtype=torch.float64).to(cuda, dtype)
+    with torch.no_grad():
+        a0 = hip(u)  # the evaluation path: no derivative computed or stored
     a, r = hip(u), ref(u)
     assert rel_l2(a, r) < tol
+    assert torch.equal(a0, a.detach())
     a.sum().backward()
     r.sum().backward()
     for p, q in zip(hip.parameters(), ref.parameters()):
