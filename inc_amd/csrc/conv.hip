@@ -286,20 +286,26 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int B, int Ci, int
   constexpr int GS = kWgTP + 1;             // odd row strides: the 16 rows of a fragment
   const int XS = (kWgTP + K - 1) | 1;       // read hit distinct banks
   float* gs = reinterpret_cast<float*>(smem_raw);  // [MT*16][GS]
-  float* xs = gs + MT * 16 * GS;                   // [Ci][XS]
+  float* xs = gs + MT * 16 * GS;                   // [Ci][XS], then a row of ones and one of zeros
   const int tid = threadIdx.x, lane = tid & 63;
+  // the bias column reads the ones row and padding columns the zeros row, so
+  // every B-fragment read is an unconditional LDS read (a conditional read
+  // makes the MFMAs wait for every read in flight); written once, ordered by
+  // the first job's barriers
+  for (int e = tid; e < 2 * kWgTP; e += 256) xs[Ci * XS + e] = e < kWgTP ? 1.f : 0.f;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int tphase = wave % WT, pphase = wave / WT;
   const bool active = pphase < WP;
   const int ncw = Ci * K, ntile = (ncw + 1 + 15) / 16;
   const int lr = lane & 15, lk = lane >> 4;
-  // this lane's B-fragment source per column tile: offset into xs, or the
-  // ones column (-1), or padding (-2)
+  // this lane's B-fragment source per column tile: offset into xs of its
+  // input row and tap, of the ones row (the bias column) or of the zeros row
+  // (padding)
   int off[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int tile = tphase + t * WT, col = tile * 16 + lr;
-    off[t] = tile >= ntile || col > ncw ? -2 : (col == ncw ? -1 : (col / K) * XS + col % K);
+    off[t] = tile >= ntile || col > ncw ? Ci * XS + kWgTP : (col == ncw ? Ci * XS : (col / K) * XS + col % K);
   }
   v4f acc[MT][NT];
 #pragma unroll
@@ -355,11 +361,11 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int B, int Ci, int
 #pragma unroll
         for (int m = 0; m < MT; ++m) fa[m] = gs[(m * 16 + lr) * GS + p];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) fb[t] = off[t] >= 0 ? xs[off[t] + p] : (off[t] == -1 ? 1.f : 0.f);
+        for (int t = 0; t < NT; ++t) fb[t] = xs[off[t] + p];
       };
       load(pphase, a, bv);
       for (int s = pphase; s < kWgTP / 4; s += WP) {
-        if (s + WP < kWgTP / 4) load(s + WP, an, bn);
+        load(min(s + WP, kWgTP / 4 - 1), an, bn);  // unconditional: the last one is unused
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -378,8 +384,8 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int B, int Ci, int
   for (int m = 0; m < MT; ++m)
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      const int col = (tphase + t * WT) * 16 + lr;
-      if (off[t] == -2) continue;
+      const int tile = tphase + t * WT, col = tile * 16 + lr;
+      if (tile >= ntile || col > ncw) continue;  // padding
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int o = m * 16 + 4 * lk + r;
@@ -399,7 +405,8 @@ int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const f
   const int WT = ntile >= 4 ? 4 : (ntile >= 2 ? 2 : 1);
   const int ntw = (ntile + WT - 1) / WT;
   const int NT = ntw <= 4 ? ntw : (ntw <= 6 ? 6 : 8);
-  const size_t lds = (size_t(MT) * 16 * (kWgTP + 1) + size_t(Ci) * ((kWgTP + K - 1) | 1)) * sizeof(float);
+  const size_t lds =
+      (size_t(MT) * 16 * (kWgTP + 1) + size_t(Ci) * ((kWgTP + K - 1) | 1) + 2 * kWgTP) * sizeof(float);
   if (MT > 4 || ntw > 8 || lds > 150 * 1024) return INC_EUNSUPPORTED;
   const int width = Co * ncw + Co;
   const int64_t slots = std::min<int64_t>(512, work_elems / width);  // two blocks per CU
