@@ -471,8 +471,6 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int C
   const int R = Ci * K, RS = R + 4, CS = Ci + 4, XW = TI + K - 1;
   float* ws = reinterpret_cast<float*>(smem_raw);  // [CT*16][RS]
   float* xs = ws + CT * 16 * RS;                   // [XW + 1][CS] (row XW spare)
-  constexpr int OS = TI + 4;                       // output tile row stride
-  float* os = xs;                                  // [CT*16][OS], after the MFMAs
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, q = lane >> 4;
@@ -605,50 +603,43 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int C
       mma(a1, b1);
       __builtin_amdgcn_sched_barrier(0);
     }
-    // Epilogue through LDS: the accumulators (lane (lr, q) holds D[4q + r]
-    // [16t + lr] of tile (m, t)) go to os[o][j], then each wavefront writes
-    // whole row segments (64 or 128 positions of one channel) with the bias,
-    // mul and GELU applied; mul is read the same coalesced way, all of a
-    // thread's values in flight at once.
-    __syncthreads();  // every wavefront is done with xs, which os reuses
+    // Epilogue straight from the accumulators: lane (lr, q) of tile (m, t)
+    // holds D[16m + 4q + r][16 (wave PW + t) + lr], so a store writes 16
+    // consecutive positions of four channels; mul is read the same way, all
+    // of a lane's values in flight at once (clamped addresses, unused past
+    // the edges).  Staging the tile through LDS for whole-row stores cost
+    // two barriers per job and measured 4-7% slower.
+    float mv[CT][PW][4];
 #pragma unroll
     for (int m = 0; m < CT; ++m)
 #pragma unroll
       for (int t = 0; t < PW; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) os[(m * 16 + 4 * q + r) * OS + p0 + 16 * t + lr] = acc[m][t][r];
-    __syncthreads();
-    constexpr int RPP = 256 / TI, NR = CT * 16 / RPP;  // rows per pass, rows per thread
-    const int jj = tid % TI, ob = tid / TI;           // ob is wave-uniform
-    const int i = i0 + jj;
-    if (i < N) {
-      const size_t base = (size_t(b) * Co + ob) * N + i;
-      float mv[NR];
-#pragma unroll
-      for (int u = 0; u < NR; ++u) mv[u] = 1.f;
-      if (mul) {
-#pragma unroll
-        for (int u = 0; u < NR; ++u)
-          if (ob + RPP * u < Co) mv[u] = mul[base + size_t(RPP * u) * N];
-      }
-#pragma unroll
-      for (int u = 0; u < NR; ++u) {
-        const int o = ob + RPP * u;
-        if (o >= Co) continue;
-        const size_t at = base + size_t(RPP * u) * N;
-        float v = (os[o * OS + jj] + (bias ? bias[o] : 0.f)) * mv[u];
-        if (act == INC_ACT_GELU) {
-          if (dy) {
-            float gv;
-            dy[at] = gelu_and_grad(v, gv);
-            v = gv;
-          } else {
-            v = gelu_value(v);
-          }
+        for (int r = 0; r < 4; ++r) {
+          const int o = min(m * 16 + 4 * q + r, Co - 1), i = min(i0 + p0 + 16 * t + lr, N - 1);
+          mv[m][t][r] = mul ? mul[(size_t(b) * Co + o) * N + i] : 1.f;
         }
-        y[at] = v;
-      }
-    }
+#pragma unroll
+    for (int m = 0; m < CT; ++m)
+#pragma unroll
+      for (int t = 0; t < PW; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int o = m * 16 + 4 * q + r, i = i0 + p0 + 16 * t + lr;
+          if (o >= Co || i >= N) continue;
+          const size_t at = (size_t(b) * Co + o) * N + i;
+          float v = (acc[m][t][r] + (bias ? bias[o] : 0.f)) * mv[m][t][r];
+          if (act == INC_ACT_GELU) {
+            if (dy) {
+              float gv;
+              dy[at] = gelu_and_grad(v, gv);
+              v = gv;
+            } else {
+              v = gelu_value(v);
+            }
+          }
+          y[at] = v;
+        }
   }
 }
 
@@ -673,9 +664,8 @@ int conv_fwd_mfma(int B, int Ci, int Co, int N, int K, bool flip, const float* x
                   const float* bias, float* y, const float* mul, int act, float* dy, hipStream_t st) {
   if (Ci % 16 != 0 || Ci > 64 || Co < 16 || Co > 64) return INC_EUNSUPPORTED;
   const int CT = (Co + 15) / 16;
-  auto lds_of = [&](int pw) {
-    const size_t tile = std::max(size_t(64 * pw + K) * (Ci + 4), size_t(CT) * 16 * (64 * pw + 4));
-    return (size_t(CT) * 16 * (Ci * K + 4) + tile) * sizeof(float);
+  auto lds_of = [&](int pw) {  // staged weights and the input tile (with its spare row)
+    return (size_t(CT) * 16 * (Ci * K + 4) + size_t(64 * pw + K) * (Ci + 4)) * sizeof(float);
   };
   const int PW = (N > 64 && lds_of(2) <= 80 * 1024) ? 2 : 1;
   const size_t lds = lds_of(PW);
