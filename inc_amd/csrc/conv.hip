@@ -507,38 +507,42 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int C
   // 16-byte LDS write.  The next job's tile is fetched before this job's
   // MFMAs and staged after them, so its HBM latency overlaps the matrix work.
   constexpr int U = 4 * PW + 1;  // items <= 16 * (64 PW + 8) <= 256 U (Ci <= 64, K <= 9)
+  // A thread's items are the same in every job: quad and row of item u,
+  // computed once (the global channel offset clamped to the last quad for
+  // items past the tile, which are loaded and not staged)
+  int jr[U], cq[U];
+  {
+    int c4 = tid / XW, j = tid - c4 * XW;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      jr[u] = j;
+      cq[u] = c4;
+      j += 256;
+      while (j >= XW) {
+        j -= XW;
+        ++c4;
+      }
+    }
+  }
   auto fetch = [&](int job, v4f* v) __attribute__((always_inline)) {
     const int b = job / tiles, i0 = (job - b * tiles) * TI;
     const float* xb = x + size_t(b) * Ci * N;
-    int c4 = tid / XW, j = tid - c4 * XW;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       // unconditional loads (items past the tile read quad c4n - 1 and are
       // not staged): a conditional load's value is copied at the branch
       // join, and that copy would wait for it
-      int pos = i0 - P + j;  // in [-P, N + TI + K): one wrap unless N is small
+      int pos = i0 - P + jr[u];  // in [-P, N + TI + K): one wrap unless N is small
       pos = pos < 0 ? pos + N : (pos >= N ? pos - N : pos);
       if (pos >= N) pos %= N;
-      const float* s = xb + size_t(4 * min(c4, c4n - 1)) * N + pos;
+      const float* s = xb + size_t(4 * min(cq[u], c4n - 1)) * N + pos;
       v[u] = v4f{s[0], s[N], s[2 * size_t(N)], s[3 * size_t(N)]};
-      j += 256;
-      while (j >= XW) {
-        j -= XW;
-        ++c4;
-      }
     }
   };
   auto stage = [&](const v4f* v) __attribute__((always_inline)) {
-    int c4 = tid / XW, j = tid - c4 * XW;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (tid + 256 * u < items) *reinterpret_cast<v4f*>(xs + j * CS + 4 * c4) = v[u];
-      j += 256;
-      while (j >= XW) {
-        j -= XW;
-        ++c4;
-      }
-    }
+    for (int u = 0; u < U; ++u)
+      if (tid + 256 * u < items) *reinterpret_cast<v4f*>(xs + jr[u] * CS + 4 * cq[u]) = v[u];
   };
   v4f nx[U];
   if (int(blockIdx.x) < njobs) fetch(blockIdx.x, nx);
