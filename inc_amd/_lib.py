@@ -161,6 +161,18 @@ def require_gpu(*tensors):
                 f"{t.device}. Move the state to 'cuda'.")
 
 
+def require_like(ref, *tensors, what="parameter"):
+    """Every tensor on the GPU with ref's dtype.  The kernels read weights
+    and biases through raw pointers of the state's element type, so a
+    float32 weight under a float64 state, or a host tensor, must be refused
+    before any launch (torch.nn.Conv1d raises on such a mix as well)."""
+    require_gpu(ref, *tensors)
+    for t in tensors:
+        if t is not None and t.dtype != ref.dtype:
+            raise TypeError(f"{what} has dtype {t.dtype} but the input is {ref.dtype}; "
+                            "convert the model or the state so that they match")
+
+
 def ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 

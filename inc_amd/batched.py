@@ -162,11 +162,16 @@ class RaggedBurgers:
     def advance(self, u, s=None, steps=1, step0=0, want_traj=False):
         """``steps`` steps from rollout step ``step0`` with a fixed correction."""
         _lib.require_gpu(u, s)
+        if u.numel() != self.total:
+            raise ValueError(f"packed state has {u.numel()} values, batch has {self.total}")
+        if s is not None and s.numel() != self.total:
+            raise ValueError(f"packed correction has {s.numel()} values, batch has {self.total}")
+        for name, t in (("state", u), ("correction", s)):
+            if t is not None and t.dtype != self.dtype:
+                raise TypeError(f"packed {name} is {t.dtype}, the ragged batch was built for {self.dtype}")
         if self._team_auto:
             self._team_auto = False
             self.team = autotune.tune_ragged(self)
-        if u.numel() != self.total:
-            raise ValueError(f"packed state has {u.numel()} values, batch has {self.total}")
         need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
         return _RaggedAdvance.apply(u, s, self, steps, step0, want_traj, need)
 

@@ -1,7 +1,8 @@
 """Constructor-parameter objects of the 1D solvers.
 
 ``KSSimParams`` keeps the reference's constructor (config/config_1d.py:16-41)
-and adds ``dealias`` (2/3-rule mask of the ETDRK4 scheme).  ``BGSimParams``
+and adds ``dealias`` (2/3-rule mask of the ETDRK4 scheme) and ``pack_pairs``
+(two trajectories per complex FFT).  ``BGSimParams``
 is imported by the reference trainer (trainer/trainer_1d.py:344-355) but is
 missing from its config module; this one carries every attribute
 BurgersSolverTorch reads (solver_1d.py:46-66, 76-84, 106-111), with the
@@ -16,7 +17,7 @@ class KSSimParams:
     """Parameters of KSSolverTorch (config/config_1d.py:16-41)."""
 
     def __init__(self, domain_size, resolution=64, dt=1e-2, random_ratio=1, batch_size=None,
-                 device="cuda", time_scheme="ETRK2", total_time=50, dealias=True):
+                 device="cuda", time_scheme="ETRK2", total_time=50, dealias=True, pack_pairs=True):
         self.dtype = torch.float64
         self.resolution = resolution
         self.dt = dt
@@ -26,6 +27,10 @@ class KSSimParams:
         self.total_time = total_time
         self.warm_up = 2000
         self.dealias = dealias
+        # two real trajectories share one complex FFT (default); False runs one
+        # trajectory per FFT, which keeps a trajectory finite when its FFT
+        # partner holds a NaN or Inf, as the reference's per-row torch.fft does
+        self.pack_pairs = pack_pairs
         self.set_domain_size(domain_size)
         self.batch_size = len(self.domain_size) if batch_size is None else batch_size
         self.init_alpha()

@@ -81,7 +81,7 @@ class _CircConv1d(torch.autograd.Function):
 def circular_conv1d(x, weight, bias=None):
     """Periodic 'same' convolution of x [B, Ci, N] with weight [Co, Ci, K]
     (torch.nn.Conv1d(padding=K//2, padding_mode='circular') semantics)."""
-    _lib.require_gpu(x, weight, bias)
+    _lib.require_like(x, weight, bias, what="circular_conv1d weight/bias")
     return _CircConv1d.apply(x, weight, bias)
 
 
@@ -177,18 +177,22 @@ class CorrectionCNN(nn.Module):
         """(inc_cnn_desc, tensors to keep alive) for evaluating this network
         inside a rollout kernel (inc_burgers_inc_fwd), or None when its shape
         is not one the kernel runs (width 32 or 48, kernel size 5 or 7, 2..8
-        layers, the HIP backend)."""
+        layers, the HIP backend, every parameter on the GPU in ``dtype``: the
+        kernels read the weights through pointers of the state's element
+        type, and a mismatched model takes the per-step path, which raises)."""
         convs = list(self.convs)
         c = convs[0].weight.shape[0]
         k = convs[0].k
         if (convs[0].backend != "hip" or not 2 <= len(convs) <= 8 or c not in (32, 48) or k not in (5, 7)
                 or any(cv.k != k for cv in convs)):
             return None
+        if any(p.dtype != dtype or not p.is_cuda for p in self.parameters()):
+            return None
         d = _lib.CnnDesc(layers=len(convs), channels=c, kernel=k, scale=self.scale)
         keep = []
         for i, cv in enumerate(convs):
-            w = cv.weight.detach().to(dtype).contiguous()
-            b = cv.bias.detach().to(dtype).contiguous()
+            w = cv.weight.detach().contiguous()
+            b = cv.bias.detach().contiguous()
             keep += [w, b]
             d.weight[i] = w.data_ptr()
             d.bias[i] = b.data_ptr()
@@ -197,7 +201,7 @@ class CorrectionCNN(nn.Module):
     def forward(self, u, L=None):
         if self.convs[0].backend == "hip":
             params = [t for conv in self.convs for t in (conv.weight, conv.bias)]
-            _lib.require_gpu(u, *params)
+            _lib.require_like(u, *params, what="CorrectionCNN parameter")
             if torch.is_grad_enabled() and (u.requires_grad or any(p.requires_grad for p in params)):
                 s = _CNN.apply(u, *params)
             else:

@@ -63,6 +63,46 @@ _SOURCES = {"homogeneous": L.INC_SRC_NONE, "viscosity": L.INC_SRC_NONE,
             "linear": L.INC_SRC_LINEAR, "learning": L.INC_SRC_LEARNING}
 
 
+def _burgers_correction(s, u):
+    """correction_s as the [B, N] array the kernels read.  The reference adds
+    it by broadcasting (source + correction_s, solver_1d.py:116-123), so a
+    scalar, [N] or [B, 1] correction is expanded to the state's shape here;
+    autograd sums its gradient back to the given shape."""
+    if s is None:
+        return None
+    if not isinstance(s, torch.Tensor):
+        s = torch.as_tensor(s, device=u.device)
+    L.require_gpu(s)
+    s = s.to(dtype=u.dtype)
+    if s.shape != u.shape:
+        try:
+            s = s.expand(u.shape)
+        except RuntimeError:
+            raise ValueError(f"correction_s of shape {tuple(s.shape)} does not broadcast to the state's "
+                             f"{tuple(u.shape)}") from None
+    return s
+
+
+def _ks_correction(s, u):
+    """correction_s as the [B, N] array the kernels read.  The reference
+    transforms it along its last axis and adds the spectrum by broadcasting
+    (solver_1d.py:249-253), so [N] (one correction for every trajectory) and
+    [B, N] are the shapes with that meaning; [N] is expanded here."""
+    if s is None:
+        return None
+    L.require_gpu(s)
+    s = s.to(dtype=u.dtype)
+    if s.shape != u.shape:
+        if s.dim() == 0 or s.shape[-1] != u.shape[-1]:
+            raise ValueError(f"correction_s must be [N] or [batch, N] = {tuple(u.shape)} (got {tuple(s.shape)})")
+        try:
+            s = s.expand(u.shape)
+        except RuntimeError:
+            raise ValueError(f"correction_s of shape {tuple(s.shape)} does not broadcast to the state's "
+                             f"{tuple(u.shape)}") from None
+    return s
+
+
 class _BurgersAdvance(torch.autograd.Function):
     """`steps` steps of the Burgers map; returns the final state, or the
     [B, steps, N] trajectory when ``want_traj``."""
@@ -368,31 +408,37 @@ class BurgersSolverTorch:
         """float -> (value, None); tensor with 1 or b entries -> (0, [b] tensor)."""
         if not isinstance(v, torch.Tensor):
             return float(v), None
-        key = (name, id(v), v._version, dtype, b)
+        # the cache holds v itself, so the identity test below cannot be
+        # fooled by a new tensor at a recycled address or id; in-place edits
+        # bump _version
+        key = (v._version, dtype, b)
         hit = self._cache.get(name)
-        if hit is not None and hit[0] == key:
-            return 0.0, hit[1]
+        if hit is not None and hit[0] is v and hit[1] == key:
+            return 0.0, hit[2]
         flat = v.detach().reshape(-1).to(device=device, dtype=dtype)
         if flat.numel() == 1:
             flat = flat.expand(b)
         if flat.numel() != b:
             raise ValueError(f"{name} must be a scalar or have one value per trajectory ([batch_size,1])")
         flat = flat.contiguous()
-        self._cache[name] = (key, flat)
+        self._cache[name] = (v, key, flat)
         return 0.0, flat
 
     def _forcing(self, b, dtype, device):
         p = self.param
-        key = ("forcing", id(p.A), id(p.omega), id(p.phi), id(p.l), dtype, b)
+        srcs = (p.A, p.omega, p.phi, p.l)
+        # held by the cache entry (identity, not id(): no recycled ids) with
+        # their versions (in-place edits of the forcing are seen)
+        key = (tuple(getattr(v, "_version", None) for v in srcs), dtype, b)
         hit = self._cache.get("forcing")
-        if hit is not None and hit[0] == key:
-            return hit[1]
+        if hit is not None and all(a is c for a, c in zip(hit[0], srcs)) and hit[1] == key:
+            return hit[2]
         J = p.J if not callable(getattr(p, "J", None)) else p.J()
         arrs = []
         for v in (p.A, p.omega, p.phi, p.l):
             a = torch.as_tensor(v, device=device).to(dtype).reshape(b, -1)[:, :J].contiguous()
             arrs.append(a)
-        self._cache["forcing"] = (key, (J, arrs))
+        self._cache["forcing"] = (srcs, key, (J, arrs))
         return J, arrs
 
     def _desc(self, u, dt, t, x=None):
@@ -494,14 +540,12 @@ class BurgersSolverTorch:
         return _BurgersRHS.apply(u, self, L.INC_PART_DIFFUSION, self.t)
 
     def _advance(self, u, dt, steps=1, want_traj=False):
-        L.require_gpu(u, self.correction_s)
+        L.require_gpu(u)
         if self._team_auto:
             self._team_auto = False
             self.team = autotune.tune_burgers(u.shape[-1], u.shape[0], u.dtype, self.time_scheme,
                                               self.param.convection_term)
-        s = self.correction_s
-        if s is not None:
-            s = s.to(dtype=u.dtype)
+        s = _burgers_correction(self.correction_s, u)
         need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
         return _BurgersAdvance.apply(u, s, self, dt, steps, self.t, want_traj, need)
 
@@ -532,10 +576,8 @@ class BurgersSolverTorch:
         """Per-trajectory adaptive sub-stepping on the GPU (adaptive_mode
         'trajectory'): no host synchronisation; sub-step counts land in
         ``self.last_substeps``."""
-        L.require_gpu(u, self.correction_s)
-        s = self.correction_s
-        if s is not None:
-            s = s.to(dtype=u.dtype)
+        L.require_gpu(u)
+        s = _burgers_correction(self.correction_s, u)
         need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
         return _BurgersAdaptive.apply(u, s, self, steps, self.t, want_traj, need)
 
@@ -577,7 +619,7 @@ class BurgersSolverTorch:
         the network's parameters require it) the rollout is differentiable:
         the adjoint kernel runs the network's transpose
         (inc_burgers_inc_train_fwd / _adj, shapes per inc_train_supported)."""
-        L.require_gpu(u)
+        L.require_like(u, *model.parameters(), what="correction network parameter")
         if self.adaptive_CFL is not None:
             raise ValueError("rollout_inc runs a fixed time step; adaptive_CFL is set")
         cnn = model.fused_desc(u.dtype)
@@ -616,15 +658,13 @@ class BurgersSolverTorch:
         tensor is formed.  Fixed time step only."""
         if self.adaptive_CFL is not None:
             raise ValueError("rollout_mse runs a fixed time step; adaptive_CFL is set")
-        L.require_gpu(u, self.correction_s, target)
+        L.require_gpu(u, target)
         if self._team_auto:
             self._team_auto = False
             self.team = autotune.tune_burgers(u.shape[-1], u.shape[0], u.dtype, self.time_scheme,
                                               self.param.convection_term)
         steps = target.shape[1]
-        s = self.correction_s
-        if s is not None:
-            s = s.to(dtype=u.dtype)
+        s = _burgers_correction(self.correction_s, u)
         need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
         loss, traj = _BurgersRolloutMSE.apply(u, s, target, self, self.dt, steps, self.t, need)
         for _ in range(steps):
@@ -740,6 +780,9 @@ class KSSolverTorch:
         self.correction_s = None
         self.correction_u = None
         self.dealias = bool(getattr(sim_params, "dealias", True))
+        # two real trajectories per complex FFT (default) or one per FFT; the
+        # unpacked form keeps a trajectory finite when its FFT partner is not
+        self.pack_pairs = bool(getattr(sim_params, "pack_pairs", True))
         radix = getattr(sim_params, "radix", 0) or 0
         self._radix_auto = radix == "auto" or (radix == 0 and autotune.enabled())
         self.radix = 0 if radix == "auto" else int(radix)
@@ -774,12 +817,25 @@ class KSSolverTorch:
 
     # -- tables and descriptor -----------------------------------------------------
     def _tables(self, domain_size, b, dtype, dev, scheme, dt):
-        ds = domain_size if isinstance(domain_size, torch.Tensor) else torch.as_tensor(domain_size)
-        key = (ds.data_ptr() if ds.numel() else 0, ds._version, tuple(ds.shape), str(ds.device), dtype, scheme,
-               float(dt), self.dealias, b)
-        hit = self._tab.get(key)
-        if hit is not None:
-            return hit
+        conf = (dtype, scheme, float(dt), self.dealias, self.pack_pairs, b)
+        if not isinstance(domain_size, torch.Tensor) or not domain_size.is_cuda:
+            # host values: keyed by value (no device synchronisation involved)
+            ds = torch.as_tensor(domain_size, dtype=torch.float64)
+            key = ("values", tuple(ds.reshape(-1).tolist())) + conf
+            hit = self._tab.get(key)
+            if hit is not None:
+                return hit[1]
+        else:
+            # a device tensor: keyed by storage, shape and version.  The
+            # entry holds the tensor, so the caching allocator cannot hand
+            # its block to a new tensor while the entry lives, and an
+            # in-place edit bumps _version (shared by every view).
+            ds = domain_size
+            key = ("device", ds.data_ptr(), ds.storage_offset(), tuple(ds.shape), tuple(ds.stride()), ds.dtype,
+                   ds._version) + conf
+            hit = self._tab.get(key)
+            if hit is not None and hit[0] is not None and hit[0].data_ptr() == ds.data_ptr():
+                return hit[1]
         Lf = ds.detach().reshape(-1).to(device=dev, dtype=torch.float64)
         if Lf.numel() not in (1, b):
             raise ValueError("domain_size must have one value or one per trajectory")
@@ -792,14 +848,14 @@ class KSSolverTorch:
         rc = L.load().inc_ks_tables(L.dtype_code(dtype), scheme, n, float(dt), int(self.dealias), L.ptr(uniq),
                                     count, L.ptr(table), L.stream())
         L.check(rc, "KSSolverTorch coefficient tables")
-        if count == 1:
+        if count == 1 and self.pack_pairs:
             idx = None
         else:
             idx = inv.to(torch.int32).expand(b).contiguous() if inv.numel() == 1 else inv.to(torch.int32).contiguous()
         entry = (table, idx, uniq)
         if len(self._tab) > 8:
             self._tab.clear()
-        self._tab[key] = entry
+        self._tab[key] = (ds if ds.is_cuda else None, entry)
         return entry
 
     def _desc(self, u, scheme, substeps, domain_size):
@@ -821,13 +877,11 @@ class KSSolverTorch:
         return d, [table, idx, uniq]
 
     def _advance(self, u, domain_size, scheme, substeps, steps=1, want_traj=False):
-        L.require_gpu(u, self.correction_s)
+        L.require_gpu(u)
         if self._radix_auto:
             self._radix_auto = False
             self.radix = autotune.tune_ks(u.shape[-1], u.shape[0], u.dtype, self.time_scheme, dt=self.dt)
-        s = self.correction_s
-        if s is not None:
-            s = s.to(dtype=u.dtype)
+        s = _ks_correction(self.correction_s, u)
         if domain_size is None:
             domain_size = self.domain_size
         need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
@@ -872,7 +926,7 @@ class KSSolverTorch:
         trajectory) as BurgersSolverTorch.rollout_mse does."""
         if self.correction_u is not None:
             raise ValueError("fused rollout does not apply correction_u; use step()")
-        L.require_gpu(u, self.correction_s, target)
+        L.require_gpu(u, target)
         scheme = _KS_SCHEMES[self.time_scheme]
         substeps = 2 if self.doublestep else 1
         if domain_size is None:
@@ -880,8 +934,6 @@ class KSSolverTorch:
         if self._radix_auto:
             self._radix_auto = False
             self.radix = autotune.tune_ks(u.shape[-1], u.shape[0], u.dtype, self.time_scheme, dt=self.dt)
-        s = self.correction_s
-        if s is not None:
-            s = s.to(dtype=u.dtype)
+        s = _ks_correction(self.correction_s, u)
         need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
         return _KSRolloutMSE.apply(u, s, target, self, scheme, substeps, target.shape[1], domain_size, need)

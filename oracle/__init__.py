@@ -22,19 +22,27 @@ Pinning (what checks each function against something other than itself):
   vectors (hand-composed Euler step from reference pieces, forcing), mean
   conservation, Laplacian eigenvalues.
 * SSP-RK3 (no reference counterpart): amplification factor
-  1+z+z^2/2+z^3/6 on a single diffusive mode; constant-state fixed point.
+  1+z+z^2/2+z^3/6 on a single diffusive mode; constant-state fixed point;
+  the stage times t, t+dt, t+dt/2 by Simpson's rule on a spatially constant
+  state under spatially constant forcing; third order in dt with
+  time-dependent forcing under transport (a wrong stage time gives order 1).
 * KS ETD1 / ETRK2 / double step (solver_1d.py:198-288): golden vectors
   (including correction_s and per-trajectory L), single-mode exponential decay,
   mean conservation, constant forcing integrates to +s*dt.
 * ETDRK4 + 2/3 dealias (no reference counterpart): contour-integral
   coefficients checked against the direct formulas (large |hL|) and Taylor
   series (small |hL|); f1+4f2+f3 = (e^{hL}-1)/L identity; fourth-order
-  convergence in dt on a smooth solution; constant forcing exact.
+  convergence in dt on a smooth solution; constant forcing exact.  The 2/3
+  mask: a single mode n/6 < k < n/3 evolves by the exact linear map with the
+  mask (its product is cut) and not without it; two modes whose products
+  reach floor(n/3) and floor(n/3)+1 show the cut between them (n = 256, 512).
 * Adjoints: autograd through this oracle versus FP64 central differences and
   versus reference autograd gradients in tests/golden.
-* ``initial_conditions`` (solver_1d.py:214-223, RNG-based): parity unpinned
-  beyond its deterministic part (the random perturbation comes from a
-  different generator on the GPU).
+* ``initial_conditions`` (solver_1d.py:214-223): the deterministic part
+  cos(2 pi x) + alpha sin(2 pi x) on the endpoint-inclusive grid, and the
+  noise's unit Frobenius norm over the whole batch.  The random draws
+  themselves come from a different generator on the GPU and are not
+  compared value by value.
 """
 from .burgers import (BurgersOracle, weno_reconstruction, burgers_params,
                       WENO_TABLE, WENO_EPS, WENO_GAMMA)

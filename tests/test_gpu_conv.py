@@ -104,3 +104,38 @@ def test_cnn_backends_agree(cuda, dtype, tol):
     r.sum().backward()
     for p, q in zip(hip.parameters(), ref.parameters()):
         assert rel_l2(p.grad, q.grad) < 10 * tol
+
+
+# The KS INC and ragged INC workloads' layers (B = 256, N = 512, 48 -> 48 k7
+# and the thin 1 -> 48, 48 -> 1 layers): enough (row, position-tile) jobs that
+# every block of the matrix-core forward / input-gradient kernels and of the
+# weight-gradient kernel loops over several jobs, with the next job's tile
+# prefetched and the staging indices computed once per block.
+@pytest.mark.parametrize("shape", [(256, 48, 48, 512, 7), (256, 1, 48, 512, 7), (256, 48, 1, 512, 7)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_conv_multi_job_blocks_fp32(cuda, shape):
+    bsz, ci, co, n, k = shape
+    gen = torch.Generator().manual_seed(sum(shape) + 2)
+    x = torch.randn(bsz, ci, n, generator=gen, dtype=torch.float64)
+    w = torch.randn(co, ci, k, generator=gen, dtype=torch.float64) / (ci * k) ** 0.5
+    b = torch.randn(co, generator=gen, dtype=torch.float64)
+    g = torch.randn(bsz, co, n, generator=gen, dtype=torch.float64)
+    xr, wr, br = (t.clone().requires_grad_(True) for t in (x, w, b))
+    yr = ref_conv(xr, wr, br)
+    (yr * g).sum().backward()
+    xg, wg, bg = (t.to(cuda, torch.float32).requires_grad_(True) for t in (x, w, b))
+    y = circular_conv1d(xg, wg, bg)
+    (y * g.to(cuda, torch.float32)).sum().backward()
+    assert rel_l2(y, yr.detach()) < 1e-5
+    assert rel_l2(xg.grad, xr.grad) < 1e-5
+    assert rel_l2(wg.grad, wr.grad) < 1e-5
+    assert rel_l2(bg.grad, br.grad) < 1e-5
+    # the fused GELU epilogue on the same multi-job launch
+    if ci > 1 or co > 1:
+        v = ref_conv(x, w, b)
+        y2, dy2 = _conv_fwd(xg.detach().contiguous(), wg.detach().contiguous(), bg.detach().contiguous(),
+                            act=INC_ACT_GELU)
+        vr = v.clone().requires_grad_(True)
+        (dvr,) = torch.autograd.grad(F.gelu(vr).sum(), vr)
+        assert rel_l2(y2, F.gelu(v)) < 1e-5
+        assert rel_l2(dy2, dvr) < 1e-5

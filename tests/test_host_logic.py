@@ -70,3 +70,27 @@ def test_correction_cnn_output_scale():
     yb.sum().backward()
     for pa, pb in zip(a.parameters(), b.parameters()):
         assert torch.allclose(pa.grad, 0.05 * pb.grad, rtol=1e-12, atol=1e-300)
+
+
+def test_ks_initial_conditions_structure_and_batch_norm():
+    # KSSolverTorch.initial_conditions (solver_1d.py:214-223) is eager torch
+    # on the solver's device, no HIP call: cos(2 pi x) + alpha sin(2 pi x) on
+    # the endpoint-inclusive linspace(0, 1, N), plus noise eps/||eps|| whose
+    # Frobenius norm over the whole batch is 1.
+    import inc_amd
+    torch.manual_seed(0)
+    p = inc_amd.KSSimParams([22.0, 30.0, 64.0, 100.0], resolution=40, device="cpu")
+    sv = inc_amd.KSSolverTorch(p)
+    u = sv.initial_conditions()
+    x = torch.linspace(0, 1, 40)
+    det = torch.cos(2 * torch.pi * x) + p.alpha[:, None] * torch.sin(2 * torch.pi * x)
+    noise = (u - det).double()
+    assert abs(float(torch.linalg.norm(noise)) - 1.0) < 1e-5
+    assert float((noise.norm(dim=1) - 1).abs().max()) > 1e-2
+    sv.alpha = p.alpha[:3]
+    try:
+        sv.initial_conditions()
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("mismatched alpha / domain_size lengths must raise ValueError")

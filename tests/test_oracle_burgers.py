@@ -188,3 +188,59 @@ def test_burgers_gradient_central_difference():
         fd = (f(u + h * du, s + h * ds) - f(u - h * du, s - h * ds)) / (2 * h)
         an = (u0.grad * du).sum() + (s0.grad * ds).sum()
         np.testing.assert_allclose(fd.item(), an.item(), rtol=1e-6)
+
+
+def _learning_solver(dt, time_scheme, b=2, n=64, l=1.0, nu=0.1):
+    """'learning' forcing sum_j A_j sin(omega_j t + 2 pi l x / L + phi_j) with
+    two modes per trajectory (solver_1d.py:74-85), seeded."""
+    g = torch.Generator().manual_seed(11)
+    A = 0.5 + torch.rand(b, 1, 2, generator=g, dtype=torch.float64)
+    om = 1 + 3 * torch.rand(b, 1, 2, generator=g, dtype=torch.float64)
+    ph = 2 * math.pi * torch.rand(b, 1, 2, generator=g, dtype=torch.float64)
+    ll = torch.full((b, 1, 2), l, dtype=torch.float64)
+    p = burgers_params(n, dt, nu=nu, batch_size=b, time_scheme=time_scheme, equation_type="learning",
+                       J=2, A=A, omega=om, phi=ph, l=ll)
+    return BurgersOracle(p), (A, om, ph)
+
+
+@pytest.mark.parametrize("scheme", ["SSPRK3", "Euler"])
+def test_stage_times_on_a_spatially_constant_state(scheme):
+    # A constant state has zero convection (every cell reconstructs the same
+    # flux) and zero diffusion; with l = 0 the forcing f(t) is constant in
+    # space, so u_t = f(t) and one SSP-RK3 step is exactly Simpson's rule
+    # u + dt/6 (f(t) + 4 f(t + dt/2) + f(t + dt)) -- the stage times t,
+    # t + dt, t + dt/2 -- and one Euler step is u + dt f(t).
+    dt, t0 = 0.07, 0.3
+    so, (A, om, ph) = _learning_solver(dt, scheme, l=0.0)
+    so.t = t0
+    u = torch.tensor([[0.4], [-0.9]], dtype=torch.float64).expand(2, 64).contiguous()
+
+    def f(t):
+        return (A[:, 0] * torch.sin(om[:, 0] * t + ph[:, 0])).sum(-1, keepdim=True)
+
+    if scheme == "SSPRK3":
+        want = u + dt / 6 * (f(t0) + 4 * f(t0 + dt / 2) + f(t0 + dt))
+    else:
+        want = u + dt * f(t0)
+    np.testing.assert_allclose(so.step(u), want, rtol=0, atol=1e-14)
+    assert so.t == pytest.approx(t0 + dt)
+
+
+def test_sspk3_third_order_with_time_dependent_forcing():
+    # Smooth viscous solution with 'learning' forcing (l = 1) over t in
+    # [0.2, 0.7]: the error against a dt = 1/1280 run falls ~8x per halving
+    # of dt (order 3).  A forcing evaluated at a wrong stage time drops the
+    # order to 1 (ratio ~2), so this pins the stage times under transport.
+    def run(dt, tend=0.5):
+        so, _ = _learning_solver(dt, "SSPRK3")
+        so.t = 0.2
+        x = so.x
+        u = torch.stack([0.3 * torch.sin(x), 0.2 * torch.cos(2 * x) + 0.1])
+        for _ in range(int(round(tend / dt))):
+            u = so.step(u)
+        return u
+
+    ref = run(1 / 1280)
+    errs = [float((run(dt) - ref).norm()) for dt in (1 / 20, 1 / 40, 1 / 80)]
+    ratios = [errs[i] / errs[i + 1] for i in range(2)]
+    assert all(6.5 < r < 9.5 for r in ratios), (errs, ratios)

@@ -169,3 +169,75 @@ def test_ks_gradient_central_difference_etdrk4():
         fd = (f(u + h * du, s + h * ds) - f(u - h * du, s - h * ds)) / (2 * h)
         an = (u0.grad * du).sum() + (s0.grad * ds).sum()
         np.testing.assert_allclose(fd.item(), an.item(), rtol=1e-7)
+
+
+def _two_mode_state(n, k1, k2, L):
+    x = torch.arange(n, dtype=torch.float64) * (L / n)
+    return (0.3 * torch.cos(2 * math.pi * k1 * x / L) + 0.2 * torch.cos(2 * math.pi * k2 * x / L + 0.4))[None]
+
+
+@pytest.mark.parametrize("n,k1", [(256, 43), (256, 60), (256, 85), (512, 86), (512, 120), (512, 170)])
+def test_dealias_kills_the_product_of_a_mode_above_n_over_6(n, k1):
+    # u = a cos(2 pi k1 x / L) with n/6 < k1 < n/3: u^2 holds modes 0 and
+    # 2 k1 only.  Mode 0 has g = 0 and |2 k1| (aliased or not) is above the
+    # 2/3 cut n/3, so with the mask the nonlinear term vanishes at every
+    # stage and one ETDRK4 step is the exact linear map e^{(k^2 - k^4) dt} of
+    # the single mode.  Without the mask the product mode survives (2 k1 is
+    # never the Nyquist index n/2 here, where g = 0 anyway).
+    assert n / 6 < k1 < n / 3 and 2 * k1 != n // 2
+    L = 2 * math.pi * k1 / 0.9  # wavenumber 0.9: the mode neither grows nor decays much
+    x = torch.arange(n, dtype=torch.float64) * (L / n)
+    u = 0.3 * torch.cos(2 * math.pi * k1 * x / L)[None]
+    k = 2 * math.pi * k1 / L
+    exact = math.exp((k * k - k ** 4) * 0.25) * u
+    for dealias in (True, False):
+        so = KSOracle(ks_params([L], resolution=n, dt=0.25, time_scheme="ETDRK4", dealias=dealias))
+        so.doublestep = False
+        err = float((so.step(u, so.domain_size) - exact).norm() / exact.norm())
+        if dealias:
+            assert err < 1e-13, err
+        else:
+            assert err > 1e-3, err
+
+
+@pytest.mark.parametrize("n,k1,k2", [(256, 43, 42), (512, 86, 85)])
+def test_dealias_cut_is_between_floor_and_ceil_of_n_over_3(n, k1, k2):
+    # Two modes whose products (k1 + k2, 2 k1, 2 k2) reach exactly the last
+    # kept wavenumber (85 at n = 256, 170 at n = 512) and the first cut one
+    # (86, 171).  After one ETDRK4 step with
+    # the mask every |k| > n/3 is zero to rounding and |k| = floor(n/3) is
+    # not; without the mask the first cut wavenumber is populated.
+    L = 2 * math.pi * k1 / 0.8
+    u = _two_mode_state(n, k1, k2, L)
+    kk = (torch.fft.fftfreq(n, d=1.0 / n)).abs()
+    last, first_cut = n // 3, n // 3 + 1
+    assert {last, first_cut} <= {k1 + k2, 2 * k1, 2 * k2}
+    for dealias in (True, False):
+        so = KSOracle(ks_params([L], resolution=n, dt=0.25, time_scheme="ETDRK4", dealias=dealias))
+        so.doublestep = False
+        sp = torch.fft.fft(so.step(u, so.domain_size)[0]).abs()
+        sp = sp / sp.max()
+        if dealias:
+            assert float(sp[kk >= first_cut].max()) < 1e-14
+            assert float(sp[kk == last].max()) > 1e-3
+        else:
+            assert float(sp[kk == first_cut].max()) > 1e-3
+
+
+def test_initial_conditions_noise_has_unit_frobenius_norm_over_the_batch():
+    # solver_1d.py:214-223: cos(2 pi x) + alpha sin(2 pi x) on the
+    # endpoint-inclusive linspace(0, 1, N), plus eps / ||eps|| with the norm
+    # taken over the whole [B, N] batch (Frobenius), not per trajectory.
+    n = 48
+    alpha = torch.tensor([0.0, 0.5, -1.3], dtype=torch.float64)
+    so = KSOracle(ks_params([22.0, 30.0, 64.0], resolution=n, alpha=alpha))
+    u = so.initial_conditions(generator=torch.Generator().manual_seed(3))
+    x = torch.linspace(0, 1, n, dtype=torch.float64)
+    assert float(x[-1]) == 1.0
+    det = torch.cos(2 * math.pi * x) + alpha[:, None] * torch.sin(2 * math.pi * x)
+    noise = u - det
+    np.testing.assert_allclose(float(torch.linalg.norm(noise)), 1.0, rtol=1e-12)
+    # per-trajectory norms are not each 1 (the batch shares one normaliser)
+    assert float((noise.norm(dim=1) - 1).abs().max()) > 1e-2
+    with pytest.raises(ValueError):
+        KSOracle(ks_params([22.0, 30.0], resolution=n, alpha=alpha)).initial_conditions()
