@@ -106,6 +106,7 @@ SIGNATURES = {
     "inc_conv1d_fwd": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp]),
     "inc_conv1d_bwd_data": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp]),
     "inc_conv1d_bwd_weight": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp]),
+    "inc_conv1d_bwd_weight_noalloc": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp]),
     "inc_conv1d_fwd_act": (ctypes.c_int, [ctypes.c_int] * 7 + [_vp, _vp, _vp, _vp, _vp, _vp]),
     "inc_conv1d_bwd_data_mul": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp]),
 }

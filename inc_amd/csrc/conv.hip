@@ -18,6 +18,7 @@
 //     (row, tile) pairs written to a caller workspace, then a second kernel
 //     reduces the partials in a fixed order (deterministic).
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "common.hpp"
@@ -816,6 +817,41 @@ int conv_check(int dtype, int act, int B, int Ci, int Co, int N, int K) {
   return INC_OK;
 }
 
+// ---- allocation-free weight gradient ------------------------------------------
+// The partial sums of inc_conv1d_bwd_weight_noalloc go to this file-scope
+// device array instead of a caller workspace.  Each GPU the library runs on
+// holds its own copy until the process ends.  In host code `scratch` names a
+// host-side stand-in, not device memory: the device address of each GPU's
+// copy comes from hipGetSymbolAddress, taken once per device with that
+// device current, and only that address is passed to the kernels.
+constexpr size_t kScratchBytes = size_t(32) << 20;  // 32 MiB per GPU
+__device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (nullptr and the
+// error text set on failure).
+void* scratch_address() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("conv1d weight gradient (no allocation): no usable current device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("conv1d weight gradient (no allocation): hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 }  // namespace inc
 
@@ -864,6 +900,26 @@ int inc_conv1d_bwd_weight(int dtype, int B, int Ci, int Co, int N, int K, const 
   hipStream_t st = static_cast<hipStream_t>(stream);
   return dtype == INC_F32 ? inc::conv_wgrad<float>(B, Ci, Co, N, K, g, x, gw, gb, work, work_elems, st)
                           : inc::conv_wgrad<double>(B, Ci, Co, N, K, g, x, gw, gb, work, work_elems, st);
+}
+
+int inc_conv1d_bwd_weight_noalloc(int dtype, int B, int Ci, int Co, int N, int K, const void* g, const void* x,
+                                  void* gw, void* gb, void* stream) {
+  inc::clear_error();
+  int rc = inc::conv_check(dtype, INC_ACT_NONE, B, Ci, Co, N, K);
+  if (rc) return rc;
+  INC_REQUIRE(B > 0, INC_EINVAL, "conv1d weight gradient of an empty batch");
+  const int64_t esize = dtype == INC_F32 ? 4 : 8;
+  const int64_t capacity = int64_t(inc::kScratchBytes) / esize;
+  const int64_t width = int64_t(Co) * Ci * K + Co;
+  INC_REQUIRE(width <= capacity, INC_EINVAL,
+              "conv1d weight gradient (no allocation): one partial row of Co*Ci*K + Co = %lld elements exceeds the "
+              "%zu-byte scratch array",
+              static_cast<long long>(width), inc::kScratchBytes);
+  void* work = inc::scratch_address();
+  if (work == nullptr) return INC_EHIP;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return dtype == INC_F32 ? inc::conv_wgrad<float>(B, Ci, Co, N, K, g, x, gw, gb, work, capacity, st)
+                          : inc::conv_wgrad<double>(B, Ci, Co, N, K, g, x, gw, gb, work, capacity, st);
 }
 
 }  // extern "C"

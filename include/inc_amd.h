@@ -339,6 +339,20 @@ int inc_conv1d_bwd_data(int dtype, int B, int Ci, int Co, int n, int K, const vo
  */
 int inc_conv1d_bwd_weight(int dtype, int B, int Ci, int Co, int n, int K, const void* g, const void* x,
                           void* gw, void* gb, void* work, int64_t work_elems, void* stream);
+/*
+ * inc_conv1d_bwd_weight without a workspace argument and without any device
+ * allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The per-block partial sums go to a 32 MiB array that
+ * the library holds on each GPU it runs on until the process ends; a call
+ * whose partial row (Co*Ci*K + Co elements) does not fit returns INC_EINVAL.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (different streams, or host threads) race and must be ordered by the
+ * caller.  Results equal inc_conv1d_bwd_weight's given a workspace of the
+ * same size.
+ */
+int inc_conv1d_bwd_weight_noalloc(int dtype, int B, int Ci, int Co, int n, int K, const void* g, const void* x,
+                                  void* gw, void* gb, void* stream);
 
 /*
  * Activation fused into the convolution that produces it (the correction
