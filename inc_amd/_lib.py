@@ -70,6 +70,16 @@ class CnnDesc(ctypes.Structure):
     ]
 
 
+class ConvRows(ctypes.Structure):
+    """inc_conv_rows (include/inc_amd.h)."""
+    _fields_ = [
+        ("count", ctypes.c_int32), ("njobs", ctypes.c_int32),
+        ("tile", _vp), ("n", _vp), ("offset", _vp),
+    ]
+
+
+INC_CONV_ROW_TILE = 128
+
 #: every symbol include/inc_amd.h declares, with its ctypes signature
 SIGNATURES = {
     "inc_version": (ctypes.c_int, []),
@@ -109,6 +119,20 @@ SIGNATURES = {
     "inc_conv1d_bwd_weight_noalloc": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp]),
     "inc_conv1d_fwd_act": (ctypes.c_int, [ctypes.c_int] * 7 + [_vp, _vp, _vp, _vp, _vp, _vp]),
     "inc_conv1d_bwd_data_mul": (ctypes.c_int, [ctypes.c_int] * 6 + [_vp, _vp, _vp, _vp, _vp]),
+    "inc_conv1d_rows_fwd_act": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ConvRows)]
+                                + [ctypes.c_int] * 3 + [_vp] * 6),
+    "inc_conv1d_rows_bwd_data_mul": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ConvRows)] + [ctypes.c_int] * 3
+                                     + [_vp] * 5),
+    "inc_conv1d_rows_bwd_weight": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ConvRows)] + [ctypes.c_int] * 3
+                                   + [_vp] * 5 + [ctypes.c_int64, _vp]),
+    "inc_ragged_inc_workspace": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), ctypes.POINTER(CnnDesc), ctypes.c_int,
+                                                ctypes.POINTER(ctypes.c_int64)]),
+    "inc_ragged_inc_fwd": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), ctypes.POINTER(CnnDesc),
+                                          ctypes.POINTER(ConvRows), _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                          ctypes.c_int64, _vp]),
+    "inc_ragged_inc_adj": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), ctypes.POINTER(CnnDesc),
+                                          ctypes.POINTER(ConvRows), _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
+                                          _vp, ctypes.c_int64, _vp]),
 }
 
 _lib = None

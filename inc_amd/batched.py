@@ -69,6 +69,67 @@ class _RaggedAdvance(torch.autograd.Function):
         return g_u, g_s, None, None, None, None, None
 
 
+class _RaggedIncRollout(torch.autograd.Function):
+    """The INC rollout of a ragged batch with a CorrectionCNN
+    (inc_ragged_inc_fwd / _adj): per step one ragged convolution per network
+    layer over the trajectories still inside their horizon and one ragged
+    solver step; the backward evaluates the network again at every step's
+    input state.  Returns the [steps, total] trajectory, differentiable in u0
+    and in the network's parameters (``params`` in the order
+    [w0, b0, w1, b1, ...])."""
+
+    @staticmethod
+    def forward(ctx, u0, rb, model, steps, need_grad, *params):
+        ctx.set_materialize_grads(False)
+        u0 = u0.contiguous()
+        cdesc, keep_w = model.ragged_desc(u0.dtype)
+        d = rb._desc(0)
+        rows, active = rb._rows(steps)
+        lib = _lib.load()
+        elems = ctypes.c_int64(0)
+        _lib.check(lib.inc_ragged_inc_workspace(ctypes.byref(d), ctypes.byref(cdesc), 0, ctypes.byref(elems)),
+                   "RaggedBurgers INC rollout workspace")
+        work = u0.new_empty(elems.value)
+        traj = u0.new_empty(steps, rb.total)
+        ns = 3 if rb.scheme == _lib.INC_SSPRK3 else 1
+        stages = u0.new_empty(steps, ns, rb.total) if need_grad else None
+        rc = lib.inc_ragged_inc_fwd(ctypes.byref(d), ctypes.byref(cdesc), ctypes.byref(rows), active, _lib.ptr(u0),
+                                    steps, _lib.ptr(traj), None, _lib.ptr(stages), _lib.ptr(work), work.numel(),
+                                    _lib.stream())
+        _lib.check(rc, "RaggedBurgers INC rollout")
+        if need_grad:
+            ctx.save_for_backward(u0, traj, stages)
+        ctx.rb, ctx.model, ctx.steps = rb, model, steps
+        ctx.param_shapes = [p.shape for p in params]
+        ctx.keep = (keep_w, active)
+        return traj
+
+    @staticmethod
+    def backward(ctx, g_traj):
+        u0, traj, stages = ctx.saved_tensors
+        rb, steps = ctx.rb, ctx.steps
+        if g_traj is None:
+            g_traj = traj.new_zeros(steps, rb.total)
+        g_traj = g_traj.contiguous()
+        cdesc, keep_w = ctx.model.ragged_desc(u0.dtype)
+        d = rb._desc(0)
+        rows, active = rb._rows(steps)
+        lib = _lib.load()
+        elems = ctypes.c_int64(0)
+        _lib.check(lib.inc_ragged_inc_workspace(ctypes.byref(d), ctypes.byref(cdesc), 1, ctypes.byref(elems)),
+                   "RaggedBurgers INC rollout workspace")
+        work = u0.new_empty(elems.value)
+        g_u0 = torch.empty_like(u0)
+        sizes = [math.prod(sh) for sh in ctx.param_shapes]
+        g_w = u0.new_empty(sum(sizes))
+        rc = lib.inc_ragged_inc_adj(ctypes.byref(d), ctypes.byref(cdesc), ctypes.byref(rows), active, _lib.ptr(u0),
+                                    _lib.ptr(traj), _lib.ptr(stages), _lib.ptr(g_traj), _lib.ptr(g_u0),
+                                    _lib.ptr(g_w), steps, _lib.ptr(work), work.numel(), _lib.stream())
+        _lib.check(rc, "RaggedBurgers INC rollout backward")
+        grads = [g.view(sh) for g, sh in zip(g_w.split(sizes), ctx.param_shapes)]
+        return (g_u0, None, None, None, None, *grads)
+
+
 class RaggedBurgers:
     """A heterogeneous batch of periodic Burgers problems on [0, L).
 
@@ -119,6 +180,20 @@ class RaggedBurgers:
         self.dx_d = torch.tensor([self.L / v for v in self.n], dtype=dtype, device=dev)
         self.nmin = min(self.n) if self.n else 8
         self.nmax = max(self.n) if self.n else 8
+        # the correction network's work: 128-position tiles (row, first
+        # position), trajectories with the longest horizon first, so that the
+        # rows still inside their horizon at a step are a prefix of the list
+        T = _lib.INC_CONV_ROW_TILE
+        hz = self.horizon or [1 << 30] * self.count
+        by_h = sorted(range(self.count), key=lambda i: -hz[i])
+        tiles, self._tile_h = [], []
+        for i in by_h:
+            for i0 in range(0, self.n[i], T):
+                tiles += [i, i0]
+                self._tile_h.append(hz[i])
+        self.ntiles = len(self._tile_h)
+        self.tiles_d = torch.tensor(tiles or [0, 0], dtype=torch.int32, device=dev)
+        self._active = {}
 
     def _desc(self, step0=0):
         d = RaggedDesc()
@@ -130,6 +205,18 @@ class RaggedBurgers:
         d.horizon = self.horizon_d.data_ptr() if self.horizon_d is not None else None
         d.nu, d.dt, d.dx = self.nu_d.data_ptr(), self.dt_d.data_ptr(), self.dx_d.data_ptr()
         return d
+
+    def _rows(self, steps):
+        """(inc_conv_rows, host int32 array [steps] of the tiles still inside
+        their horizon at each step)."""
+        act = self._active.get(steps)
+        if act is None:
+            counts = [sum(1 for h in self._tile_h if h > k) for k in range(steps)]
+            act = (ctypes.c_int32 * max(steps, 1))(*counts)
+            self._active[steps] = act
+        r = _lib.ConvRows(count=self.count, njobs=self.ntiles, tile=self.tiles_d.data_ptr(),
+                          n=self.n_d.data_ptr(), offset=self.off_d.data_ptr())
+        return r, act
 
     # -- packing --------------------------------------------------------------------
     def pack(self, fields):
@@ -158,6 +245,11 @@ class RaggedBurgers:
                     m[h:, self.offsets[i]:self.offsets[i] + self.n[i]] = 0
         return m
 
+    def _fusable(self, model, u0):
+        from .correction import CorrectionCNN
+        return (isinstance(model, CorrectionCNN) and u0.is_cuda and model.ragged_desc(u0.dtype) is not None
+                and 2 * self.nmin >= model.convs[0].k)
+
     # -- stepping -------------------------------------------------------------------
     def advance(self, u, s=None, steps=1, step0=0, want_traj=False):
         """``steps`` steps from rollout step ``step0`` with a fixed correction."""
@@ -175,13 +267,28 @@ class RaggedBurgers:
         need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
         return _RaggedAdvance.apply(u, s, self, steps, step0, want_traj, need)
 
-    def rollout(self, u0, steps, model=None):
-        """[steps, total] trajectory.  With ``model`` (a correction network
-        taking [count, n] -> [count, n]) the correction is re-evaluated on
-        every resolution group before every step (INC); without it the whole
-        rollout is one launch."""
+    def rollout(self, u0, steps, model=None, fused=True):
+        """[steps, total] trajectory.  Without ``model`` the whole rollout is
+        one launch.  With a CorrectionCNN whose parameters live on the GPU in
+        the state's dtype, the INC rollout runs as one call of the library
+        (inc_ragged_inc_fwd; its adjoint inc_ragged_inc_adj): per step one
+        ragged convolution per layer over every resolution at once, skipping
+        trajectories past their horizon, and one ragged solver step.  Any
+        other correction network (a callable [count, n] -> [count, n]), or
+        ``fused=False``, is evaluated per resolution group before every step."""
         if model is None:
             return self.advance(u0, None, steps=steps, step0=0, want_traj=True)
+        if fused and self._fusable(model, u0):
+            if u0.numel() != self.total:
+                raise ValueError(f"packed state has {u0.numel()} values, batch has {self.total}")
+            if u0.dtype != self.dtype:
+                raise TypeError(f"packed state is {u0.dtype}, the ragged batch was built for {self.dtype}")
+            if self._team_auto:
+                self._team_auto = False
+                self.team = autotune.tune_ragged(self)
+            params = [t for cv in model.convs for t in (cv.weight, cv.bias)]
+            need = torch.is_grad_enabled() and (u0.requires_grad or any(p.requires_grad for p in params))
+            return _RaggedIncRollout.apply(u0.reshape(-1), self, model, steps, need, *params)
         outs, u = [], u0
         for k in range(steps):
             s = torch.cat([model(v).reshape(-1) for v in self.group_views(u)])

@@ -198,6 +198,28 @@ class CorrectionCNN(nn.Module):
             d.bias[i] = b.data_ptr()
         return d, keep
 
+    def ragged_desc(self, dtype):
+        """(inc_cnn_desc, tensors to keep alive) for the ragged INC rollout
+        (inc_ragged_inc_fwd / _adj: any width up to 64, odd kernel size up to
+        9, 2..8 layers, the HIP backend, every parameter on the GPU in
+        ``dtype``), or None."""
+        convs = list(self.convs)
+        c, k = convs[0].weight.shape[0], convs[0].k
+        if (convs[0].backend != "hip" or not 2 <= len(convs) <= 8 or not 1 <= c <= 64 or k % 2 != 1 or k > 9
+                or any(cv.k != k for cv in convs)):
+            return None
+        if any(p.dtype != dtype or not p.is_cuda for p in self.parameters()):
+            return None
+        d = _lib.CnnDesc(layers=len(convs), channels=c, kernel=k, scale=self.scale)
+        keep = []
+        for i, cv in enumerate(convs):
+            w = cv.weight.detach().contiguous()
+            b = cv.bias.detach().contiguous()
+            keep += [w, b]
+            d.weight[i] = w.data_ptr()
+            d.bias[i] = b.data_ptr()
+        return d, keep
+
     def forward(self, u, L=None):
         if self.convs[0].backend == "hip":
             params = [t for conv in self.convs for t in (conv.weight, conv.bias)]

@@ -34,6 +34,46 @@ __device__ __forceinline__ int wrapn(int j, int n) {
   return j < 0 ? j + n : j;
 }
 
+// Rows of a convolution.  Uniform: B rows of N points, row b's [C][N] block
+// at element C*b*N.  Ragged (tile != nullptr): row r has n[r] points and its
+// [C][n[r]] block starts at element C*off[r]; the work is a list of
+// kRowTile-position tiles (row, first position), of which a launch takes the
+// first njobs (lists are ordered so that rows still inside their horizon come
+// first).  Positions of a tile past its row's end are neither read into the
+// result nor written.
+constexpr int kRowTile = 128;
+struct RowTiles {
+  const int* tile;
+  const int* n;
+  const int64_t* off;
+};
+
+// How a weight gradient runs: over uniform rows or ragged row tiles (njobs
+// of them); writing or adding to the per-block partial sums; with or without
+// the final reduction (a caller that accumulates over several calls reduces
+// once at the end, over the largest partial count any call used, *nparts).
+struct WgradMode {
+  RowTiles rt;
+  int njobs;
+  bool accumulate;
+  bool reduce;
+  int* nparts;
+};
+
+struct JobPos {
+  int b, i0, n;
+  int64_t base;  // first point of the row (element offset C * base)
+};
+
+__device__ __forceinline__ JobPos job_pos(const RowTiles& rt, int job, int tiles, int TI, int N) {
+  if (rt.tile) {
+    const int b = rt.tile[2 * job], i0 = rt.tile[2 * job + 1];
+    return JobPos{b, i0, rt.n[b], rt.off[b]};
+  }
+  const int b = job / tiles;
+  return JobPos{b, (job - b * tiles) * TI, N, int64_t(b) * N};
+}
+
 // Exact (erf) GELU, torch.nn.GELU(approximate='none'), and its derivative
 // from one erf evaluation: g = gelu(v) = v h with h = (1 + erf(v/sqrt 2))/2,
 // returns gelu'(v) = h + v exp(-v^2/2) / sqrt(2 pi).
@@ -63,22 +103,25 @@ __device__ __forceinline__ T gelu_value(T v) {
 // dy = gelu'(v) (the next layer's input and the derivative its input
 // gradient needs), or y alone when dy is NULL (no gradient wanted).
 template <typename T, int K, int CPT, bool FLIP>
-__global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int N, const T* __restrict__ x,
+__global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int Nu, const T* __restrict__ x,
                                                       const T* __restrict__ w, const T* __restrict__ bias,
                                                       T* __restrict__ y, const T* __restrict__ mul, int act,
-                                                      T* __restrict__ dy) {
+                                                      T* __restrict__ dy, RowTiles rt) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int P = K / 2;
   constexpr int TI = kWave * kPPT;
+  static_assert(TI == kRowTile, "ragged tiles are one block's positions");
   constexpr int XW = TI + K - 1;
   T* xs = reinterpret_cast<T*>(smem_raw);  // [Ci][XW]
-  const int b = blockIdx.z;
-  const int i0 = blockIdx.x * TI;
+  // uniform rows: grid (tiles, channel groups, B); ragged: grid (jobs, channel groups)
+  const JobPos jp = rt.tile ? job_pos(rt, blockIdx.x, 0, TI, 0)
+                            : JobPos{int(blockIdx.z), int(blockIdx.x) * TI, Nu, int64_t(blockIdx.z) * Nu};
+  const int N = jp.n, i0 = jp.i0;
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
   const int lane = tid % kWave;
   const int o0 = (blockIdx.y * kWaves + wave) * CPT;
-  const T* xb = x + size_t(b) * Ci * N;
+  const T* xb = x + jp.base * Ci;
   for (int e = tid; e < Ci * XW; e += blockDim.x) {
     int c = e / XW, j = e - c * XW;
     xs[e] = xb[size_t(c) * N + wrapn(i0 - P + j, N)];
@@ -111,7 +154,7 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int N, co
     const int o = o0 + a;
     if (o >= Co) continue;
     const T bo = bias ? bias[o] : T(0);
-    const size_t row = (size_t(b) * Co + o) * N;
+    const size_t row = size_t(jp.base) * Co + size_t(o) * N;
 #pragma unroll
     for (int p = 0; p < kPPT; ++p) {
       const int i = i0 + p0 + p;
@@ -139,8 +182,9 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int N, co
 // feeds two weight pairs.  LDS rows are skewed to an odd stride so the
 // threads of a wavefront (consecutive input channels) hit distinct banks.
 template <typename T, int K, int MAXP>
-__global__ void __launch_bounds__(256) conv_wgrad_kernel(int B, int Ci, int Co, int N, int TI, const T* __restrict__ g,
-                                                        const T* __restrict__ x, T* __restrict__ part) {
+__global__ void __launch_bounds__(256) conv_wgrad_kernel(int njobs, int Ci, int Co, int Nu, int TI,
+                                                        const T* __restrict__ g, const T* __restrict__ x,
+                                                        T* __restrict__ part, int accumulate, RowTiles rt) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int P = K / 2;
   const int GW = TI | 1;                   // gradient row stride (odd)
@@ -158,12 +202,13 @@ __global__ void __launch_bounds__(256) conv_wgrad_kernel(int B, int Ci, int Co, 
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int k = 0; k < K; ++k) acc[q][r][k] = T(0);
-  const int tiles = (N + TI - 1) / TI;
-  for (int job = blockIdx.x; job < B * tiles; job += gridDim.x) {
-    const int b = job / tiles, i0 = (job - b * tiles) * TI;
+  const int tiles = (Nu + TI - 1) / TI;
+  for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
+    const JobPos jp = job_pos(rt, job, tiles, TI, Nu);
+    const int N = jp.n, i0 = jp.i0;
     __syncthreads();
-    const T* gb = g + size_t(b) * Co * N;
-    const T* xb = x + size_t(b) * Ci * N;
+    const T* gb = g + jp.base * Co;
+    const T* xb = x + jp.base * Ci;
     for (int e = tid; e < Co2 * TI; e += nthr) {
       int o = e / TI, j = e - o * TI;
       gs[o * GW + j] = (o < Co && i0 + j < N) ? gb[size_t(o) * N + i0 + j] : T(0);
@@ -225,11 +270,14 @@ __global__ void __launch_bounds__(256) conv_wgrad_kernel(int B, int Ci, int Co, 
         const int oo = o + (r >> 1), cc = c + (r & 1);
         if (oo < Co && cc < Ci)
 #pragma unroll
-          for (int k = 0; k < K; ++k) out[(size_t(oo) * Ci + cc) * K + k] = acc[q][r][k];
+          for (int k = 0; k < K; ++k) {
+            T* dst = out + (size_t(oo) * Ci + cc) * K + k;
+            *dst = accumulate ? *dst + acc[q][r][k] : acc[q][r][k];
+          }
       }
     }
   }
-  if (tid < Co) out[npairs * K + tid] = bacc;
+  if (tid < Co) out[npairs * K + tid] = accumulate ? out[npairs * K + tid] + bacc : bacc;
 }
 
 // Sum of nparts partial rows: a block covers 16 consecutive elements with 16
@@ -277,11 +325,13 @@ __global__ void __launch_bounds__(256) reduce_parts_kernel(int nparts, int width
 // written to slot blockIdx.x * WP + w / WT of the workspace.
 typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kWgTP = 128;  // positions per job
+static_assert(kWgTP == kRowTile, "ragged tiles are one job's positions");
 
 template <int MT, int NT>
-__global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int B, int Ci, int Co, int N, int K, int WT, int WP,
+__global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci, int Co, int Nu, int K, int WT, int WP,
                                                              const float* __restrict__ g,
-                                                             const float* __restrict__ x, float* __restrict__ part) {
+                                                             const float* __restrict__ x, float* __restrict__ part,
+                                                             int accumulate, RowTiles rt) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int P = K / 2;
   constexpr int GS = kWgTP + 1;             // odd row strides: the 16 rows of a fragment
@@ -313,12 +363,13 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int B, int Ci, int
   for (int m = 0; m < MT; ++m)
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[m][t] = v4f{0.f, 0.f, 0.f, 0.f};
-  const int tiles = (N + kWgTP - 1) / kWgTP;
+  const int tiles = (Nu + kWgTP - 1) / kWgTP;
   const int xw = kWgTP + K - 1;
-  for (int job = blockIdx.x; job < B * tiles; job += gridDim.x) {
-    const int b = job / tiles, i0 = (job - b * tiles) * kWgTP;
-    const float* gb = g + size_t(b) * Co * N;
-    const float* xb = x + size_t(b) * Ci * N;
+  for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
+    const JobPos jp = job_pos(rt, job, tiles, kWgTP, Nu);
+    const int N = jp.n, i0 = jp.i0;
+    const float* gb = g + jp.base * Co;
+    const float* xb = x + jp.base * Ci;
     // Stage the job's tiles with every load of a thread in flight before its
     // LDS writes (the staging is latency-bound otherwise): all MT * 8
     // gradient values, then the input rows eight at a time.
@@ -391,8 +442,8 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int B, int Ci, int
       for (int r = 0; r < 4; ++r) {
         const int o = m * 16 + 4 * lk + r;
         if (o >= Co) continue;
-        if (col < ncw) out[size_t(o) * ncw + col] = acc[m][t][r];
-        else out[size_t(Co) * ncw + o] = acc[m][t][r];
+        float* dst = col < ncw ? out + size_t(o) * ncw + col : out + size_t(Co) * ncw + o;
+        *dst = accumulate ? *dst + acc[m][t][r] : acc[m][t][r];
       }
     }
 }
@@ -400,7 +451,7 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int B, int Ci, int
 // MFMA weight gradient when the shape fits (Co <= 64, Ci*K + 1 <= 512, LDS);
 // returns INC_EUNSUPPORTED (without touching the error text) otherwise.
 int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const float* x, float* dw, float* db,
-                    float* part, int64_t work_elems, hipStream_t st) {
+                    float* part, int64_t work_elems, hipStream_t st, const WgradMode& md) {
   const int MT = (Co + 15) / 16;
   const int ncw = Ci * K, ntile = (ncw + 1 + 15) / 16;
   const int WT = ntile >= 4 ? 4 : (ntile >= 2 ? 2 : 1);
@@ -413,11 +464,15 @@ int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const f
   const int64_t slots = std::min<int64_t>(512, work_elems / width);  // two blocks per CU
   if (slots < 1) return INC_EUNSUPPORTED;
   const int WP = int(std::min<int64_t>(4 / WT, slots));
-  const int jobs = B * ((N + kWgTP - 1) / kWgTP);
+  const int jobs = md.rt.tile ? md.njobs : B * ((N + kWgTP - 1) / kWgTP);
   const int nblk = int(std::min<int64_t>(jobs, slots / WP));
+  if (md.nparts) *md.nparts = std::max(*md.nparts, nblk * WP);
+  if (nblk < 1) return INC_OK;  // no active rows: the partial sums stay as they are
   dim3 grid(nblk), block(256);
-#define INC_WGM(M, T) \
-  hipLaunchKernelGGL((conv_wgrad_mfma_kernel<M, T>), grid, block, lds, st, B, Ci, Co, N, K, WT, WP, g, x, part)
+  const int accumulate = md.accumulate ? 1 : 0;
+#define INC_WGM(M, T)                                                                                                 \
+  hipLaunchKernelGGL((conv_wgrad_mfma_kernel<M, T>), grid, block, lds, st, jobs, Ci, Co, N, K, WT, WP, g, x, part, \
+                     accumulate, md.rt)
 #define INC_WGM_T(M)                  \
   switch (NT) {                       \
     case 1: INC_WGM(M, 1); break;     \
@@ -436,7 +491,7 @@ int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const f
 #undef INC_WGM_T
 #undef INC_WGM
   int rc = check_launch("conv1d weight gradient (matrix cores)");
-  if (rc) return rc;
+  if (rc || !md.reduce) return rc;
   hipLaunchKernelGGL(reduce_parts_kernel<float>, dim3((width + 15) / 16), dim3(256), 0, st, nblk * WP, width, part,
                      dw, db, Co * ncw);
   return check_launch("conv1d weight gradient reduction");
@@ -460,12 +515,12 @@ int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const f
 // flip: the input-gradient operator, as in conv_fwd_kernel; the epilogue is
 // conv_fwd_kernel's (bias, mul, GELU with its derivative).
 template <int CT, int PW>
-__global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int Co, int N, int K, int flip,
+__global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, int Co, int Nu, int K, int flip,
                                                             const float* __restrict__ x,
                                                             const float* __restrict__ w,
                                                             const float* __restrict__ bias, float* __restrict__ y,
                                                             const float* __restrict__ mul, int act,
-                                                            float* __restrict__ dy) {
+                                                            float* __restrict__ dy, RowTiles rt) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int TI = 64 * PW;
   const int P = K / 2;
@@ -499,8 +554,8 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int C
         if (k < K) dst[k * Ci] = v[u][k];
     }
   }
-  const int tiles = (N + TI - 1) / TI, nchunk = R / 16, c4n = Ci / 4;
-  const int njobs = B * tiles, items = c4n * XW;
+  const int tiles = (Nu + TI - 1) / TI, nchunk = R / 16, c4n = Ci / 4;
+  const int items = c4n * XW;
   const int p0 = wave * 16 * PW;
   // A tile is c4n * XW items (channel quad c4, row j): item e = tid + 256 u
   // of this thread is quad e / XW at row e % XW, fetched as four coalesced
@@ -526,8 +581,9 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int C
     }
   }
   auto fetch = [&](int job, v4f* v) __attribute__((always_inline)) {
-    const int b = job / tiles, i0 = (job - b * tiles) * TI;
-    const float* xb = x + size_t(b) * Ci * N;
+    const JobPos jp = job_pos(rt, job, tiles, TI, Nu);
+    const int N = jp.n, i0 = jp.i0;
+    const float* xb = x + jp.base * Ci;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       // unconditional loads (items past the tile read quad c4n - 1 and are
@@ -548,7 +604,9 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int C
   v4f nx[U];
   if (int(blockIdx.x) < njobs) fetch(blockIdx.x, nx);
   for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
-    const int b = job / tiles, i0 = (job - b * tiles) * TI;
+    const JobPos jp = job_pos(rt, job, tiles, TI, Nu);
+    const int N = jp.n, i0 = jp.i0;
+    const size_t ybase = size_t(jp.base) * Co;
     __syncthreads();  // the weights are staged and the previous tile consumed
     stage(nx);
     __syncthreads();
@@ -622,7 +680,7 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int C
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int o = min(m * 16 + 4 * q + r, Co - 1), i = min(i0 + p0 + 16 * t + lr, N - 1);
-          mv[m][t][r] = mul ? mul[(size_t(b) * Co + o) * N + i] : 1.f;
+          mv[m][t][r] = mul ? mul[ybase + size_t(o) * N + i] : 1.f;
         }
 #pragma unroll
     for (int m = 0; m < CT; ++m)
@@ -632,7 +690,7 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int B, int Ci, int C
         for (int r = 0; r < 4; ++r) {
           const int o = m * 16 + 4 * q + r, i = i0 + p0 + 16 * t + lr;
           if (o >= Co || i >= N) continue;
-          const size_t at = (size_t(b) * Co + o) * N + i;
+          const size_t at = ybase + size_t(o) * N + i;
           float v = (acc[m][t][r] + (bias ? bias[o] : 0.f)) * mv[m][t][r];
           if (act == INC_ACT_GELU) {
             if (dy) {
@@ -665,17 +723,40 @@ int cu_count() {
 // 16*PW with PW = 2 when N > 64 and two blocks' LDS fit a CU, else 1.
 // Blocks loop over jobs; the grid is what the CUs hold at once (the
 // occupancy query: a block that started late would extend the tail).
+// Resident blocks per CU of a kernel at a dynamic LDS size, queried once per
+// (kernel, LDS size) and cached: the query is a runtime call, and a call
+// that is being captured into a hipGraph then only enqueues its kernel.
+int blocks_per_cu(const void* fn, size_t lds) {
+  struct Entry {
+    const void* fn;
+    size_t lds;
+    int per_cu;
+  };
+  static std::mutex mu;
+  static Entry cache[64];
+  static int used = 0;
+  std::lock_guard<std::mutex> lock(mu);
+  for (int i = 0; i < used; ++i)
+    if (cache[i].fn == fn && cache[i].lds == lds) return cache[i].per_cu;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds) != hipSuccess) return 0;
+  if (used < 64) cache[used++] = Entry{fn, lds, per_cu};
+  return per_cu;
+}
+
 int conv_fwd_mfma(int B, int Ci, int Co, int N, int K, bool flip, const float* x, const float* w,
-                  const float* bias, float* y, const float* mul, int act, float* dy, hipStream_t st) {
+                  const float* bias, float* y, const float* mul, int act, float* dy, hipStream_t st,
+                  RowTiles rt, int rjobs) {
   if (Ci % 16 != 0 || Ci > 64 || Co < 16 || Co > 64) return INC_EUNSUPPORTED;
   const int CT = (Co + 15) / 16;
   auto lds_of = [&](int pw) {  // staged weights and the input tile (with its spare row)
     return (size_t(CT) * 16 * (Ci * K + 4) + size_t(64 * pw + K) * (Ci + 4)) * sizeof(float);
   };
-  const int PW = (N > 64 && lds_of(2) <= 80 * 1024) ? 2 : 1;
+  // ragged rows come as kRowTile = 128-position tiles: two 16-position groups per wavefront
+  const int PW = rt.tile ? 2 : ((N > 64 && lds_of(2) <= 80 * 1024) ? 2 : 1);
   const size_t lds = lds_of(PW);
   if (lds > 150 * 1024) return INC_EUNSUPPORTED;
-  const int64_t jobs = int64_t(B) * ((N + 64 * PW - 1) / (64 * PW));
+  const int64_t jobs = rt.tile ? rjobs : int64_t(B) * ((N + 64 * PW - 1) / (64 * PW));
   const void* fn;
   switch (CT * 2 + PW - 1) {
     case 2: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<1, 1>); break;
@@ -687,22 +768,23 @@ int conv_fwd_mfma(int B, int Ci, int Co, int N, int K, bool flip, const float* x
     case 8: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<4, 1>); break;
     default: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<4, 2>); break;
   }
-  int per_cu = 0;
-  INC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds));
+  const int per_cu = blocks_per_cu(fn, lds);
   if (per_cu < 1) return INC_EUNSUPPORTED;
+  if (jobs < 1) return INC_OK;
   const int nblk = int(std::min<int64_t>(jobs, int64_t(cu_count()) * per_cu));
   int fl = flip ? 1 : 0;
-  void* args[] = {&B, &Ci, &Co, &N, &K, &fl, &x, &w, &bias, &y, &mul, &act, &dy};
+  int nj = int(jobs);
+  void* args[] = {&nj, &Ci, &Co, &N, &K, &fl, &x, &w, &bias, &y, &mul, &act, &dy, &rt};
   INC_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(256), args, lds, st));
   return check_launch("conv1d (matrix cores)");
 }
 
 template <typename T, int K, bool FLIP>
 int conv_fwd_k(int cpt, dim3 grid, size_t lds, hipStream_t st, int Ci, int Co, int N, const T* x, const T* w,
-               const T* b, T* y, const T* mul, int act, T* dy) {
+               const T* b, T* y, const T* mul, int act, T* dy, RowTiles rt) {
   dim3 block(kWaves * kWave);
 #define INC_CF(C) \
-  hipLaunchKernelGGL((conv_fwd_kernel<T, K, C, FLIP>), grid, block, lds, st, Ci, Co, N, x, w, b, y, mul, act, dy)
+  hipLaunchKernelGGL((conv_fwd_kernel<T, K, C, FLIP>), grid, block, lds, st, Ci, Co, N, x, w, b, y, mul, act, dy, rt)
   switch (cpt) {
     case 1: INC_CF(1); break;
     case 2: INC_CF(2); break;
@@ -726,18 +808,20 @@ int conv_cpt(int Co) {
 
 template <typename T, bool FLIP>
 int conv_fwd_launch(int B, int Ci, int Co, int N, int K, const void* x, const void* w, const void* bias, void* y,
-                    const void* mul, int act, void* dy, hipStream_t st) {
+                    const void* mul, int act, void* dy, hipStream_t st, RowTiles rt = RowTiles{}, int rjobs = 0) {
   if constexpr (std::is_same<T, float>::value) {
     const int rc = conv_fwd_mfma(B, Ci, Co, N, K, FLIP, static_cast<const float*>(x), static_cast<const float*>(w),
                                  static_cast<const float*>(bias), static_cast<float*>(y),
-                                 static_cast<const float*>(mul), act, static_cast<float*>(dy), st);
+                                 static_cast<const float*>(mul), act, static_cast<float*>(dy), st, rt, rjobs);
     if (rc != INC_EUNSUPPORTED) return rc;
   }
   const int TI = kWave * kPPT;
   size_t lds = size_t(Ci) * (TI + K - 1) * sizeof(T);
   INC_REQUIRE(lds <= 150 * 1024, INC_EUNSUPPORTED, "conv1d: %d input channels need %zu bytes of LDS", Ci, lds);
   const int cpt = conv_cpt(Co);
-  dim3 grid((N + TI - 1) / TI, (Co + kWaves * cpt - 1) / (kWaves * cpt), B);
+  const unsigned cg = (Co + kWaves * cpt - 1) / (kWaves * cpt);
+  if (rt.tile && rjobs < 1) return INC_OK;
+  dim3 grid = rt.tile ? dim3(rjobs, cg, 1) : dim3((N + TI - 1) / TI, cg, B);
   const T* xp = static_cast<const T*>(x);
   const T* wp = static_cast<const T*>(w);
   const T* bp = static_cast<const T*>(bias);
@@ -745,23 +829,24 @@ int conv_fwd_launch(int B, int Ci, int Co, int N, int K, const void* x, const vo
   T* yp = static_cast<T*>(y);
   T* dp = static_cast<T*>(dy);
   switch (K) {
-    case 1: return conv_fwd_k<T, 1, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp);
-    case 3: return conv_fwd_k<T, 3, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp);
-    case 5: return conv_fwd_k<T, 5, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp);
-    case 7: return conv_fwd_k<T, 7, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp);
-    case 9: return conv_fwd_k<T, 9, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp);
+    case 1: return conv_fwd_k<T, 1, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp, rt);
+    case 3: return conv_fwd_k<T, 3, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp, rt);
+    case 5: return conv_fwd_k<T, 5, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp, rt);
+    case 7: return conv_fwd_k<T, 7, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp, rt);
+    case 9: return conv_fwd_k<T, 9, FLIP>(cpt, grid, lds, st, Ci, Co, N, xp, wp, bp, yp, mp, act, dp, rt);
     default: set_error("conv1d kernel size must be 1, 3, 5, 7 or 9 (got %d)", K); return INC_EUNSUPPORTED;
   }
 }
 
 template <typename T, int K>
-int wgrad_launch_k(int B, int Ci, int Co, int N, int TI, int nblk, size_t lds, const T* g, const T* x, T* part,
-                   hipStream_t st) {
+int wgrad_launch_k(int njobs, int Ci, int Co, int N, int TI, int nblk, size_t lds, const T* g, const T* x, T* part,
+                   hipStream_t st, int accumulate, RowTiles rt) {
   const int nblk2 = ((Co + 1) / 2) * ((Ci + 1) / 2);
   const int maxp = (nblk2 + 255) / 256;
   dim3 grid(nblk), block(256);
-#define INC_WG(MP) \
-  hipLaunchKernelGGL((conv_wgrad_kernel<T, K, MP>), grid, block, lds, st, B, Ci, Co, N, TI, g, x, part)
+#define INC_WG(MP)                                                                                            \
+  hipLaunchKernelGGL((conv_wgrad_kernel<T, K, MP>), grid, block, lds, st, njobs, Ci, Co, N, TI, g, x, part, \
+                     accumulate, rt)
   if (maxp <= 1) INC_WG(1);
   else if (maxp <= 2) INC_WG(2);
   else if (maxp <= 3) INC_WG(3);
@@ -776,34 +861,38 @@ int wgrad_launch_k(int B, int Ci, int Co, int N, int TI, int nblk, size_t lds, c
 
 template <typename T>
 int conv_wgrad(int B, int Ci, int Co, int N, int K, const void* g, const void* x, void* dw, void* db, void* work,
-               int64_t work_elems, hipStream_t st) {
+               int64_t work_elems, hipStream_t st, const WgradMode& md = WgradMode{RowTiles{}, 0, false, true, nullptr}) {
   if constexpr (std::is_same<T, float>::value) {
     const int rc = conv_wgrad_mfma(B, Ci, Co, N, K, static_cast<const float*>(g), static_cast<const float*>(x),
                                    static_cast<float*>(dw), static_cast<float*>(db), static_cast<float*>(work),
-                                   work_elems, st);
+                                   work_elems, st, md);
     if (rc != INC_EUNSUPPORTED) return rc;
   }
-  int TI = N < 128 ? N : 128;
+  int TI = md.rt.tile ? kRowTile : (N < 128 ? N : 128);
   size_t lds = (size_t((Co + 1) & ~1) * (TI | 1) + size_t((Ci + 1) & ~1) * ((TI + K - 1) | 1)) * sizeof(T);
   INC_REQUIRE(lds <= 150 * 1024, INC_EUNSUPPORTED, "conv1d weight gradient needs %zu bytes of LDS", lds);
   const int width = Co * Ci * K + Co;
-  int jobs = B * ((N + TI - 1) / TI);
+  const int jobs = md.rt.tile ? md.njobs : B * ((N + TI - 1) / TI);
   int nblk = jobs < 256 ? jobs : 256;  // about one block per CU, each looping over several tiles
   if (int64_t(nblk) * width > work_elems) nblk = int(work_elems / width);
-  INC_REQUIRE(nblk >= 1, INC_EINVAL, "conv1d weight gradient workspace too small (need >= %d elements)", width);
+  INC_REQUIRE(jobs < 1 || nblk >= 1, INC_EINVAL, "conv1d weight gradient workspace too small (need >= %d elements)",
+              width);
+  if (md.nparts) *md.nparts = std::max(*md.nparts, nblk);
+  if (nblk < 1) return INC_OK;
   const T* gp = static_cast<const T*>(g);
   const T* xp = static_cast<const T*>(x);
   T* part = static_cast<T*>(work);
+  const int acc = md.accumulate ? 1 : 0;
   int rc;
   switch (K) {
-    case 1: rc = wgrad_launch_k<T, 1>(B, Ci, Co, N, TI, nblk, lds, gp, xp, part, st); break;
-    case 3: rc = wgrad_launch_k<T, 3>(B, Ci, Co, N, TI, nblk, lds, gp, xp, part, st); break;
-    case 5: rc = wgrad_launch_k<T, 5>(B, Ci, Co, N, TI, nblk, lds, gp, xp, part, st); break;
-    case 7: rc = wgrad_launch_k<T, 7>(B, Ci, Co, N, TI, nblk, lds, gp, xp, part, st); break;
-    case 9: rc = wgrad_launch_k<T, 9>(B, Ci, Co, N, TI, nblk, lds, gp, xp, part, st); break;
+    case 1: rc = wgrad_launch_k<T, 1>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt); break;
+    case 3: rc = wgrad_launch_k<T, 3>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt); break;
+    case 5: rc = wgrad_launch_k<T, 5>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt); break;
+    case 7: rc = wgrad_launch_k<T, 7>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt); break;
+    case 9: rc = wgrad_launch_k<T, 9>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt); break;
     default: set_error("conv1d kernel size must be 1, 3, 5, 7 or 9 (got %d)", K); return INC_EUNSUPPORTED;
   }
-  if (rc) return rc;
+  if (rc || !md.reduce) return rc;
   hipLaunchKernelGGL(reduce_parts_kernel<T>, dim3((width + 15) / 16), dim3(256), 0, st, nblk, width, part,
                      static_cast<T*>(dw), static_cast<T*>(db), Co * Ci * K);
   return check_launch("conv1d weight gradient reduction");
@@ -814,6 +903,267 @@ int conv_check(int dtype, int act, int B, int Ci, int Co, int N, int K) {
   INC_REQUIRE(act == INC_ACT_NONE || act == INC_ACT_GELU, INC_EINVAL, "unknown activation %d", act);
   INC_REQUIRE(B >= 0 && Ci >= 1 && Co >= 1 && N >= 1, INC_EINVAL, "conv1d: bad sizes");
   INC_REQUIRE(K % 2 == 1 && K <= 9 && K <= N, INC_EUNSUPPORTED, "conv1d kernel size must be odd, <= 9 and <= n");
+  return INC_OK;
+}
+
+int rows_check(int dtype, int act, const inc_conv_rows* r, int Ci, int Co, int K) {
+  INC_REQUIRE(r != nullptr, INC_EINVAL, "null rows descriptor");
+  INC_REQUIRE(dtype == INC_F32 || dtype == INC_F64, INC_EINVAL, "unsupported dtype %d", dtype);
+  INC_REQUIRE(act == INC_ACT_NONE || act == INC_ACT_GELU, INC_EINVAL, "unknown activation %d", act);
+  INC_REQUIRE(r->count >= 0 && r->njobs >= 0 && Ci >= 1 && Co >= 1, INC_EINVAL, "conv1d rows: bad sizes");
+  INC_REQUIRE(K % 2 == 1 && K <= 9, INC_EUNSUPPORTED, "conv1d kernel size must be odd and <= 9");
+  INC_REQUIRE(r->njobs == 0 || (r->tile && r->n && r->offset), INC_EINVAL, "conv1d rows: tile, n, offset missing");
+  return INC_OK;
+}
+
+RowTiles row_tiles(const inc_conv_rows* r) { return RowTiles{r->tile, r->n, r->offset}; }
+
+// ---- the INC rollout of a ragged batch --------------------------------------------
+// out = a x + y + z (y, z optional), elementwise over n values.
+template <typename T>
+__global__ void __launch_bounds__(256) lincomb_kernel(int64_t n, T a, const T* __restrict__ x,
+                                                     const T* __restrict__ y, const T* __restrict__ z,
+                                                     T* __restrict__ out) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+    T v = a * x[i];
+    if (y) v += y[i];
+    if (z) v += z[i];
+    out[i] = v;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) fill_kernel(int64_t n, T v, T* __restrict__ out) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) out[i] = v;
+}
+
+template <typename T>
+int lincomb(int64_t n, T a, const T* x, const T* y, const T* z, T* out, hipStream_t st) {
+  if (n <= 0) return INC_OK;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(lincomb_kernel<T>, dim3(unsigned(blocks)), dim3(256), 0, st, n, a, x, y, z, out);
+  return check_launch("ragged INC elementwise");
+}
+
+template <typename T>
+int fill(int64_t n, T v, T* out, hipStream_t st) {
+  if (n <= 0) return INC_OK;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(unsigned(blocks)), dim3(256), 0, st, n, v, out);
+  return check_launch("ragged INC fill");
+}
+
+// Partial-sum slots per layer for the accumulated weight gradients (the
+// matrix-core kernel uses at most 512, the vector-ALU kernel 256).
+constexpr int kIncSlots = 512;
+
+// Workspace of the ragged INC rollout, in elements of the state's type.
+// forward: two activation buffers [C][total], the correction [total] and the
+// scaled last layer (C K + 1); adjoint: the recomputed activations and their
+// GELU derivatives ((L - 1) each), two cotangent buffers [C][total], four
+// [total] vectors, the scaled last layer and every layer's partial sums.
+struct IncLayout {
+  int64_t act[2], s, wl, y[8], dy[8], g[2], lam_s, lam, lam_sol, gx, part[8], total;
+};
+
+IncLayout inc_layout(const inc_ragged_desc* d, const inc_cnn_desc* cnn, bool adj) {
+  IncLayout L{};
+  const int64_t C = cnn->channels, tot = d->total, K = cnn->kernel;
+  int64_t o = 0;
+  auto take = [&](int64_t n) {
+    const int64_t at = o;
+    o += (n + 63) / 64 * 64;  // 256- or 512-byte aligned pieces
+    return at;
+  };
+  L.wl = take(C * K + 1);
+  if (!adj) {
+    L.act[0] = take(C * tot);
+    L.act[1] = take(C * tot);
+    L.s = take(tot);
+  } else {
+    for (int l = 0; l < cnn->layers - 1; ++l) {
+      L.y[l] = take(C * tot);
+      L.dy[l] = take(C * tot);
+    }
+    L.g[0] = take(C * tot);
+    L.g[1] = take(C * tot);
+    L.lam_s = take(tot);
+    L.lam = take(tot);
+    L.lam_sol = take(tot);
+    L.gx = take(tot);
+    for (int l = 0; l < cnn->layers; ++l) {
+      const int64_t ci = l == 0 ? 1 : C, co = l == cnn->layers - 1 ? 1 : C;
+      L.part[l] = take(kIncSlots * (co * ci * K + co));
+    }
+  }
+  L.total = o;
+  return L;
+}
+
+int inc_check(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* r) {
+  INC_REQUIRE(d && cnn && r, INC_EINVAL, "null descriptor");
+  INC_REQUIRE(d->dtype == INC_F32 || d->dtype == INC_F64, INC_EINVAL, "unsupported dtype %d", d->dtype);
+  INC_REQUIRE(cnn->layers >= 2 && cnn->layers <= 8, INC_EUNSUPPORTED, "the network needs 2..8 layers (got %d)",
+              cnn->layers);
+  INC_REQUIRE(cnn->channels >= 1 && cnn->channels <= 64, INC_EUNSUPPORTED, "network width must be 1..64");
+  INC_REQUIRE(cnn->kernel % 2 == 1 && cnn->kernel <= 9, INC_EUNSUPPORTED, "kernel size must be odd and <= 9");
+  INC_REQUIRE(2 * d->nmin >= cnn->kernel, INC_EUNSUPPORTED, "every trajectory needs >= K/2 points");
+  for (int l = 0; l < cnn->layers; ++l) INC_REQUIRE(cnn->weight[l] && cnn->bias[l], INC_EINVAL, "missing layer %d", l);
+  INC_REQUIRE(r->count == d->count, INC_EINVAL, "rows (%d) and trajectories (%d) differ", r->count, d->count);
+  return rows_check(d->dtype, INC_ACT_GELU, r, 1, 1, cnn->kernel);
+}
+
+// The network on the state `in` for the rows of the first `jobs` tiles:
+// hidden activations to y[l] (with their GELU derivatives to dy[l] when dy
+// is given), the scaled correction to s.
+template <typename T>
+int inc_network(const inc_cnn_desc* cnn, const T* wl, const T* in, T* const* y, T* const* dy, T* s, RowTiles rt,
+                int jobs, hipStream_t st) {
+  const int C = cnn->channels, K = cnn->kernel, nl = cnn->layers;
+  const T* x = in;
+  for (int l = 0; l < nl - 1; ++l) {
+    const int ci = l == 0 ? 1 : C;
+    const int rc = conv_fwd_launch<T, false>(0, ci, C, 0, K, x, cnn->weight[l], cnn->bias[l], y[l], nullptr,
+                                             INC_ACT_GELU, dy ? dy[l] : nullptr, st, rt, jobs);
+    if (rc) return rc;
+    x = y[l];
+  }
+  if (s == nullptr) return INC_OK;  // the adjoint's recomputation needs the hidden layers only
+  return conv_fwd_launch<T, false>(0, C, 1, 0, K, x, wl, wl + C * K, s, nullptr, INC_ACT_NONE, nullptr, st, rt, jobs);
+}
+
+template <typename T>
+int inc_fwd(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* r, const int32_t* active,
+            const T* u0, int steps, T* traj, T* corr, T* stages, T* work, hipStream_t st) {
+  const IncLayout L = inc_layout(d, cnn, false);
+  const int C = cnn->channels, K = cnn->kernel, nl = cnn->layers;
+  const int64_t tot = d->total;
+  const int ns = d->scheme == INC_SSPRK3 ? 3 : 1;
+  T* wl = work + L.wl;
+  int rc = lincomb<T>(int64_t(C) * K, T(cnn->scale), static_cast<const T*>(cnn->weight[nl - 1]), nullptr, nullptr,
+                      wl, st);
+  if (!rc) rc = lincomb<T>(1, T(cnn->scale), static_cast<const T*>(cnn->bias[nl - 1]), nullptr, nullptr, wl + C * K,
+                           st);
+  if (!rc && corr) rc = fill<T>(int64_t(steps) * tot, T(0), corr, st);
+  if (rc) return rc;
+  T* bufs[8];
+  for (int l = 0; l < nl - 1; ++l) bufs[l] = work + L.act[l & 1];
+  const RowTiles rt = row_tiles(r);
+  inc_ragged_desc dk = *d;
+  const T* in = u0;
+  for (int k = 0; k < steps; ++k) {
+    const int jobs = active ? active[k] : r->njobs;
+    T* s = corr ? corr + int64_t(k) * tot : work + L.s;
+    rc = inc_network<T>(cnn, wl, in, bufs, nullptr, s, rt, jobs, st);
+    if (rc) return rc;
+    dk.step0 = k;
+    T* out = traj + int64_t(k) * tot;
+    rc = inc_burgers_ragged_fwd(&dk, in, s, out, 1, nullptr, stages ? stages + int64_t(k) * ns * tot : nullptr, st);
+    if (rc) return rc;
+    in = out;
+  }
+  return INC_OK;
+}
+
+template <typename T>
+int inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* r, const int32_t* active,
+            const T* u0, const T* traj, const T* stages, const T* g_traj, T* g_u0, T* g_w, int steps, T* work,
+            hipStream_t st) {
+  const IncLayout L = inc_layout(d, cnn, true);
+  const int C = cnn->channels, K = cnn->kernel, nl = cnn->layers;
+  const int64_t tot = d->total;
+  const int ns = d->scheme == INC_SSPRK3 ? 3 : 1;
+  T* wl = work + L.wl;
+  int rc = lincomb<T>(int64_t(C) * K, T(cnn->scale), static_cast<const T*>(cnn->weight[nl - 1]), nullptr, nullptr,
+                      wl, st);
+  if (!rc) rc = lincomb<T>(1, T(cnn->scale), static_cast<const T*>(cnn->bias[nl - 1]), nullptr, nullptr, wl + C * K,
+                           st);
+  // every layer's partial sums and the network's input cotangent start at
+  // zero (rows past their horizon are never written by the network's passes)
+  if (!rc) rc = fill<T>(L.total - L.part[0], T(0), work + L.part[0], st);
+  if (!rc) rc = fill<T>(tot, T(0), work + L.gx, st);
+  if (rc) return rc;
+  T* y[8];
+  T* dy[8];
+  for (int l = 0; l < nl - 1; ++l) {
+    y[l] = work + L.y[l];
+    dy[l] = work + L.dy[l];
+  }
+  T* lam = work + L.lam;
+  T* lam_s = work + L.lam_s;
+  T* lam_sol = work + L.lam_sol;
+  T* gx = work + L.gx;
+  T* g[2] = {work + L.g[0], work + L.g[1]};
+  const RowTiles rt = row_tiles(r);
+  int nparts[8] = {};
+  int64_t width[8];
+  for (int l = 0; l < nl; ++l) {
+    const int ci = l == 0 ? 1 : C, co = l == nl - 1 ? 1 : C;
+    width[l] = int64_t(co) * ci * K + co;
+  }
+  inc_ragged_desc dk = *d;
+  const T* lam_in = g_traj + int64_t(steps - 1) * tot;  // cotangent of the last state
+  for (int k = steps - 1; k >= 0; --k) {
+    const int jobs = active ? active[k] : r->njobs;
+    const T* in = k == 0 ? u0 : traj + int64_t(k - 1) * tot;
+    // the solver step's transpose: cotangents of its input state and of s
+    dk.step0 = k;
+    rc = inc_burgers_ragged_adj(&dk, stages + int64_t(k) * ns * tot, lam_in, lam_sol, lam_s, 1, st);
+    if (rc) return rc;
+    // the network again at the step's input state, with its GELU derivatives
+    rc = inc_network<T>(cnn, wl, in, y, dy, nullptr, rt, jobs, st);
+    if (rc) return rc;
+    if (jobs > 0) {
+      // last layer (C -> 1): weight gradient of lam_s x y[L-2] (scaled at the end), input cotangent
+      // scale W^T lam_s times the GELU derivative
+      WgradMode md{rt, jobs, true, false, &nparts[nl - 1]};
+      rc = conv_wgrad<T>(0, C, 1, 0, K, lam_s, y[nl - 2], nullptr, nullptr, work + L.part[nl - 1],
+                         kIncSlots * width[nl - 1], st, md);
+      if (!rc) rc = conv_fwd_launch<T, true>(0, 1, C, 0, K, lam_s, wl, nullptr, g[0], dy[nl - 2], INC_ACT_NONE,
+                                             nullptr, st, rt, jobs);
+      int cur = 0;
+      for (int l = nl - 2; l >= 1 && !rc; --l) {
+        WgradMode mh{rt, jobs, true, false, &nparts[l]};
+        rc = conv_wgrad<T>(0, C, C, 0, K, g[cur], y[l - 1], nullptr, nullptr, work + L.part[l],
+                           kIncSlots * width[l], st, mh);
+        if (!rc) rc = conv_fwd_launch<T, true>(0, C, C, 0, K, g[cur], cnn->weight[l], nullptr, g[cur ^ 1], dy[l - 1],
+                                               INC_ACT_NONE, nullptr, st, rt, jobs);
+        cur ^= 1;
+      }
+      if (!rc) {
+        WgradMode m0{rt, jobs, true, false, &nparts[0]};
+        rc = conv_wgrad<T>(0, 1, C, 0, K, g[cur], in, nullptr, nullptr, work + L.part[0], kIncSlots * width[0], st,
+                           m0);
+      }
+      if (!rc) rc = conv_fwd_launch<T, true>(0, C, 1, 0, K, g[cur], cnn->weight[0], nullptr, gx, nullptr,
+                                             INC_ACT_NONE, nullptr, st, rt, jobs);
+      if (rc) return rc;
+    }
+    // cotangent of this step's input state: through the solver, through the
+    // network, and the loss's own cotangent of that state (for k > 0)
+    T* dst = k == 0 ? g_u0 : lam;
+    rc = lincomb<T>(tot, T(1), lam_sol, gx, k == 0 ? nullptr : g_traj + int64_t(k - 1) * tot, dst, st);
+    if (rc) return rc;
+    lam_in = lam;
+  }
+  // reduce every layer's partial sums into g_w = [w0, b0, w1, b1, ...]; the
+  // last layer's times the output scale
+  int64_t at = 0;
+  for (int l = 0; l < nl; ++l) {
+    const int nw = int(width[l]) - (l == nl - 1 ? 1 : C);
+    if (nparts[l] > 0) {
+      hipLaunchKernelGGL(reduce_parts_kernel<T>, dim3(unsigned((width[l] + 15) / 16)), dim3(256), 0, st, nparts[l],
+                         int(width[l]), work + L.part[l], g_w + at, g_w + at + nw, nw);
+      rc = check_launch("ragged INC weight-gradient reduction");
+    } else {
+      rc = fill<T>(width[l], T(0), g_w + at, st);
+    }
+    if (rc) return rc;
+    if (l == nl - 1 && cnn->scale != 1.0)
+      rc = lincomb<T>(width[l], T(cnn->scale), g_w + at, nullptr, nullptr, g_w + at, st);
+    if (rc) return rc;
+    at += width[l];
+  }
   return INC_OK;
 }
 
@@ -920,6 +1270,96 @@ int inc_conv1d_bwd_weight_noalloc(int dtype, int B, int Ci, int Co, int N, int K
   hipStream_t st = static_cast<hipStream_t>(stream);
   return dtype == INC_F32 ? inc::conv_wgrad<float>(B, Ci, Co, N, K, g, x, gw, gb, work, capacity, st)
                           : inc::conv_wgrad<double>(B, Ci, Co, N, K, g, x, gw, gb, work, capacity, st);
+}
+
+int inc_conv1d_rows_fwd_act(int dtype, int act, const inc_conv_rows* r, int Ci, int Co, int K, const void* x,
+                            const void* w, const void* bias, void* y, void* dy, void* stream) {
+  inc::clear_error();
+  int rc = inc::rows_check(dtype, act, r, Ci, Co, K);
+  if (rc || r->njobs == 0) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const inc::RowTiles rt = inc::row_tiles(r);
+  return dtype == INC_F32
+             ? inc::conv_fwd_launch<float, false>(0, Ci, Co, 0, K, x, w, bias, y, nullptr, act, dy, st, rt, r->njobs)
+             : inc::conv_fwd_launch<double, false>(0, Ci, Co, 0, K, x, w, bias, y, nullptr, act, dy, st, rt, r->njobs);
+}
+
+int inc_conv1d_rows_bwd_data_mul(int dtype, const inc_conv_rows* r, int Ci, int Co, int K, const void* g,
+                                 const void* w, const void* m, void* gx, void* stream) {
+  inc::clear_error();
+  int rc = inc::rows_check(dtype, INC_ACT_NONE, r, Ci, Co, K);
+  if (rc || r->njobs == 0) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const inc::RowTiles rt = inc::row_tiles(r);
+  return dtype == INC_F32
+             ? inc::conv_fwd_launch<float, true>(0, Co, Ci, 0, K, g, w, nullptr, gx, m, INC_ACT_NONE, nullptr, st, rt,
+                                                 r->njobs)
+             : inc::conv_fwd_launch<double, true>(0, Co, Ci, 0, K, g, w, nullptr, gx, m, INC_ACT_NONE, nullptr, st, rt,
+                                                  r->njobs);
+}
+
+int inc_conv1d_rows_bwd_weight(int dtype, const inc_conv_rows* r, int Ci, int Co, int K, const void* g, const void* x,
+                               void* gw, void* gb, void* work, int64_t work_elems, void* stream) {
+  inc::clear_error();
+  int rc = inc::rows_check(dtype, INC_ACT_NONE, r, Ci, Co, K);
+  if (rc) return rc;
+  INC_REQUIRE(r->njobs > 0, INC_EINVAL, "conv1d weight gradient of an empty set of rows");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const inc::WgradMode md{inc::row_tiles(r), r->njobs, false, true, nullptr};
+  return dtype == INC_F32 ? inc::conv_wgrad<float>(0, Ci, Co, 0, K, g, x, gw, gb, work, work_elems, st, md)
+                          : inc::conv_wgrad<double>(0, Ci, Co, 0, K, g, x, gw, gb, work, work_elems, st, md);
+}
+
+int inc_ragged_inc_workspace(const inc_ragged_desc* d, const inc_cnn_desc* cnn, int adjoint, int64_t* elems) {
+  inc::clear_error();
+  INC_REQUIRE(d && cnn && elems, INC_EINVAL, "null argument");
+  INC_REQUIRE(cnn->layers >= 2 && cnn->layers <= 8 && cnn->channels >= 1 && cnn->kernel >= 1, INC_EINVAL,
+              "bad network description");
+  *elems = inc::inc_layout(d, cnn, adjoint != 0).total;
+  return INC_OK;
+}
+
+int inc_ragged_inc_fwd(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* rows,
+                       const int32_t* active_tiles, const void* u0, int steps, void* traj, void* corr, void* stages,
+                       void* work, int64_t work_elems, void* stream) {
+  inc::clear_error();
+  int rc = inc::inc_check(d, cnn, rows);
+  if (rc) return rc;
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(u0 && traj && work, INC_EINVAL, "u0, traj and work are required");
+  INC_REQUIRE(work_elems >= inc::inc_layout(d, cnn, false).total, INC_EINVAL, "workspace too small");
+  if (d->count == 0) return INC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (d->dtype == INC_F32)
+    return inc::inc_fwd<float>(d, cnn, rows, active_tiles, static_cast<const float*>(u0), steps,
+                               static_cast<float*>(traj), static_cast<float*>(corr), static_cast<float*>(stages),
+                               static_cast<float*>(work), st);
+  return inc::inc_fwd<double>(d, cnn, rows, active_tiles, static_cast<const double*>(u0), steps,
+                              static_cast<double*>(traj), static_cast<double*>(corr), static_cast<double*>(stages),
+                              static_cast<double*>(work), st);
+}
+
+int inc_ragged_inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* rows,
+                       const int32_t* active_tiles, const void* u0, const void* traj, const void* stages,
+                       const void* g_traj, void* g_u0, void* g_w, int steps, void* work, int64_t work_elems,
+                       void* stream) {
+  inc::clear_error();
+  int rc = inc::inc_check(d, cnn, rows);
+  if (rc) return rc;
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(u0 && traj && stages && g_traj && g_u0 && g_w && work, INC_EINVAL, "missing buffer");
+  INC_REQUIRE(work_elems >= inc::inc_layout(d, cnn, true).total, INC_EINVAL, "workspace too small");
+  if (d->count == 0) return INC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (d->dtype == INC_F32)
+    return inc::inc_adj<float>(d, cnn, rows, active_tiles, static_cast<const float*>(u0),
+                               static_cast<const float*>(traj), static_cast<const float*>(stages),
+                               static_cast<const float*>(g_traj), static_cast<float*>(g_u0), static_cast<float*>(g_w),
+                               steps, static_cast<float*>(work), st);
+  return inc::inc_adj<double>(d, cnn, rows, active_tiles, static_cast<const double*>(u0),
+                              static_cast<const double*>(traj), static_cast<const double*>(stages),
+                              static_cast<const double*>(g_traj), static_cast<double*>(g_u0),
+                              static_cast<double*>(g_w), steps, static_cast<double*>(work), st);
 }
 
 }  // extern "C"

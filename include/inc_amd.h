@@ -371,6 +371,69 @@ int inc_conv1d_fwd_act(int dtype, int act, int B, int Ci, int Co, int n, int K, 
 int inc_conv1d_bwd_data_mul(int dtype, int B, int Ci, int Co, int n, int K, const void* g,
                             const void* w, const void* m, void* gx, void* stream);
 
+/*
+ * Ragged rows of the correction network's convolutions: row r (one trajectory
+ * of a ragged batch) has n[r] >= K/2 points, and its [C][n[r]] block of a
+ * C-channel array starts at element C * offset[r], so the packed [total]
+ * state of inc_ragged_desc is the C = 1 case and every layer of the network
+ * runs over all resolutions in one launch.  The work is a list of
+ * INC_CONV_ROW_TILE-position tiles (row, first position) in device memory; a
+ * call processes the first njobs tiles (a list ordered by horizon, longest
+ * first, leaves out the trajectories whose rollout has ended by taking a
+ * prefix).  Same arithmetic as inc_conv1d_fwd_act / _bwd_data_mul /
+ * _bwd_weight on each row.
+ */
+#define INC_CONV_ROW_TILE 128
+typedef struct inc_conv_rows {
+  int32_t count;         /* rows */
+  int32_t njobs;         /* tiles processed: the first njobs of tile */
+  const int32_t* tile;   /* [2 * njobs] (row, first position) pairs */
+  const int32_t* n;      /* [count] points per row */
+  const int64_t* offset; /* [count] first point of each row in the packed layout */
+} inc_conv_rows;
+
+int inc_conv1d_rows_fwd_act(int dtype, int act, const inc_conv_rows* r, int Ci, int Co, int K, const void* x,
+                            const void* w, const void* bias, void* y, void* dy, void* stream);
+int inc_conv1d_rows_bwd_data_mul(int dtype, const inc_conv_rows* r, int Ci, int Co, int K, const void* g,
+                                 const void* w, const void* m, void* gx, void* stream);
+int inc_conv1d_rows_bwd_weight(int dtype, const inc_conv_rows* r, int Ci, int Co, int K, const void* g, const void* x,
+                               void* gw, void* gb, void* work, int64_t work_elems, void* stream);
+
+/*
+ * INC training rollout of a ragged batch: simulate_rollout(model, solver, u0,
+ * steps, 'INC') (trainer/trainer_1d.py:154-183) over the uneven batch of
+ * inc_burgers_ragged_fwd (one solver per batch in trainer_1d.py:338-358),
+ * with a network of circular Conv1d layers 1 -> C -> ... -> C -> 1 and GELU
+ * between them (inc_cnn_desc: width C <= 64, K odd <= 9, 2..8 layers,
+ * s = scale * net(u); the reference's correction networks stack such
+ * convolutions, models/models_1d/resnet.py:5-52).  Step k runs the network
+ * as one ragged convolution per layer over the first active_tiles[k] tiles of
+ * rows (host array [steps]; NULL = rows->njobs every step), then one ragged
+ * solver step with s as its correction; trajectories past their horizon keep
+ * their state and get no correction.
+ *   traj   [steps, total]        states after each step
+ *   corr   [steps, total] or NULL corrections of each step (0 past a horizon)
+ *   stages [steps, ns, total]    stage states (ns = 3 SSP-RK3, 1 Euler), or
+ *                                NULL when no adjoint follows
+ * inc_ragged_inc_adj is its adjoint: from u0, traj, stages and the
+ * trajectory cotangent g_traj [steps, total] it writes g_u0 [total] and the
+ * network's parameter gradient summed over the batch, g_w [P] =
+ * [w0, b0, w1, b1, ...] in the weight and bias tensors' layouts.  The
+ * network is evaluated again at every step's input state; no activation is
+ * kept between the two calls.  work: workspace of work_elems elements of
+ * the state's type, at least what inc_ragged_inc_workspace reports for the
+ * forward (adjoint = 0) or the adjoint (adjoint = 1).  The weight gradients
+ * are summed over steps in a fixed order (deterministic).
+ */
+int inc_ragged_inc_workspace(const inc_ragged_desc* d, const inc_cnn_desc* cnn, int adjoint, int64_t* elems);
+int inc_ragged_inc_fwd(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* rows,
+                       const int32_t* active_tiles, const void* u0, int steps, void* traj, void* corr, void* stages,
+                       void* work, int64_t work_elems, void* stream);
+int inc_ragged_inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* rows,
+                       const int32_t* active_tiles, const void* u0, const void* traj, const void* stages,
+                       const void* g_traj, void* g_u0, void* g_w, int steps, void* work, int64_t work_elems,
+                       void* stream);
+
 #ifdef __cplusplus
 }
 #endif
