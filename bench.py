@@ -276,56 +276,260 @@ def extra_ks_inc():
             "trajectory_steps_per_s": b * h / t, "ms_per_train_step": t * 1e3}
 
 
-def extra_ragged_inc(plain_sample=16):
-    """Uneven Burgers batch: 2048 trajectories, N in {64..1024}, nu ~ LogU[0.003, 0.03], dt = min(CFL 0.4,
-    viscous), horizons ~ U{16..64}, 64-step INC rollout with a CNN(4x32,k5) correction, fwd+adj, autotuned block size;
-    against the plain per-trajectory path (one launch per trajectory per step) on a sample."""
+MFMA_F32_PEAK_TFS = 157.3  # dense fp32 matrix peak
+
+
+def ragged_workload(count=2048, steps=64, seed=1004):
+    """The uneven INC batch: `count` Burgers trajectories, N uniform in
+    {64, 128, 256, 512, 1024}, nu ~ LogU[0.003, 0.03], dt = min(CFL 0.4,
+    0.3 dx^2 / nu) per trajectory (inc_amd.batched.stable_dt), horizons ~ U{16..64}, SSP-RK3 + WENO5, fp32;
+    the correction network is CorrectionCNN(4 x 32, k = 5) with its output
+    scaled by 0.01; the loss is the MSE to the same batch at 1.2 nu over the
+    steps inside each trajectory's horizon."""
     import torch
-    import inc_amd
-    from inc_amd import autotune
-    from inc_amd.batched import stable_dt
-    from inc_amd.correction import CorrectionCNN
-    count, steps = 2048, 64
-    gen = torch.Generator().manual_seed(1004)
+    gen = torch.Generator().manual_seed(seed)
     choices = [64, 128, 256, 512, 1024]
     ns = [choices[int(i)] for i in torch.randint(0, 5, (count,), generator=gen)]
     nus = torch.exp(torch.empty(count, dtype=torch.float64).uniform_(math.log(0.003), math.log(0.03),
                                                                       generator=gen)).tolist()
+    from inc_amd.batched import stable_dt
     dts = [stable_dt(n, nu) for n, nu in zip(ns, nus)]
-    hz = torch.randint(16, 65, (count,), generator=gen).tolist()
+    hz = torch.randint(16, steps + 1, (count,), generator=gen).tolist()
+    return ns, nus, dts, hz
+
+
+def batched_ragged_inc(plain_sample=16, replays=20):
+    """The batched workload of the metric: one training step of the uneven INC
+    batch (ragged_workload) -- rollout with the network, masked MSE cotangent,
+    adjoint through solver and network -- captured once in a hipGraph and
+    replayed (3 warm-up replays, then the median of `replays`); the launch
+    shape is autotuned.  Beside it: the same step run eagerly, the
+    per-resolution-group path, and the plain per-trajectory path (one
+    BurgersSolverTorch launch per trajectory per step, network per step) on a
+    sample of trajectories."""
+    import torch
+    import inc_amd
+    from inc_amd import autotune
+    from inc_amd.correction import CorrectionCNN
+    steps = 64
+    ns, nus, dts, hz = ragged_workload(steps=steps)
+    count = len(ns)
     rb = inc_amd.RaggedBurgers(ns, nus, dts, horizon=hz)
     tuning = {}
     rb.team = autotune.tune_ragged(rb, record=tuning)
     dev = torch.device("cuda")
     u0 = rb.pack([sine_state(n, 3000 + i) for i, n in enumerate(ns)])
-    cnn = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(2)).to(dev)
+    cnn = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(2), scale=0.01).to(dev)
+    params = list(cnn.parameters())
     mask = rb.valid_mask(steps)
+    msum = float(mask.sum())
     with torch.no_grad():
         target = inc_amd.RaggedBurgers(ns, [1.2 * v for v in nus], dts, horizon=hz, team=rb.team).rollout(u0, steps)
 
     def train_step():
-        traj = rb.rollout(u0, steps, model=lambda v: 0.01 * cnn(v))
-        ((((traj - target) ** 2) * mask).sum() / mask.sum()).backward()
+        traj = rb.rollout(u0, steps, model=cnn)
+        torch.autograd.backward(traj, (2.0 / msum) * (traj - target) * mask)
+        return traj
+
+    def eager_step(fused=True):
+        traj = rb.rollout(u0, steps, model=cnn, fused=fused)
+        ((((traj - target) ** 2) * mask).sum() / msum).backward()
         cnn.zero_grad(set_to_none=True)
-    t = _timed(train_step, reps=2)
+
+    # capture: warm-up on a side stream, then one captured training step
+    for p in params:
+        p.grad = None
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            train_step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        traj_g = train_step()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ts = []
+    for r in range(3 + replays):
+        a.record()
+        graph.replay()
+        b.record()
+        torch.cuda.synchronize()
+        if r >= 3:
+            ts.append(a.elapsed_time(b) * 1e-3)
+    t_graph = sorted(ts)[len(ts) // 2]
+    finite = bool(torch.isfinite(traj_g).all()) and all(bool(torch.isfinite(p.grad).all()) for p in params)
+    del graph, traj_g
+    for p in params:
+        p.grad = None
+    t_eager = _timed(eager_step, reps=3)
+    t_group = _timed(lambda: eager_step(False), reps=2)
     traj_steps = sum(hz)
-    # plain: one BurgersSolverTorch launch per trajectory per step (same CNN), fwd+adj
+    # dominant kernel, measured live: the hidden 32 -> 32 layer's forward
+    # (inc_conv1d_rows_fwd_act over every trajectory, as at step 0)
+    import ctypes
+    import inc_amd._lib as L
+    rows, _ = rb._rows(1)
+    x = torch.randn(32 * rb.total, device=dev)
+    y = torch.empty_like(x)
+    w = params[2].detach().contiguous()
+    bias = params[3].detach().contiguous()
+    lib = L.load()
+
+    def hidden():
+        L.check(lib.inc_conv1d_rows_fwd_act(L.INC_F32, L.INC_ACT_GELU, ctypes.byref(rows), 32, 32, 5, L.ptr(x),
+                                            L.ptr(w), L.ptr(bias), L.ptr(y), None, L.stream()), "hidden layer")
+    hidden()
+    torch.cuda.synchronize()
+    kt = []
+    for _ in range(20):
+        a.record()
+        hidden()
+        b.record()
+        torch.cuda.synchronize()
+        kt.append(a.elapsed_time(b) * 1e-3)
+    t_kernel = sorted(kt)[len(kt) // 2]
+    flops = 2.0 * 32 * 32 * 5 * rb.total
+    # plain path on a sample: one solver launch per trajectory per step, the
+    # network per step, fwd + adj
     idx = list(range(0, count, count // plain_sample))[:plain_sample]
 
     def plain_step():
         for i in idx:
             sv = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=dts[i], resolution=ns[i], nu=nus[i],
                                                                 batch_size=1, time_scheme="SSPRK3", device="cuda"))
-            res, _ = inc_amd.simulate_rollout(lambda v: 0.01 * cnn(v), sv, sine_state(ns[i], 3000 + i)
-                                              .float().to(dev)[None], hz[i], "INC")
+            res, _ = inc_amd.simulate_rollout(cnn, sv, sine_state(ns[i], 3000 + i).float().to(dev)[None], hz[i],
+                                              "INC", fused=False)
             (res ** 2).mean().backward()
         cnn.zero_grad(set_to_none=True)
     tp = _timed(plain_step, reps=1)
     plain_rate = sum(hz[i] for i in idx) / tp
-    return {"workload": "ragged Burgers INC fwd+adj: 2048 traj, N in {64..1024}, horizons 16..64, CNN(4x32,k5)",
-            "trajectory_steps_per_s": traj_steps / t, "ms_per_train_step": t * 1e3, "team": rb.team,
-            "autotune_ms": tuning, "plain_trajectory_steps_per_s": plain_rate,
-            "batched_over_plain": traj_steps / t / plain_rate}
+    value = traj_steps / t_graph
+    solver_bytes = sum(7 * 4 * n * h for n, h in zip(ns, hz))
+    return {
+        "metric": "fwd+adjoint trajectory-steps/s, batched (hipGraph-captured, autotuned, uneven batch)",
+        "value": value, "unit": "trajectory-steps/s", "ms_per_step": t_graph * 1e3, "dtype": "f32",
+        "config": {"workload": f"uneven Burgers INC training step: {count} trajectories, N in {{64..1024}}, "
+                               "nu LogU[0.003,0.03], dt min(CFL 0.4, viscous), horizons U{16..64}, SSP-RK3/WENO5, "
+                               "CorrectionCNN 4x32 k5 (x0.01), masked MSE", "trajectory_steps": traj_steps,
+                   "team": rb.team, "autotune_ms": tuning},
+        "eager_ms_per_step": t_eager * 1e3, "per_group_path_ms_per_step": t_group * 1e3,
+        "plain_trajectory_steps_per_s": plain_rate, "batched_over_plain": value / plain_rate,
+        "finite": finite,
+        "roofline": {"bound": "mfma", "kernel": "conv_fwd_mfma_kernel (hidden 32->32 layer, all trajectories)",
+                     "achieved": flops / t_kernel / 1e12, "peak": MFMA_F32_PEAK_TFS, "unit": "TFLOP/s",
+                     "frac": flops / t_kernel / 1e12 / MFMA_F32_PEAK_TFS, "kernel_us": t_kernel * 1e6,
+                     "flops_per_launch": flops, "traffic": None},
+        "hbm": {"solver_algorithmic_bytes": solver_bytes,
+                "GBps_over_step": solver_bytes / t_graph / 1e9,
+                "frac": solver_bytes / t_graph / 1e9 / HBM_PEAK_GBS},
+    }
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_time(fn, threads, reps=1, warmup=False):
+    """Median wall time of fn() on `threads` host threads."""
+    import torch
+    torch.set_num_threads(threads)
+    if warmup:
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def cpu_baselines():
+    """The FP64 oracle (a PyTorch-CPU restatement of the reference's op
+    sequence) on a bounded sample of every configuration, forward + adjoint
+    (autograd through the oracle), on 1 host thread and on all of them.
+    cfg0 is the reference's own algorithm exactly (Euler, WENO5 with LF
+    splitting, dense-matrix diffusion) at its full size."""
+    import torch
+    from oracle import BurgersOracle, KSOracle, burgers_params, ks_params
+    from oracle import simulate_rollout as o_rollout
+    from inc_amd.correction import CorrectionCNN
+    cores = len(os.sched_getaffinity(0))
+    out = {"cpu_model": cpu_model(), "cores": cores}
+
+    def cfg0():
+        so = BurgersOracle(burgers_params(256, 0.005, nu=0.01, batch_size=1))
+        u = sine_state(256, 1000)[None].clone().requires_grad_(True)
+        cur, outs = u, []
+        for _ in range(200):
+            cur = so.step(cur)
+            outs.append(cur)
+        torch.stack(outs, 1).square().mean().backward()
+
+    def cfg0_fwd():
+        so = BurgersOracle(burgers_params(256, 0.005, nu=0.01, batch_size=1))
+        with torch.no_grad():
+            cur = sine_state(256, 1000)[None]
+            for _ in range(200):
+                cur = so.step(cur)
+
+    def ks_case(b, n, Lk, h, cnn=None, ckpt=None):
+        so = KSOracle(ks_params([Lk] * b, resolution=n, dt=0.25, time_scheme="ETDRK4"))
+        so.doublestep = False
+        u = initial_state(b, n, Lk, 1001, kmax=16 if cnn else 10).requires_grad_(True)
+        if cnn is None:
+            res, _ = o_rollout(None, so, u, h, None, L=so.domain_size)
+        else:
+            res, _ = o_rollout(lambda v, L_: cnn(v), so, u, h, "INC", L=so.domain_size)
+        res.square().mean().backward()
+
+    def burgers_inc(b, n, h, cnn):
+        so = BurgersOracle(burgers_params(n, 0.01, nu=0.01, batch_size=b, time_scheme="SSPRK3"))
+        u = torch.stack([sine_state(n, 2000 + i) for i in range(b)]).requires_grad_(True)
+        res, _ = o_rollout(cnn, so, u, h, "INC")
+        res.square().mean().backward()
+
+    cnn2 = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(2), backend="torch").double()
+    cnn3 = CorrectionCNN(6, 48, 7, generator=torch.Generator().manual_seed(3), backend="torch", scale=0.01).double()
+    cnn4 = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(2), backend="torch", scale=0.01).double()
+    ns, nus, dts, hz = ragged_workload()
+    sample = list(range(0, len(ns), len(ns) // 4))[:4]
+
+    def cfg4():
+        for i in sample:
+            so = BurgersOracle(burgers_params(ns[i], dts[i], nu=nus[i], batch_size=1, time_scheme="SSPRK3"))
+            u = sine_state(ns[i], 3000 + i)[None].clone().requires_grad_(True)
+            res, _ = o_rollout(cnn4, so, u, hz[i], "INC")
+            res.square().mean().backward()
+
+    cases = {
+        "cfg0": (cfg0, 200, "reference algorithm: Burgers Euler/WENO5/dense D2, B=1, N=256, 200 steps, fp64"),
+        "cfg0_forward_only": (cfg0_fwd, 200, "the same, forward only"),
+        "cfg1": (lambda: ks_case(256, 256, 64.0, 2), 256 * 2, "KS ETDRK4, B=256, N=256, 2 of 1000 steps"),
+        "cfg2": (lambda: burgers_inc(16, 128, 16, cnn2), 16 * 16,
+                 "Burgers INC + CNN(4x32,k5), 16 of 512 trajectories, N=128, 16 steps"),
+        "cfg3": (lambda: ks_case(2, 512, 100.0, 8, cnn=cnn3), 2 * 8,
+                 "KS INC + CNN(6x48,k7), 2 of 1024 trajectories, N=512, 8 of 64 steps"),
+        "cfg4": (cfg4, sum(hz[i] for i in sample),
+                 "uneven Burgers INC + CNN(4x32,k5), 4 of 2048 trajectories with their own N, nu, dt, horizon"),
+    }
+    first = True
+    for key, (fn, tsteps, what) in cases.items():
+        row = {"sample": what, "trajectory_steps": tsteps,
+               "unit": "trajectory-steps/s" + (" (fwd)" if key.endswith("only") else " (fwd+adj)")}
+        for threads in (1, cores):
+            row[f"value_{threads}_threads"] = tsteps / _cpu_time(fn, threads, warmup=first)
+            first = False
+        out[key] = row
+    torch.set_num_threads(cores)
+    return out
 
 
 def extra_ks_saturation(b=8192, h=200):
@@ -500,12 +704,19 @@ def main():
         out["plain_trajectory_steps_per_s"] = plain
         out["batched_over_plain"] = value / plain
     if not args.no_extras:
+        out["batched"] = batched_ragged_inc()
         out["other_workloads"] = {"ks_saturated": extra_ks_saturation(), "ks_mse_fused": extra_ks_mse(),
                                   "burgers_saturated": extra_burgers_saturation(), "burgers_inc": extra_burgers_inc(),
                                   "burgers_inc_eval": extra_burgers_inc_eval(),
-                                  "ks_inc_checkpointed": extra_ks_inc(), "ragged_burgers_inc": extra_ragged_inc()}
+                                  "ks_inc_checkpointed": extra_ks_inc()}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baselines"] = cpu_baselines()
+        if "batched" in out:
+            c4 = out["cpu_baselines"]["cfg4"]
+            out["batched"]["cpu_baseline"] = {
+                "value": c4[f"value_{out['cpu_baselines']['cores']}_threads"], "unit": c4["unit"],
+                "cores": out["cpu_baselines"]["cores"], "kind": "port", "sample": c4["sample"]}
     print(json.dumps(out))
 
 

@@ -35,7 +35,7 @@ def test_ragged_packing_groups_and_order():
 def test_stable_dt_is_min_of_advective_and_viscous():
     n, nu = 1024, 0.03
     dx = 2 * math.pi / n
-    assert stable_dt(n, nu) == min(0.4 * dx, 0.628 * dx * dx / nu) == 0.628 * dx * dx / nu
+    assert stable_dt(n, nu) == min(0.4 * dx, 0.3 * dx * dx / nu) == 0.3 * dx * dx / nu
     assert stable_dt(64, 0.003) == 0.4 * (2 * math.pi / 64)
 
 
