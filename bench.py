@@ -366,31 +366,7 @@ def batched_ragged_inc(plain_sample=16, replays=20):
     t_eager = _timed(eager_step, reps=3)
     t_group = _timed(lambda: eager_step(False), reps=2)
     traj_steps = sum(hz)
-    # dominant kernel, measured live: the hidden 32 -> 32 layer's forward
-    # (inc_conv1d_rows_fwd_act over every trajectory, as at step 0)
-    import ctypes
-    import inc_amd._lib as L
-    rows, _ = rb._rows(1)
-    x = torch.randn(32 * rb.total, device=dev)
-    y = torch.empty_like(x)
-    w = params[2].detach().contiguous()
-    bias = params[3].detach().contiguous()
-    lib = L.load()
-
-    def hidden():
-        L.check(lib.inc_conv1d_rows_fwd_act(L.INC_F32, L.INC_ACT_GELU, ctypes.byref(rows), 32, 32, 5, L.ptr(x),
-                                            L.ptr(w), L.ptr(bias), L.ptr(y), None, L.stream()), "hidden layer")
-    hidden()
-    torch.cuda.synchronize()
-    kt = []
-    for _ in range(20):
-        a.record()
-        hidden()
-        b.record()
-        torch.cuda.synchronize()
-        kt.append(a.elapsed_time(b) * 1e-3)
-    t_kernel = sorted(kt)[len(kt) // 2]
-    flops = 2.0 * 32 * 32 * 5 * rb.total
+    dom = ragged_dominant_kernel(rb, params[2], params[3])
     # plain path on a sample: one solver launch per trajectory per step, the
     # network per step, fwd + adj
     idx = list(range(0, count, count // plain_sample))[:plain_sample]
@@ -417,14 +393,67 @@ def batched_ragged_inc(plain_sample=16, replays=20):
         "eager_ms_per_step": t_eager * 1e3, "per_group_path_ms_per_step": t_group * 1e3,
         "plain_trajectory_steps_per_s": plain_rate, "batched_over_plain": value / plain_rate,
         "finite": finite,
-        "roofline": {"bound": "mfma", "kernel": "conv_fwd_mfma_kernel (hidden 32->32 layer, all trajectories)",
-                     "achieved": flops / t_kernel / 1e12, "peak": MFMA_F32_PEAK_TFS, "unit": "TFLOP/s",
-                     "frac": flops / t_kernel / 1e12 / MFMA_F32_PEAK_TFS, "kernel_us": t_kernel * 1e6,
-                     "flops_per_launch": flops, "traffic": None},
+        "roofline": dom,
         "hbm": {"solver_algorithmic_bytes": solver_bytes,
                 "GBps_over_step": solver_bytes / t_graph / 1e9,
                 "frac": solver_bytes / t_graph / 1e9 / HBM_PEAK_GBS},
     }
+
+
+def ragged_dominant_kernel(rb=None, w=None, bias=None, reps=20, calibrate_copy=False):
+    """The dominant kernel of the uneven INC training step, measured live:
+    the hidden 32 -> 32 layer's forward with GELU (conv_fwd_mfma_kernel through
+    inc_conv1d_rows_fwd_act) over every trajectory of the batch, as at step 0,
+    timed with HIP events on the launch stream (median of `reps`).  The
+    algorithmic flops are 2 * 32 * 32 * 5 per point of the batch; the traffic
+    is the copy-calibrated FETCH_SIZE + WRITE_SIZE of this launch recorded
+    under profiles/."""
+    import ctypes
+    import torch
+    import inc_amd
+    import inc_amd._lib as L
+    dev = torch.device("cuda")
+    if rb is None:
+        ns, nus, dts, hz = ragged_workload()
+        rb = inc_amd.RaggedBurgers(ns, nus, dts, horizon=hz, team=128)
+        gen = torch.Generator().manual_seed(5)
+        w = (torch.randn(32, 32, 5, generator=gen) * (2.0 / 160) ** 0.5).to(dev)
+        bias = torch.zeros(32, device=dev)
+    if calibrate_copy:
+        src = torch.randn(64 * 2 ** 20, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    rows, _ = rb._rows(1)
+    x = torch.randn(32 * rb.total, device=dev)
+    y = torch.empty_like(x)
+    w = w.detach().contiguous()
+    bias = bias.detach().contiguous()
+    lib = L.load()
+
+    def hidden():
+        L.check(lib.inc_conv1d_rows_fwd_act(L.INC_F32, L.INC_ACT_GELU, ctypes.byref(rows), 32, 32, 5, L.ptr(x),
+                                            L.ptr(w), L.ptr(bias), L.ptr(y), None, L.stream()), "hidden layer")
+    hidden()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kt = []
+    for _ in range(reps):
+        a.record()
+        hidden()
+        b.record()
+        torch.cuda.synchronize()
+        kt.append(a.elapsed_time(b) * 1e-3)
+    t_kernel = sorted(kt)[len(kt) // 2]
+    flops = 2.0 * 32 * 32 * 5 * rb.total
+    return {"bound": "mfma", "kernel": "conv_fwd_mfma_kernel<2,2> (hidden 32->32 layer + GELU, every trajectory)",
+            "achieved": flops / t_kernel / 1e12, "peak": MFMA_F32_PEAK_TFS, "unit": "TFLOP/s",
+            "frac": flops / t_kernel / 1e12 / MFMA_F32_PEAK_TFS, "kernel_us": t_kernel * 1e6,
+            "flops_per_launch": flops, "traffic": read_profile_value("ragged_inc_hidden_conv_traffic.md",
+                                                                    "traffic_bytes_per_launch"),
+            "algorithmic_bytes_per_launch": 2 * 32 * 4 * rb.total}
 
 
 def cpu_model():

@@ -293,6 +293,12 @@ class RaggedBurgers:
             params = [t for cv in model.convs for t in (cv.weight, cv.bias)]
             need = torch.is_grad_enabled() and (u0.requires_grad or any(p.requires_grad for p in params))
             return _RaggedIncRollout.apply(u0.reshape(-1), self, model, steps, need, *params)
+        if torch.cuda.is_current_stream_capturing():
+            # the per-group path under hipGraph capture is not validated (a
+            # capture of it, then with library convolutions, crashed in
+            # hipStreamEndCapture with no cause shown); the one-call path is
+            raise RuntimeError("RaggedBurgers.rollout with a per-group correction network cannot be captured; "
+                               "use a CorrectionCNN (the one-call path) inside a hipGraph")
         outs, u = [], u0
         for k in range(steps):
             s = torch.cat([model(v).reshape(-1) for v in self.group_views(u)])

@@ -56,32 +56,53 @@ def cos_modes(b, n, L, seed, kmax, sd=0.5):
     return u
 
 
-def row(name, tf, horizons, f64, grad=None, wgrad=None):
+def row(name, tf, horizons, f64, grad=None, wgrad=None, g64=None):
     hz = ", ".join(f"{h}: {e:.1e}" for h, e in horizons)
     g = "" if grad is None else f"{grad:.1e}"
     if wgrad is not None:
         g += f" (weights: {wgrad:.1e})"
+    if g64 is not None:
+        g += f"; fp64 {g64:.1e}"
     return f"| {name} | {tf:.1e} | {hz} | {f64:.1e} | {g} |"
 
 
+def linear_weights(shape, seed):
+    """Fixed weights w of the functional sum(w * trajectory): its cotangent is
+    w in every precision, so the gradient error measures the adjoint itself
+    (an MSE to a nearby target would measure the target's rounding)."""
+    return torch.randn(shape, generator=torch.Generator().manual_seed(seed), dtype=torch.float64)
+
+
 def burgers_plain():
-    """Burgers, no correction, N=256, nu=0.01, dt=0.005, 200 steps (SSP-RK3)."""
+    """Burgers, no correction, N=256, nu=0.01, dt=0.005, 200 steps (SSP-RK3); gradient of a fixed
+    linear functional of the trajectory with respect to u0."""
     n, H = 256, 200
     u0 = sine_states(1, n, 1000)
+    wts = linear_weights((1, H, n), 7)
     so = BurgersOracle(burgers_params(n, 0.005, nu=0.01, time_scheme="SSPRK3"))
-    ref = [u0]
+    ug = u0.clone().requires_grad_(True)
+    ref = [ug]
     for _ in range(H):
         ref.append(so.step(ref[-1]))
-    out = {}
+    (torch.stack(ref[1:], 1) * wts).sum().backward()
+    gref = ug.grad
+    ref = [r.detach() for r in ref]
+    out, grads = {}, {}
     for dt_ in (torch.float32, torch.float64):
         sv = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=0.005, resolution=n, nu=0.01, time_scheme="SSPRK3",
                                                             device="cuda"))
-        out[dt_] = sv.rollout(u0.to(DEV, dt_), H).cpu()
+        u = u0.to(DEV, dt_).requires_grad_(True)
+        traj = sv.rollout(u, H)
+        (traj * wts.to(DEV, dt_)).sum().backward()
+        out[dt_] = traj.detach().cpu()
+        grads[dt_] = u.grad.cpu()
     sv = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=0.005, resolution=n, nu=0.01, time_scheme="SSPRK3",
                                                         device="cuda"))
     tf = max(rel(sv.step(ref[k].float().to(DEV)), ref[k + 1]) for k in range(0, H, 10))
     hz = [(h, rel(out[torch.float32][:, h - 1], ref[h])) for h in (1, 20, 200)]
-    return row("Burgers plain N=256, 200 steps", tf, hz, rel(out[torch.float64][:, -1], ref[H]))
+    return row("Burgers plain N=256, 200 steps (gradient: linear functional)", tf, hz,
+               rel(out[torch.float64][:, -1], ref[H]), rel(grads[torch.float32], gref),
+               g64=rel(grads[torch.float64], gref))
 
 
 def ks_plain():
@@ -209,11 +230,106 @@ def ks_inc():
                rel(res[torch.float32][1], gref))
 
 
+def ks_plain_gradient():
+    """KS ETDRK4 gradient of a fixed linear functional of the trajectory with respect to u0, at
+    horizons 10 and 100 (the chaotic dynamics amplify any perturbation over longer horizons)."""
+    n, b, L = 256, 8, 64.0
+    u0 = cos_modes(b, n, L, 1001, 10)
+    errs = []
+    for H in (10, 100):
+        wts = linear_weights((b, H, n), 8)
+        so = KSOracle(ks_params([L] * b, resolution=n, dt=0.25, time_scheme="ETDRK4"))
+        so.doublestep = False
+        ug = u0.clone().requires_grad_(True)
+        cur, outs = ug, []
+        for _ in range(H):
+            cur = so.step(cur, so.domain_size)
+            outs.append(cur)
+        (torch.stack(outs, 1) * wts).sum().backward()
+        row_e = []
+        for dt_ in (torch.float32, torch.float64):
+            sv = inc_amd.KSSolverTorch(inc_amd.KSSimParams([L], resolution=n, dt=0.25, time_scheme="ETDRK4",
+                                                           device="cuda"))
+            sv.doublestep = False
+            u = u0.to(DEV, dt_).requires_grad_(True)
+            traj = sv.rollout(u, H, torch.tensor([L], dtype=torch.float64, device=DEV))
+            (traj * wts.to(DEV, dt_)).sum().backward()
+            row_e.append(rel(u.grad, ug.grad))
+        errs.append((H, row_e[0], row_e[1]))
+    txt = "; ".join(f"horizon {H}: fp32 {e32:.1e}, fp64 {e64:.1e}" for H, e32, e64 in errs)
+    return f"| KS ETDRK4 N=256 L=64 gradient of u0 (linear functional) | | | | {txt} |"
+
+
+def ragged_inc():
+    """The uneven INC batch: 12 trajectories drawn from it (N from 64 to 1024, their own nu, dt and
+    horizon), 64 steps, CNN(4x32,k5) x 0.01, the one-call path (inc_ragged_inc_fwd / _adj); oracle per
+    trajectory.  Gradient of a fixed linear functional of the trajectory: u0 and, in brackets, the
+    weights (worst parameter tensor)."""
+    sys.path.insert(0, ROOT)
+    import bench
+    ns, nus, dts, hz = bench.ragged_workload()
+    pick = list(range(0, len(ns), len(ns) // 12))[:12]
+    ns, nus, dts, hz = ([v[i] for i in pick] for v in (ns, nus, dts, hz))
+    steps = 64
+    cnn = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(2), backend="torch", scale=0.01).double()
+    u0s = [sine_states(1, n, 3000 + i)[0] for i, n in enumerate(ns)]
+    wts = [linear_weights((steps, n), 100 + i) for i, n in enumerate(ns)]
+    refs, grefs, tf = [], [], 0.0
+    for i, n in enumerate(ns):
+        so = BurgersOracle(burgers_params(n, dts[i], nu=nus[i], batch_size=1, time_scheme="SSPRK3"))
+        ug = u0s[i][None].clone().requires_grad_(True)
+        cur, outs = ug, []
+        for k in range(steps):
+            if k < hz[i]:
+                so.correction_s = cnn(cur)
+                cur = so.step(cur)
+            outs.append(cur)
+        traj = torch.cat(outs, 0)
+        (traj * wts[i]).sum().backward()
+        refs.append(traj.detach())
+        grefs.append(ug.grad[0])
+    wref = [p.grad.clone() for p in cnn.parameters()]
+    res = {}
+    for dt_ in (torch.float32, torch.float64):
+        rb = inc_amd.RaggedBurgers(ns, nus, dts, horizon=hz, dtype=dt_)
+        net = CorrectionCNN(4, 32, 5, scale=0.01).to(DEV, dt_)
+        net.load_state_dict(cnn.state_dict())
+        u = rb.pack(u0s).requires_grad_(True)
+        traj = rb.rollout(u, steps, model=net)
+        w_packed = torch.stack([rb.pack([w[k] for w in wts]) for k in range(steps)])
+        (traj * w_packed.to(dt_)).sum().backward()
+        per = rb.unpack(traj.detach().cpu())
+        gu = rb.unpack(u.grad.cpu())
+        res[dt_] = (per, gu, max(rel(p.grad.cpu(), w) for p, w in zip(net.parameters(), wref)))
+        if dt_ == torch.float32:
+            # teacher forcing: one float32 ragged step from the oracle states with the oracle's correction
+            for k in (0, 15, 31):
+                prev = [u0s[i] if k == 0 else refs[i][k - 1] for i in range(len(ns))]
+                with torch.no_grad():
+                    s = [cnn(p[None])[0] for p in prev]
+                    nxt = rb.advance(rb.pack(prev), rb.pack(s), steps=1, step0=k)
+                for i, v in enumerate(rb.unpack(nxt.cpu())):
+                    if k < hz[i]:
+                        tf = max(tf, rel(v, refs[i][k]))
+    def stacked(per):
+        return torch.cat([p.reshape(-1) for p in per])
+    hzs = []
+    for h in (1, 16, 64):
+        a = torch.cat([res[torch.float32][0][i][h - 1] for i in range(len(ns))])
+        b = torch.cat([refs[i][h - 1] for i in range(len(ns))])
+        hzs.append((h, rel(a, b)))
+    f64 = rel(stacked(res[torch.float64][0]), stacked(refs))
+    g32 = rel(stacked(res[torch.float32][1]), stacked(grefs))
+    g64 = rel(stacked(res[torch.float64][1]), stacked(grefs))
+    return row("uneven INC batch: 12 of its trajectories, 64 steps (fwd+adj, one-call path)", tf, hzs, f64, g32,
+               res[torch.float32][2], g64=g64)
+
+
 def main():
     print("| workload | fp32 one-step (teacher-forced) | fp32 free-running, by horizon | fp64 kernels "
           "at the last horizon | fp32 gradient |")
     print("|---|---|---|---|---|")
-    for fn in (burgers_plain, ks_plain, burgers_inc, ks_inc):
+    for fn in (burgers_plain, ks_plain, ks_plain_gradient, burgers_inc, ks_inc, ragged_inc):
         print(fn(), flush=True)
 
 
