@@ -206,9 +206,38 @@ def extra_burgers_inc():
         cnn.zero_grad(set_to_none=True)
     t = _timed(lambda: train_step(True))
     tp = _timed(lambda: train_step(False))
+    # the fused kernels, timed with HIP events on the launch stream: the
+    # forward (burgers_inc_fwd_kernel) and the adjoint (burgers_inc_adj_kernel)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    tf = ta = float("inf")
+    for _ in range(4):
+        sv = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=0.01, resolution=n, nu=0.01, batch_size=b,
+                                                            time_scheme="SSPRK3", device="cuda"))
+        u = u0.clone().requires_grad_(True)
+        ev[0].record()
+        res, _ = inc_amd.simulate_rollout(cnn, sv, u, h, "INC")
+        ev[1].record()
+        g = (2.0 / res.numel()) * (res.detach() - target)
+        ev[2].record()
+        torch.autograd.backward(res, g)
+        ev[3].record()
+        torch.cuda.synchronize()
+        tf = min(tf, ev[0].elapsed_time(ev[1]) * 1e-3)
+        ta = min(ta, ev[2].elapsed_time(ev[3]) * 1e-3)
+        cnn.zero_grad(set_to_none=True)
+    pts = b * n * h
+    # matrix-core work of the adjoint: the hidden 32 -> 32 layers evaluated again, their input
+    # gradients and their weight gradients (3 x 2 x 32 x 32 x 5 x 2 flops per point-step)
+    mflops = 3 * 2 * (2 * 32 * 32 * 5) * pts
     return {"workload": "Burgers INC fwd+adj B=512 N=128 16 steps SSP-RK3/WENO5 + CNN(4x32,k5)",
             "trajectory_steps_per_s": b * h / t, "ms_per_train_step": t * 1e3,
-            "per_step_path_ms_per_train_step": tp * 1e3, "fused_over_per_step": tp / t}
+            "per_step_path_ms_per_train_step": tp * 1e3, "fused_over_per_step": tp / t,
+            "fwd_kernel_ms": tf * 1e3, "adj_kernel_ms": ta * 1e3,
+            "roofline": {"bound": "mfma", "kernel": "burgers_inc_adj_kernel<32,5,2,2>",
+                         "achieved": mflops / ta / 1e12, "peak": MFMA_F32_PEAK_TFS, "unit": "TFLOP/s",
+                         "frac": mflops / ta / 1e12 / MFMA_F32_PEAK_TFS, "flops_per_launch": mflops,
+                         "hbm_algorithmic_bytes": 4 * 4 * n * b * h,
+                         "hbm_frac": 4 * 4 * n * b * h / ta / 1e9 / HBM_PEAK_GBS}}
 
 
 def extra_burgers_inc_eval(b=512, n=128, h=1000):
@@ -272,8 +301,35 @@ def extra_ks_inc():
         ((res - target) ** 2).mean().backward()
         cnn.zero_grad(set_to_none=True)
     t = _timed(train_step, reps=2)
+    # dominant kernel, measured live: the hidden 48 -> 48 k7 layer with GELU over the batch
+    import ctypes
+    import inc_amd._lib as L
+    x = torch.randn(b, 48, n, device=dev)
+    y, dy = torch.empty_like(x), torch.empty_like(x)
+    w = cnn.convs[1].weight.detach().contiguous()
+    bias = cnn.convs[1].bias.detach().contiguous()
+    lib = L.load()
+
+    def hidden():
+        L.check(lib.inc_conv1d_fwd_act(L.INC_F32, L.INC_ACT_GELU, b, 48, 48, n, 7, L.ptr(x), L.ptr(w), L.ptr(bias),
+                                       L.ptr(y), L.ptr(dy), L.stream()), "hidden layer")
+    hidden()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    kt = []
+    for _ in range(10):
+        ev[0].record()
+        hidden()
+        ev[1].record()
+        torch.cuda.synchronize()
+        kt.append(ev[0].elapsed_time(ev[1]) * 1e-3)
+    tk = sorted(kt)[len(kt) // 2]
+    flops = 2.0 * 48 * 48 * 7 * b * n
     return {"workload": "KS INC fwd+adj B=1024 N=512 L=100 64 steps ETDRK4, checkpoint every 8, CNN(6x48,k7)",
-            "trajectory_steps_per_s": b * h / t, "ms_per_train_step": t * 1e3}
+            "trajectory_steps_per_s": b * h / t, "ms_per_train_step": t * 1e3,
+            "roofline": {"bound": "mfma", "kernel": "conv_fwd_mfma_kernel (hidden 48->48 k7 + GELU, B=1024, N=512)",
+                         "achieved": flops / tk / 1e12, "peak": MFMA_F32_PEAK_TFS, "unit": "TFLOP/s",
+                         "frac": flops / tk / 1e12 / MFMA_F32_PEAK_TFS, "kernel_us": tk * 1e6,
+                         "flops_per_launch": flops}}
 
 
 MFMA_F32_PEAK_TFS = 157.3  # dense fp32 matrix peak
@@ -627,6 +683,32 @@ def extra_ks_mse(b=B, h=H):
             "unfused_trajectory_steps_per_s": b * h / tu}
 
 
+def latency_bound(solver, Ld, u0, t_headline, b=2):
+    """The chain of dependent steps alone: the same rollout and adjoint for a
+    single FFT pair (B = 2), whose time is set by step latency, not by how
+    many trajectories share the chip.  ceiling_ms = that time, the bound the
+    headline workload cannot beat with this algorithm however little HBM it
+    moves; frac = ceiling / the headline kernels' time."""
+    import torch
+    u = u0[:b].clone()
+    cot = torch.randn(b, H, N, device=u0.device) * 1e-3
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    best = float("inf"), float("inf")
+    for _ in range(4):
+        uu = u.clone().requires_grad_(True)
+        ev[0].record()
+        traj = solver.rollout(uu, H, Ld)
+        ev[1].record()
+        torch.autograd.backward(traj, cot)
+        ev[2].record()
+        torch.cuda.synchronize()
+        best = (min(best[0], ev[0].elapsed_time(ev[1]) * 1e-3), min(best[1], ev[1].elapsed_time(ev[2]) * 1e-3))
+    ceiling = best[0] + best[1]
+    return {"what": "fwd+adj rollout of one FFT pair (B=2), same N, L, dt, steps", "ceiling_ms": ceiling * 1e3,
+            "fwd_us_per_step": best[0] / H * 1e6, "adj_us_per_step": best[1] / H * 1e6,
+            "us_per_fft": (best[0] + best[1]) / H / 16 * 1e6, "frac": ceiling / t_headline}
+
+
 def plain_path(solver, u0, Ld, ntraj=8, h=50):
     """One launch per trajectory per step (no batching), fwd + adjoint."""
     import torch
@@ -728,6 +810,7 @@ def main():
         out["compute"] = {"bound": "valu", "kernel": "ks_adj_kernel", "achieved": adj_flops / t_adj / 1e12,
                           "peak": VALU_PEAK_TFS, "unit": "TFLOP/s", "frac": adj_flops / t_adj / 1e12 / VALU_PEAK_TFS,
                           "flops_per_launch": adj_flops}
+    out["latency_bound"] = latency_bound(solver, Ld, u0, t_fwd + t_adj)
     if not args.no_plain:
         plain = plain_path(solver, u0, Ld)
         out["plain_trajectory_steps_per_s"] = plain
