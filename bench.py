@@ -206,6 +206,14 @@ def extra_burgers_inc():
         cnn.zero_grad(set_to_none=True)
     t = _timed(lambda: train_step(True))
     tp = _timed(lambda: train_step(False))
+
+    def train_step_mse():
+        sv = inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=0.01, resolution=n, nu=0.01, batch_size=b,
+                                                            time_scheme="SSPRK3", device="cuda"))
+        loss, _, _ = inc_amd.simulate_rollout_mse(sv, u0, target, model=cnn)
+        loss.backward()
+        cnn.zero_grad(set_to_none=True)
+    tm = _timed(train_step_mse)
     # the fused kernels, timed with HIP events on the launch stream: the
     # forward (burgers_inc_fwd_kernel) and the adjoint (burgers_inc_adj_kernel)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -232,6 +240,7 @@ def extra_burgers_inc():
     return {"workload": "Burgers INC fwd+adj B=512 N=128 16 steps SSP-RK3/WENO5 + CNN(4x32,k5)",
             "trajectory_steps_per_s": b * h / t, "ms_per_train_step": t * 1e3,
             "per_step_path_ms_per_train_step": tp * 1e3, "fused_over_per_step": tp / t,
+            "loss_fused_ms_per_train_step": tm * 1e3,
             "fwd_kernel_ms": tf * 1e3, "adj_kernel_ms": ta * 1e3,
             "roofline": {"bound": "mfma", "kernel": "burgers_inc_adj_kernel<32,5,2,2>",
                          "achieved": mflops / ta / 1e12, "peak": MFMA_F32_PEAK_TFS, "unit": "TFLOP/s",

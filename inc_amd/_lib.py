@@ -100,6 +100,10 @@ SIGNATURES = {
                                                  ctypes.c_int, _vp, _vp, _vp, _vp]),
     "inc_burgers_inc_train_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.POINTER(CnnDesc), _vp, _vp, _vp,
                                                  _vp, _vp, ctypes.c_int, _vp]),
+    "inc_burgers_inc_train_fwd_mse": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.POINTER(CnnDesc), _vp, _vp,
+                                                     ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "inc_burgers_inc_train_adj_mse": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.POINTER(CnnDesc), _vp, _vp,
+                                                     _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "inc_burgers_adaptive_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_double, ctypes.c_int, _vp, _vp,
                                                 _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     "inc_burgers_adaptive_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp, _vp,

@@ -907,6 +907,7 @@ __global__ void __launch_bounds__(kBlock) burgers_inc_fwd_kernel(BArgs<T> a, Cnn
   }
   const size_t row = size_t(c.b) * n;
   T u0[PPT], uc[PPT], sv[PPT], xv[PPT], R[PPT];
+  T lsum = T(0);  // squared errors to the target (fused MSE)
 #pragma unroll
   for (int p = 0; p < PPT; ++p) {
     const int i = c.i0 + p;
@@ -958,11 +959,26 @@ __global__ void __launch_bounds__(kBlock) burgers_inc_fwd_kernel(BArgs<T> a, Cnn
       for (int p = 0; p < PPT; ++p)
         if (c.i0 + p < n) a.traj[at + c.i0 + p] = u0[p];
     }
+    if (a.target && own) {
+#pragma unroll
+      for (int p = 0; p < PPT; ++p)
+        if (c.i0 + p < n) {
+          const T dd = u0[p] - a.target[at + c.i0 + p];
+          lsum += dd * dd;
+        }
+    }
   }
   if (own) {
 #pragma unroll
     for (int p = 0; p < PPT; ++p)
       if (c.i0 + p < n) a.out[row + c.i0 + p] = u0[p];
+  }
+  if (a.target) {
+    // one partial per wavefront of the block, unused slots zero
+    lsum = wave_sum(lsum, kWave);
+    if (c.lt % kWave == 0) a.loss_parts[size_t(c.b) * kLossSlots + c.lt / kWave] = lsum;
+    if (c.lt == 0)
+      for (int k = blockDim.x / kWave; k < kLossSlots; ++k) a.loss_parts[size_t(c.b) * kLossSlots + k] = T(0);
   }
 }
 
@@ -1207,8 +1223,14 @@ __global__ void __launch_bounds__(kTrainBlock) burgers_inc_adj_kernel(BArgs<floa
   const float dt = a.dt;
   const int parts = INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE;
   float lam, gs, st[3], cb, r, b1, b2;
+  // cotangent of output `step`: g_traj, plus *loss_scale * (traj - target)
+  // with a target (the fused MSE)
+  const float lsc = a.target ? *a.loss_scale : 0.f;
   auto gload = [&](int step) {
-    return (own && a.g_traj) ? a.g_traj[(size_t(c.b) * a.steps + step) * n + tid] : 0.f;
+    const size_t at = (size_t(c.b) * a.steps + step) * n + tid;
+    float g = (own && a.g_traj) ? a.g_traj[at] : 0.f;
+    if (own && a.target) g += lsc * (a.traj[at] - a.target[at]);
+    return g;
   };
   lam = gload(a.steps - 1);
   for (int step = a.steps - 1; step >= 0; --step) {
@@ -2200,7 +2222,8 @@ void* inc_fwd_mfma_kernel(int ppt, int c, int k, int pw) {
 
 template <typename T>
 int inc_fwd_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void* u, void* out, int steps,
-                   void* traj, void* corr, void* stream, void* stages = nullptr) {
+                   void* traj, void* corr, void* stream, void* stages = nullptr, const void* target = nullptr,
+                   void* loss_parts = nullptr) {
   BArgs<T> a;
   int rc = fill_args<T>(d, a);
   if (rc) return rc;
@@ -2270,6 +2293,8 @@ int inc_fwd_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void
   a.out = static_cast<T*>(out);
   a.traj = static_cast<T*>(traj);
   a.stages = static_cast<T*>(stages);
+  a.target = static_cast<const T*>(target);
+  a.loss_parts = static_cast<T*>(loss_parts);
   if (a.batch == 0) return INC_OK;
   void* fn;
   if constexpr (kPptMax == 2) {
@@ -2314,7 +2339,8 @@ size_t inc_adj_lds(int n, int C, int K, int nh) {
 }
 
 int inc_train_adj_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, const void* stages, const void* g_traj,
-                         const void* g_corr, void* g_u, void* g_w, int steps, void* stream) {
+                         const void* g_corr, void* g_u, void* g_w, int steps, void* stream,
+                         const void* traj = nullptr, const void* target = nullptr, const void* scale = nullptr) {
   INC_REQUIRE(d->dtype == INC_F32, INC_EUNSUPPORTED, "the training adjoint with the network inside runs float32");
   BArgs<float> a;
   int rc = fill_args<float>(d, a);
@@ -2345,6 +2371,9 @@ int inc_train_adj_launch(const inc_burgers_desc* d, const inc_cnn_desc* cd, cons
   a.stages = static_cast<float*>(const_cast<void*>(stages));
   a.g_traj = static_cast<const float*>(g_traj);
   a.out = static_cast<float*>(g_u);
+  a.traj = static_cast<float*>(const_cast<void*>(traj));
+  a.target = static_cast<const float*>(target);
+  a.loss_scale = static_cast<const float*>(scale);
   if (a.batch == 0) return INC_OK;
   void* fn = cd->kernel == 5 ? inc_adj_kernel_k<5>(nh, pw) : inc_adj_kernel_k<7>(nh, pw);
   const float* gc = static_cast<const float*>(g_corr);
@@ -2477,6 +2506,33 @@ int inc_burgers_inc_train_adj(const inc_burgers_desc* d, const inc_cnn_desc* cnn
   INC_REQUIRE(d != nullptr && cnn != nullptr, INC_EINVAL, "null descriptor");
   INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
   return inc::inc_train_adj_launch(d, cnn, stages, g_traj, g_corr, g_u, g_w, steps, stream);
+}
+
+int inc_burgers_inc_train_fwd_mse(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* u, void* u_out,
+                                  int steps, void* traj, void* corr, void* stages, const void* target,
+                                  void* loss_parts, void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(d != nullptr && cnn != nullptr, INC_EINVAL, "null descriptor");
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(stages != nullptr && traj != nullptr, INC_EINVAL, "the training forward needs stages and traj");
+  INC_REQUIRE(target != nullptr && loss_parts != nullptr, INC_EINVAL, "the MSE forward needs target and loss_parts");
+  if (d->dtype == INC_F32)
+    return inc::inc_fwd_launch<float>(d, cnn, u, u_out, steps, traj, corr, stream, stages, target, loss_parts);
+  if (d->dtype == INC_F64)
+    return inc::inc_fwd_launch<double>(d, cnn, u, u_out, steps, traj, corr, stream, stages, target, loss_parts);
+  inc::set_error("unsupported dtype %d", d->dtype);
+  return INC_EINVAL;
+}
+
+int inc_burgers_inc_train_adj_mse(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* stages,
+                                  const void* traj, const void* target, const void* scale, const void* g_traj,
+                                  const void* g_corr, void* g_u, void* g_w, int steps, void* stream) {
+  inc::clear_error();
+  INC_REQUIRE(d != nullptr && cnn != nullptr, INC_EINVAL, "null descriptor");
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(traj != nullptr && target != nullptr && scale != nullptr, INC_EINVAL,
+              "the MSE adjoint needs traj, target and scale");
+  return inc::inc_train_adj_launch(d, cnn, stages, g_traj, g_corr, g_u, g_w, steps, stream, traj, target, scale);
 }
 
 int inc_burgers_adaptive_fwd(const inc_burgers_desc* d, double cfl, int max_sub, const void* u, const void* s,

@@ -121,13 +121,21 @@ def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, c
     return res, (torch.cat(corrections, 1) if correction_term == "INC" else None)
 
 
-def simulate_rollout_mse(solver, u_initial, target, L=None):
-    """``simulate_rollout(None, solver, u_initial, target.shape[1], None, L)``
+def simulate_rollout_mse(solver, u_initial, target, L=None, model=None):
+    """``simulate_rollout(model, solver, u_initial, target.shape[1], ...)``
     followed by ``torch.nn.MSELoss()(results, target)`` (trainer_1d.py:154-183
-    and :723), with the loss computed in the forward kernel and its gradient
-    seeded inside the adjoint kernel.  Returns (loss, results).  Needs a
-    fused rollout: a fixed time step (Burgers) or no correction_u (KS)."""
+    and :716-723), with the loss computed in the forward kernel and its
+    gradient seeded inside the adjoint kernel.  Without ``model`` (no
+    correction) returns (loss, results) and needs a fused rollout: a fixed
+    time step (Burgers) or no correction_u (KS).  With a CorrectionCNN
+    ``model`` (the trainer's 'INC' rollout) the network runs inside the
+    Burgers rollout and adjoint kernels too, and (loss, results,
+    corrections) is returned."""
     solver.correction_s = None
+    if model is not None:
+        if not isinstance(solver, BurgersSolverTorch) or not isinstance(model, CorrectionCNN):
+            raise ValueError("the fused INC loss needs a BurgersSolverTorch and a CorrectionCNN")
+        return solver.rollout_inc_mse(model, u_initial, target)
     if isinstance(solver, KSSolverTorch):
         return solver.rollout_mse(u_initial, target, L)
     if isinstance(solver, BurgersSolverTorch):

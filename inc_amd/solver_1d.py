@@ -254,6 +254,58 @@ class _BurgersIncRollout(torch.autograd.Function):
         return (g_u, None, None, None, None, *grads)
 
 
+class _BurgersIncRolloutMSE(torch.autograd.Function):
+    """The INC training rollout of _BurgersIncRollout with the trainer's MSE
+    to a target trajectory fused into its kernels (inc_burgers_inc_train_fwd_mse
+    / _adj_mse, trainer_1d.py:716-723): returns (loss, results, corrections);
+    the backward seeds the adjoint with 2 (u - target) / M inside the kernel."""
+
+    @staticmethod
+    def forward(ctx, u, target, solver, model, steps, t, *params):
+        ctx.set_materialize_grads(False)
+        cdesc, keep_w = model.fused_desc(u.dtype)
+        u = u.contiguous()
+        b, n = u.shape
+        tgt = _mse_target(target, u, steps)
+        desc, keep = solver._desc(u, solver.dt, t)
+        out = torch.empty_like(u)
+        traj = u.new_empty(b, steps, n)
+        corr = u.new_empty(b, steps, n)
+        ns = 3 if solver.time_scheme == "SSPRK3" else 1
+        stages = u.new_empty(b, steps, ns, n)
+        parts = u.new_empty(b, 4)
+        rc = L.load().inc_burgers_inc_train_fwd_mse(ctypes.byref(desc), ctypes.byref(cdesc), L.ptr(u), L.ptr(out),
+                                                    steps, L.ptr(traj), L.ptr(corr), L.ptr(stages), L.ptr(tgt),
+                                                    L.ptr(parts), L.stream())
+        L.check(rc, "BurgersSolverTorch.rollout_inc_mse")
+        ctx.save_for_backward(stages, traj, tgt)
+        ctx.desc, ctx.keep, ctx.cdesc, ctx.keep_w = desc, keep, cdesc, keep_w
+        ctx.steps, ctx.shape, ctx.param_shapes = steps, (b, n), [p.shape for p in params]
+        return parts.sum() / traj.numel(), traj, corr
+
+    @staticmethod
+    def backward(ctx, g_loss, g_traj, g_corr):
+        stages, traj, tgt = ctx.saved_tensors
+        b, n = ctx.shape
+        if g_loss is None:
+            g_loss = traj.new_zeros(())
+        m = traj.numel()
+        scale = (g_loss.detach().to(traj.dtype) * (2.0 / m)).reshape(1).contiguous()
+        g_traj = g_traj.contiguous() if g_traj is not None else None
+        g_corr = g_corr.contiguous() if g_corr is not None else None
+        g_target = -scale * (traj - tgt) if ctx.needs_input_grad[1] else None
+        g_u = stages.new_empty(b, n)
+        sizes = [math.prod(sh) for sh in ctx.param_shapes]
+        g_w = stages.new_empty(b, sum(sizes))
+        rc = L.load().inc_burgers_inc_train_adj_mse(ctypes.byref(ctx.desc), ctypes.byref(ctx.cdesc), L.ptr(stages),
+                                                    L.ptr(traj), L.ptr(tgt), L.ptr(scale), L.ptr(g_traj),
+                                                    L.ptr(g_corr), L.ptr(g_u), L.ptr(g_w), ctx.steps, L.stream())
+        L.check(rc, "BurgersSolverTorch.rollout_inc_mse backward")
+        g_rows = g_w.sum(0).split(sizes)
+        grads = [g.view(sh) for g, sh in zip(g_rows, ctx.param_shapes)]
+        return (g_u, g_target, None, None, None, None, *grads)
+
+
 def inc_train_shape_supported(channels, kernel, layers, n):
     """The shapes inc_burgers_inc_train_adj runs: width 32, kernel size 5 or
     7, 2..4 layers, 8 <= n <= 128, and the block's LDS (hidden weights,
@@ -648,6 +700,26 @@ class BurgersSolverTorch:
             self.t += self.dt
         self.correction_s = corr[:, -1] if mstep > 0 else None
         return traj, corr
+
+    def rollout_inc_mse(self, model, u, target):
+        """simulate_rollout(model, self, u, steps, 'INC') followed by the
+        trainer's MSELoss(results, target) (trainer_1d.py:154-183, :716-723)
+        with the network inside the rollout and adjoint kernels and the loss
+        fused into them: returns (loss, results, corrections).  Shapes as for
+        the training path of rollout_inc (inc_train_supported)."""
+        L.require_like(u, *model.parameters(), what="correction network parameter")
+        L.require_gpu(target)
+        if self.adaptive_CFL is not None:
+            raise ValueError("rollout_inc_mse runs a fixed time step; adaptive_CFL is set")
+        if model.fused_desc(u.dtype) is None or not inc_train_supported(model, u.shape[-1], u.dtype):
+            raise ValueError("the correction network's shape is not one the in-kernel training adjoint runs")
+        steps = target.shape[1]
+        params = [t for cv in model.convs for t in (cv.weight, cv.bias)]
+        loss, traj, corr = _BurgersIncRolloutMSE.apply(u, target, self, model, steps, self.t, *params)
+        for _ in range(steps):
+            self.t += self.dt
+        self.correction_s = corr[:, -1] if steps > 0 else None
+        return loss, traj, corr
 
     def rollout_mse(self, u, target):
         """Fused rollout over ``target.shape[1]`` steps with the MSE to

@@ -203,6 +203,23 @@ int inc_burgers_inc_train_fwd(const inc_burgers_desc* d, const inc_cnn_desc* cnn
 int inc_burgers_inc_train_adj(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* stages,
                               const void* g_traj, const void* g_corr, void* g_u, void* g_w, int steps,
                               void* stream);
+/*
+ * The same training rollout with the trainer's MSE to a target trajectory
+ * fused in (CorrectionTrainer.process_batch, trainer/trainer_1d.py:716-723:
+ * loss_fn(results, target) after simulate_rollout).  The forward also writes
+ * loss_parts [batch, 4], partial sums whose total is the sum over steps and
+ * points of (traj - target)^2 (one partial per wavefront, unused slots
+ * zero); traj is required (the adjoint reads it).  The adjoint adds
+ * scale[0] * (traj[t] - target[t]) to the cotangent of every output, scale a
+ * device scalar (2 dL/dloss / (batch * steps * n) for the mean), so the
+ * backward needs no host synchronisation and no cotangent tensor.
+ */
+int inc_burgers_inc_train_fwd_mse(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* u, void* u_out,
+                                  int steps, void* traj, void* corr, void* stages, const void* target,
+                                  void* loss_parts, void* stream);
+int inc_burgers_inc_train_adj_mse(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* stages,
+                                  const void* traj, const void* target, const void* scale, const void* g_traj,
+                                  const void* g_corr, void* g_u, void* g_w, int steps, void* stream);
 
 /*
  * Adaptive sub-stepping per trajectory (BurgersSolverTorch.step_adaptive,

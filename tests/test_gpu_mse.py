@@ -17,8 +17,9 @@ import pytest
 import torch
 from torch.nn import functional as F
 
+import inc_amd
 from inc_amd import simulate_rollout_mse
-from oracle import KSOracle, ks_params
+from oracle import BurgersOracle, KSOracle, burgers_params, ks_params
 from oracle.rollout import simulate_rollout as oracle_rollout
 from tests.conftest import rel_l2
 from tests.test_gpu_burgers import make_pair, smooth
@@ -177,3 +178,53 @@ def test_ks_rollout_mse_fp64_matches_oracle(cuda):
     assert abs(lg.item() - lo.item()) <= 1e-11 * lo.item()
     assert rel_l2(traj, res.detach()) < 1e-11
     assert rel_l2(ug.grad, uo.grad) < 1e-11
+
+
+@pytest.mark.parametrize("case", [(128, 8, 5, 4, "SSPRK3", 12), (64, 5, 7, 3, "Euler", 6), (96, 3, 5, 2, "SSPRK3", 5)],
+                         ids=lambda c: "n{}_b{}_k{}_d{}_{}_h{}".format(*c))
+def test_inc_training_rollout_with_fused_mse(cuda, case):
+    # simulate_rollout(model, solver, u, H, 'INC') + MSELoss(results, target)
+    # (trainer_1d.py:154-183, :716-723) with the network inside the kernels and
+    # the loss fused into them, against the same rollout with torch's MSE and
+    # autograd (same kernels, the cotangent formed in torch), and against the
+    # FP64 oracle driven by a CPU float64 copy of the network
+    from inc_amd.correction import CorrectionCNN
+    from oracle.rollout import simulate_rollout as oracle_rollout
+    n, b, k, depth, scheme, H = case
+    gen = torch.Generator().manual_seed(n + b)
+    u0 = smooth(b, n, gen)
+    target = 0.3 * torch.randn(b, H, n, generator=gen, dtype=torch.float64)
+    cpu = CorrectionCNN(depth, 32, k, generator=torch.Generator().manual_seed(7), backend="torch", scale=0.1).double()
+    dx = 2 * math.pi / n
+    dt = 0.3 * min(dx, dx * dx / 0.02)
+    net = CorrectionCNN(depth, 32, k, scale=0.1).to(cuda)
+    net.load_state_dict(cpu.state_dict())
+
+    def solver():
+        return inc_amd.BurgersSolverTorch(inc_amd.BGSimParams(dt=dt, resolution=n, nu=0.02, batch_size=b,
+                                                              time_scheme=scheme, device="cuda"))
+    tg = target.float().to(cuda)
+    u = u0.float().to(cuda).requires_grad_(True)
+    net.zero_grad(set_to_none=True)
+    loss, traj, corr = inc_amd.simulate_rollout_mse(solver(), u, tg, model=net)
+    loss.backward()
+    g_fused = [u.grad.clone()] + [p.grad.clone() for p in net.parameters()]
+    u2 = u0.float().to(cuda).requires_grad_(True)
+    net.zero_grad(set_to_none=True)
+    res, corr2 = inc_amd.simulate_rollout(net, solver(), u2, H, "INC")
+    loss2 = torch.nn.functional.mse_loss(res, tg)
+    loss2.backward()
+    g_torch = [u2.grad.clone()] + [p.grad.clone() for p in net.parameters()]
+    assert torch.equal(traj, res.detach()) and torch.equal(corr, corr2.detach())
+    assert rel_l2(loss.detach().reshape(1), loss2.detach().reshape(1)) < 1e-6
+    for a, c in zip(g_fused, g_torch):
+        assert rel_l2(a, c) < 1e-5
+    so = BurgersOracle(burgers_params(n, dt, nu=0.02, batch_size=b, time_scheme=scheme))
+    uo = u0.clone().requires_grad_(True)
+    ref, _ = oracle_rollout(cpu, so, uo, H, "INC")
+    lo = ((ref - target) ** 2).mean()
+    lo.backward()
+    assert rel_l2(loss.detach().reshape(1), lo.detach().reshape(1)) < 1e-5
+    assert rel_l2(g_fused[0], uo.grad) < 1e-4
+    for a, p in zip(g_fused[1:], cpu.parameters()):
+        assert rel_l2(a, p.grad) < 1e-4
