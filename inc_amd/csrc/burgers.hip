@@ -221,6 +221,11 @@ struct TeamCtx {
 
 // Team-wide argmax of |u| over the team's points.  Writes per-wave partials
 // to LDS before the barrier the caller issues; finish after that barrier.
+// The common case (no NaN among the team's values) reduces the maximum alone
+// and takes the index from the first lane that holds it: lanes hold
+// consecutive points, so that lane's own first maximal index is the team's
+// first maximal index, as torch.max(dim) returns.  A NaN anywhere (it wins,
+// first NaN index) takes the full (value, index) shuffle reduction.
 template <typename T, int PPT>
 __device__ __forceinline__ void argmax_begin(const TeamCtx<T>& c, const T* u, T& v, int& idx,
                                              T* red_v, int* red_i) {
@@ -235,10 +240,22 @@ __device__ __forceinline__ void argmax_begin(const TeamCtx<T>& c, const T* u, T&
     }
   }
   const int width = c.team < kWave ? c.team : kWave;
-  for (int off = 1; off < width; off <<= 1) {
-    T ov = shfl_xor(v, off);
-    int oi = __shfl_xor(idx, off, kWave);
-    if (better(v, idx, ov, oi)) { v = ov; idx = oi; }
+  const int lane = threadIdx.x & (kWave - 1);
+  const int base = lane & ~(width - 1);  // first lane of this team's part of the wavefront
+  const uint64_t team_mask = (width == kWave ? ~0ull : ((1ull << width) - 1)) << base;
+  if ((__ballot(v != v) & team_mask) == 0) {
+    T m = v;
+    for (int off = 1; off < width; off <<= 1) m = fmax(m, shfl_xor(m, off));
+    const uint64_t hit = __ballot(v == m) & team_mask;
+    const int first = __ffsll(static_cast<unsigned long long>(hit)) - 1;
+    idx = __shfl(idx, first, kWave);
+    v = m;
+  } else {
+    for (int off = 1; off < width; off <<= 1) {
+      T ov = shfl_xor(v, off);
+      int oi = __shfl_xor(idx, off, kWave);
+      if (better(v, idx, ov, oi)) { v = ov; idx = oi; }
+    }
   }
   if (c.team > kWave && (threadIdx.x & (kWave - 1)) == 0) {
     red_v[c.wave_in_block] = v;
@@ -378,20 +395,20 @@ __device__ __forceinline__ void rhs_vjp(const BArgs<T>& a, const TeamCtx<T>& c, 
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
         const int i = c.i0 + p;
-        // N_i = -(F_i - F_{i-1})/dx  =>  Fbar_i = (c_{i+1} - c_i)/dx
-        T Fbar = (cw[p + 2] - cw[p + 1]) * a.inv_dx;
+        // N_i = -(F_i - F_{i-1})/dx  =>  Fbar_i = (c_{i+1} - c_i)/dx; the
+        // halves of d f+/du = (u + alpha)/2, d f-/du = (u - alpha)/2,
+        // d f+/dalpha = u/2 and d f-/dalpha = -u/2 are folded into it
+        T Fbar = (cw[p + 2] - cw[p + 1]) * (T(0.5) * a.inv_dx);
         T dfp[5], dfm[5];
         weno_vjp<0>(fp + p, Fbar, dfp);      // taps i-2..i+2
         weno_vjp<1>(fm + p + 1, Fbar, dfm);  // taps i-1..i+3
-        // d f+/du = (u + alpha)/2, d f-/du = (u - alpha)/2,
-        // d f+/dalpha = u/2, d f-/dalpha = -u/2
 #pragma unroll
         for (int m = -2; m <= 3; ++m) {
           T gp = (m <= 2) ? dfp[m + 2] : T(0);
           T gm = (m >= -1) ? dfm[m + 1] : T(0);
           T uu = uw[p + 2 + m];
-          T contrib = T(0.5) * (gp * (uu + alpha) + gm * (uu - alpha));
-          abar += T(0.5) * uu * (gp - gm);
+          T contrib = gp * (uu + alpha) + gm * (uu - alpha);
+          abar += uu * (gp - gm);
           if (i < n) C[(m + 2) * n + i] = contrib;
         }
       }
