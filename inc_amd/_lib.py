@@ -88,12 +88,16 @@ SIGNATURES = {
     "inc_weno_adj": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int, _vp]),
     "inc_burgers_rhs": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp]),
     "inc_burgers_rhs_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp]),
+    "inc_burgers_rhs_adj_nu": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     "inc_burgers_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "inc_burgers_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "inc_burgers_adj_nu": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "inc_burgers_fwd_mse": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp,
                                            _vp, _vp]),
     "inc_burgers_adj_mse": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                            ctypes.c_int, _vp]),
+    "inc_burgers_adj_mse_nu": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                              ctypes.c_int, _vp]),
     "inc_burgers_inc_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.POINTER(CnnDesc), _vp, _vp,
                                            ctypes.c_int, _vp, _vp, _vp]),
     "inc_burgers_inc_train_fwd": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.POINTER(CnnDesc), _vp, _vp,
@@ -108,6 +112,8 @@ SIGNATURES = {
                                                 _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     "inc_burgers_adaptive_adj": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
                                                 _vp, ctypes.c_int, _vp]),
+    "inc_burgers_adaptive_adj_nu": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), ctypes.c_int, _vp, _vp, _vp, _vp,
+                                                   _vp, _vp, _vp, ctypes.c_int, _vp]),
     "inc_burgers_ragged_fwd": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp,
                                               _vp]),
     "inc_burgers_ragged_adj": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),

@@ -199,6 +199,9 @@ struct BArgs {
   int max_sub;
   int* nsub;
   T* dts;
+  // [batch] cotangent of the viscosity (per trajectory), written by the
+  // adjoint kernels when non-null: sum over stages of cb . (D2 u)
+  T* g_nu;
 };
 
 constexpr int kBlock = 256;
@@ -358,10 +361,12 @@ __device__ __forceinline__ void rhs_eval(const BArgs<T>& a, const TeamCtx<T>& c,
 }
 
 // cotangent of the right-hand side: ub = (dR/du)^T cb for the team's points.
+// With want_nu, this thread's points add cb . dR/dnu = cb . (D2 u) to *nub.
 template <typename T, int PPT, int CONV>
 __device__ __forceinline__ void rhs_vjp(const BArgs<T>& a, const TeamCtx<T>& c, const T* u,
                                         const T* cb, T nu, int parts, T* us, T* cs, T* C,
-                                        T* red_v, int* red_i, T* ub) {
+                                        T* red_v, int* red_i, T* ub, bool want_nu = false,
+                                        T* nub = nullptr) {
   constexpr int W = PPT + 5;
   const int n = c.n;
   const bool own = c.active && c.i0 < n;
@@ -382,6 +387,13 @@ __device__ __forceinline__ void rhs_vjp(const BArgs<T>& a, const TeamCtx<T>& c, 
     for (int k = 0; k < W; ++k) uw[k] = us[wrap(c.i0 - 2 + k, n)];
 #pragma unroll
     for (int k = 0; k < PPT + 2; ++k) cw[k] = cs[wrap(c.i0 - 1 + k, n)];
+  }
+  if (want_nu && own && (parts & INC_PART_DIFFUSION)) {
+    // the forward's diffusion term is nu * ((u_{i-1} - 2 u_i + u_{i+1}) * inv_dx2)
+    T acc = T(0);
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) acc += cb[p] * ((uw[p + 1] - T(2) * uw[p + 2] + uw[p + 3]) * a.inv_dx2);
+    *nub += acc;
   }
   if (own && conv) {
     if (CONV == INC_WENO) {
@@ -478,6 +490,22 @@ __device__ __forceinline__ TeamCtx<T> make_ctx(int team, int n, int batch, int p
   c.i0 = c.lt * ppt;
   c.n = n;
   return c;
+}
+
+// Sum of v over the team, written to out[c.b] by the team's first thread.
+// Block-uniform call (it holds a barrier when teams span wavefronts); uses
+// red_v[0, kMaxWaves), which no other reader may still hold.
+template <typename T>
+__device__ __forceinline__ void team_sum_store(const TeamCtx<T>& c, T v, T* red_v, T* out) {
+  const int width = c.team < kWave ? c.team : kWave;
+  v = wave_sum(v, width);
+  if (c.team > kWave) {
+    if ((threadIdx.x & (kWave - 1)) == 0) red_v[c.wave_in_block] = v;
+    __syncthreads();
+    v = red_v[c.first_wave];
+    for (int w = 1; w < c.team / kWave; ++w) v += red_v[c.first_wave + w];
+  }
+  if (c.active && c.lt == 0) out[c.b] = v;
 }
 
 // Shared-memory layout per team (nb = number of n-sized arrays).
@@ -1552,16 +1580,22 @@ __global__ void __launch_bounds__(kBlock) burgers_rhs_adj_kernel(BArgs<T> a) {
     u[p] = ok ? a.u[row + i] : T(0);
     g[p] = ok ? a.g_traj[row + i] : T(0);
   }
-  rhs_vjp<T, PPT, CONV>(a, c, u, g, nu, a.parts, us, cs, C, red_v, red_i, ub);
+  T nubar = T(0);
+  rhs_vjp<T, PPT, CONV>(a, c, u, g, nu, a.parts, us, cs, C, red_v, red_i, ub, a.g_nu != nullptr, &nubar);
   if (own) {
 #pragma unroll
     for (int p = 0; p < PPT; ++p)
       if (c.i0 + p < n) a.out[row + c.i0 + p] = ub[p];
   }
+  if (a.g_nu) team_sum_store(c, nubar, red_v, a.g_nu);
 }
 
 // Reverse sweep over `steps` steps from the saved stage states.
-template <typename T, int PPT, int CONV>
+// NU: also the cotangent of nu (a separate instance: the accumulator's
+// register would take the plain adjoint past the count that keeps six
+// wavefronts per SIMD, 84 -> 88 VGPRs, 13.9 -> 18.9 ms at B=8192, N=256,
+// 200 SSP-RK3 steps).
+template <typename T, int PPT, int CONV, bool NU>
 __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ T red_v[2 * kMaxWaves];
@@ -1577,6 +1611,10 @@ __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
   const T dt = a.dt;
   const int parts = INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE;
   T lam[PPT], gs[PPT], st[3][PPT], cb[PPT], r[PPT], b1[PPT], b2[PPT];
+  __shared__ T nu_acc[NU ? kBlock : 1];  // per-thread cotangent of nu (see rhs_vjp)
+  constexpr bool want_nu = NU;
+  T* const nub = &nu_acc[NU ? threadIdx.x : 0];
+  if (want_nu) *nub = T(0);
   // cotangent of output `step`: g_traj, plus *loss_scale * (traj - target)
   // with a target
   const T lsc = a.target ? *a.loss_scale : T(0);
@@ -1606,7 +1644,7 @@ __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
     if (ns == 1) {
 #pragma unroll
       for (int p = 0; p < PPT; ++p) cb[p] = on ? h * lam[p] : T(0);
-      rhs_vjp<T, PPT, CONV>(a, c, st[0], cb, nu, parts, us, cs, C, red_v, red_i, r);
+      rhs_vjp<T, PPT, CONV>(a, c, st[0], cb, nu, parts, us, cs, C, red_v, red_i, r, want_nu, nub);
       __syncthreads();
       if (on) {
 #pragma unroll
@@ -1616,7 +1654,7 @@ __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
       // u' = u0/3 + 2/3 (u2 + h R(u2))
 #pragma unroll
       for (int p = 0; p < PPT; ++p) cb[p] = on ? T(2.0 / 3.0) * h * lam[p] : T(0);
-      rhs_vjp<T, PPT, CONV>(a, c, st[2], cb, nu, parts, us, cs, C, red_v, red_i, r);
+      rhs_vjp<T, PPT, CONV>(a, c, st[2], cb, nu, parts, us, cs, C, red_v, red_i, r, want_nu, nub);
       __syncthreads();
       if (on) {
 #pragma unroll
@@ -1629,7 +1667,7 @@ __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
       // u2 = 3/4 u0 + 1/4 (u1 + h R(u1))
 #pragma unroll
       for (int p = 0; p < PPT; ++p) cb[p] = on ? T(0.25) * h * b2[p] : T(0);
-      rhs_vjp<T, PPT, CONV>(a, c, st[1], cb, nu, parts, us, cs, C, red_v, red_i, r);
+      rhs_vjp<T, PPT, CONV>(a, c, st[1], cb, nu, parts, us, cs, C, red_v, red_i, r, want_nu, nub);
       __syncthreads();
       if (on) {
 #pragma unroll
@@ -1642,7 +1680,7 @@ __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
       // u1 = u0 + h R(u0)
 #pragma unroll
       for (int p = 0; p < PPT; ++p) cb[p] = on ? h * b1[p] : T(0);
-      rhs_vjp<T, PPT, CONV>(a, c, st[0], cb, nu, parts, us, cs, C, red_v, red_i, r);
+      rhs_vjp<T, PPT, CONV>(a, c, st[0], cb, nu, parts, us, cs, C, red_v, red_i, r, want_nu, nub);
       __syncthreads();
       if (on) {
 #pragma unroll
@@ -1684,6 +1722,7 @@ __global__ void __launch_bounds__(kBlock) burgers_adj_kernel(BArgs<T> a) {
       }
     }
   }
+  if (want_nu) team_sum_store(c, *nub, red_v, a.g_nu);
 }
 
 // ---- ragged batches ------------------------------------------------------------
@@ -2086,7 +2125,7 @@ int fill_args(const inc_burgers_desc* d, BArgs<T>& a) {
   return INC_OK;
 }
 
-enum class Kind { Fwd, Rhs, RhsAdj, Adj };
+enum class Kind { Fwd, Rhs, RhsAdj, Adj, AdjNu };
 
 template <typename T, int PPT, int CONV>
 void* kernel_ptr(Kind k) {
@@ -2094,7 +2133,8 @@ void* kernel_ptr(Kind k) {
     case Kind::Fwd: return reinterpret_cast<void*>(&burgers_fwd_kernel<T, PPT, CONV>);
     case Kind::Rhs: return reinterpret_cast<void*>(&burgers_rhs_kernel<T, PPT, CONV>);
     case Kind::RhsAdj: return reinterpret_cast<void*>(&burgers_rhs_adj_kernel<T, PPT, CONV>);
-    default: return reinterpret_cast<void*>(&burgers_adj_kernel<T, PPT, CONV>);
+    case Kind::AdjNu: return reinterpret_cast<void*>(&burgers_adj_kernel<T, PPT, CONV, true>);
+    default: return reinterpret_cast<void*>(&burgers_adj_kernel<T, PPT, CONV, false>);
   }
 }
 
@@ -2112,7 +2152,8 @@ void* kernel_for_ppt(Kind k, int ppt) {
 template <typename T>
 int launch(Kind kind, const inc_burgers_desc* d, BArgs<T>& a, void* stream) {
   int team = 0;
-  const bool weno_vjp = (kind == Kind::Adj || kind == Kind::RhsAdj) && d->convection == INC_WENO;
+  if (kind == Kind::Adj && a.g_nu) kind = Kind::AdjNu;
+  const bool weno_vjp = kind != Kind::Fwd && kind != Kind::Rhs && d->convection == INC_WENO;
   const int ppt_max = weno_vjp ? (std::is_same<T, float>::value ? 2 : 1) : 4;
   int ppt = pick_ppt(d->n, d->team, &team, d->batch, ppt_max);
   INC_REQUIRE(ppt > 0 && team >= 1 && team <= kBlock && (team & (team - 1)) == 0, INC_EUNSUPPORTED,
@@ -2173,7 +2214,7 @@ struct Mse {
 template <typename T>
 int burgers_call(Kind kind, const inc_burgers_desc* d, int parts, const void* u, const void* s,
                  void* out, int steps, void* traj, void* stages, const void* g_traj, void* g_s,
-                 void* stream, const Adaptive& ad, const Mse& mse) {
+                 void* stream, const Adaptive& ad, const Mse& mse, void* g_nu) {
   BArgs<T> a;
   int rc = fill_args<T>(d, a);
   if (rc) return rc;
@@ -2193,18 +2234,21 @@ int burgers_call(Kind kind, const inc_burgers_desc* d, int parts, const void* u,
   a.target = static_cast<const T*>(mse.target);
   a.loss_parts = static_cast<T*>(mse.parts);
   a.loss_scale = static_cast<const T*>(mse.scale);
+  a.g_nu = static_cast<T*>(g_nu);
   return launch<T>(kind, d, a, stream);
 }
 
 int dispatch(Kind kind, const inc_burgers_desc* d, int parts, const void* u, const void* s, void* out,
              int steps, void* traj, void* stages, const void* g_traj, void* g_s, void* stream,
-             const Adaptive& ad = Adaptive{0.0, 0, nullptr, nullptr}, const Mse& mse = Mse{}) {
+             const Adaptive& ad = Adaptive{0.0, 0, nullptr, nullptr}, const Mse& mse = Mse{},
+             void* g_nu = nullptr) {
   clear_error();
   INC_REQUIRE(d != nullptr, INC_EINVAL, "null descriptor");
   if (d->dtype == INC_F32)
-    return burgers_call<float>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream, ad, mse);
+    return burgers_call<float>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream, ad, mse, g_nu);
   if (d->dtype == INC_F64)
-    return burgers_call<double>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream, ad, mse);
+    return burgers_call<double>(kind, d, parts, u, s, out, steps, traj, stages, g_traj, g_s, stream, ad, mse,
+                                g_nu);
   set_error("unsupported dtype %d", d->dtype);
   return INC_EINVAL;
 }
@@ -2435,6 +2479,12 @@ int inc_burgers_rhs_adj(const inc_burgers_desc* d, int parts, const void* u, con
   return inc::dispatch(Kind::RhsAdj, d, parts, u, nullptr, g_u, 1, nullptr, nullptr, g, nullptr, stream);
 }
 
+int inc_burgers_rhs_adj_nu(const inc_burgers_desc* d, int parts, const void* u, const void* g, void* g_u,
+                           void* g_nu, void* stream) {
+  return inc::dispatch(Kind::RhsAdj, d, parts, u, nullptr, g_u, 1, nullptr, nullptr, g, nullptr, stream,
+                       inc::Adaptive{0.0, 0, nullptr, nullptr}, inc::Mse{}, g_nu);
+}
+
 int inc_burgers_fwd(const inc_burgers_desc* d, const void* u, const void* s, void* u_out, int steps,
                     void* traj, void* stages, void* stream) {
   if (steps < 1) {
@@ -2458,13 +2508,18 @@ int inc_burgers_ragged_adj(const inc_ragged_desc* d, const void* stages, const v
 
 int inc_burgers_adj(const inc_burgers_desc* d, const void* stages, const void* g_traj, void* g_u,
                     void* g_s, int steps, void* stream) {
+  return inc_burgers_adj_nu(d, stages, g_traj, g_u, g_s, nullptr, steps, stream);
+}
+
+int inc_burgers_adj_nu(const inc_burgers_desc* d, const void* stages, const void* g_traj, void* g_u,
+                       void* g_s, void* g_nu, int steps, void* stream) {
   if (steps < 1) {
     inc::set_error("steps must be >= 1 (got %d)", steps);
     return INC_EINVAL;
   }
   return inc::dispatch(Kind::Adj, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE,
                        nullptr, nullptr, g_u, steps, nullptr, const_cast<void*>(stages), g_traj, g_s,
-                       stream);
+                       stream, inc::Adaptive{0.0, 0, nullptr, nullptr}, inc::Mse{}, g_nu);
 }
 
 int inc_burgers_fwd_mse(const inc_burgers_desc* d, const void* u, const void* s, void* u_out, int steps,
@@ -2481,6 +2536,12 @@ int inc_burgers_fwd_mse(const inc_burgers_desc* d, const void* u, const void* s,
 
 int inc_burgers_adj_mse(const inc_burgers_desc* d, const void* stages, const void* traj, const void* target,
                         const void* scale, const void* g_traj, void* g_u, void* g_s, int steps, void* stream) {
+  return inc_burgers_adj_mse_nu(d, stages, traj, target, scale, g_traj, g_u, g_s, nullptr, steps, stream);
+}
+
+int inc_burgers_adj_mse_nu(const inc_burgers_desc* d, const void* stages, const void* traj, const void* target,
+                           const void* scale, const void* g_traj, void* g_u, void* g_s, void* g_nu, int steps,
+                           void* stream) {
   inc::clear_error();
   INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
   INC_REQUIRE(stages != nullptr && traj != nullptr && target != nullptr && scale != nullptr, INC_EINVAL,
@@ -2490,7 +2551,7 @@ int inc_burgers_adj_mse(const inc_burgers_desc* d, const void* stages, const voi
   m.scale = scale;
   return inc::dispatch(Kind::Adj, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE, nullptr,
                        nullptr, g_u, steps, const_cast<void*>(traj), const_cast<void*>(stages), g_traj, g_s,
-                       stream, inc::Adaptive{0.0, 0, nullptr, nullptr}, m);
+                       stream, inc::Adaptive{0.0, 0, nullptr, nullptr}, m, g_nu);
 }
 
 int inc_burgers_inc_fwd(const inc_burgers_desc* d, const inc_cnn_desc* cnn, const void* u, void* u_out, int steps,
@@ -2567,6 +2628,12 @@ int inc_burgers_adaptive_fwd(const inc_burgers_desc* d, double cfl, int max_sub,
 int inc_burgers_adaptive_adj(const inc_burgers_desc* d, int max_sub, const void* nsub, const void* dts,
                              const void* stages, const void* g_traj, void* g_u, void* g_s, int steps,
                              void* stream) {
+  return inc_burgers_adaptive_adj_nu(d, max_sub, nsub, dts, stages, g_traj, g_u, g_s, nullptr, steps, stream);
+}
+
+int inc_burgers_adaptive_adj_nu(const inc_burgers_desc* d, int max_sub, const void* nsub, const void* dts,
+                                const void* stages, const void* g_traj, void* g_u, void* g_s, void* g_nu,
+                                int steps, void* stream) {
   inc::clear_error();
   INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
   INC_REQUIRE(max_sub >= 1 && max_sub <= 1000, INC_EINVAL, "max_sub must be in [1, 1000] (got %d)", max_sub);
@@ -2575,7 +2642,8 @@ int inc_burgers_adaptive_adj(const inc_burgers_desc* d, int max_sub, const void*
   // cfl only selects the adaptive sweep here: the sub-step sizes are read from dts
   return inc::dispatch(Kind::Adj, d, INC_PART_CONVECTION | INC_PART_DIFFUSION | INC_PART_SOURCE, nullptr,
                        nullptr, g_u, steps, nullptr, const_cast<void*>(stages), g_traj, g_s, stream,
-                       inc::Adaptive{1.0, max_sub, const_cast<void*>(nsub), const_cast<void*>(dts)});
+                       inc::Adaptive{1.0, max_sub, const_cast<void*>(nsub), const_cast<void*>(dts)}, inc::Mse{},
+                       g_nu);
 }
 
 }  // extern "C"

@@ -103,16 +103,39 @@ def _ks_correction(s, u):
     return s
 
 
+def _nu_input(solver):
+    """The solver's viscosity when autograd is to differentiate it (a tensor
+    that requires grad), else None: passed to the autograd Functions as an
+    input so that its cotangent reaches it."""
+    nu = solver.nu
+    if torch.is_grad_enabled() and isinstance(nu, torch.Tensor) and nu.requires_grad:
+        return nu
+    return None
+
+
+def _nu_cotangent(nu, g_nu):
+    """Per-trajectory cotangent [batch] -> the gradient in nu's own shape,
+    dtype and device (a one-element nu, broadcast over the batch, gets the
+    sum)."""
+    g = g_nu.sum() if nu.numel() == 1 else g_nu
+    return g.reshape(nu.shape).to(dtype=nu.dtype, device=nu.device)
+
+
+def _nu_buffer(ctx, index, b, like):
+    return torch.empty(b, dtype=like.dtype, device=like.device) if ctx.needs_input_grad[index] else None
+
+
 class _BurgersAdvance(torch.autograd.Function):
     """`steps` steps of the Burgers map; returns the final state, or the
     [B, steps, N] trajectory when ``want_traj``."""
 
     @staticmethod
-    def forward(ctx, u, s, solver, dt, steps, t, want_traj, need_stages):
+    def forward(ctx, u, s, solver, dt, steps, t, want_traj, need_stages, nu):
         u = u.contiguous()
         s = s.contiguous() if s is not None else None
         b, n = u.shape
         desc, keep = solver._desc(u, dt, t)
+        ctx.nu = nu
         out = torch.empty_like(u)
         traj = u.new_empty(b, steps, n) if want_traj else None
         ns = 3 if solver.time_scheme == "SSPRK3" else 1
@@ -140,10 +163,12 @@ class _BurgersAdvance(torch.autograd.Function):
             g_traj[:, -1] = g
         g_u = torch.empty(b, n, dtype=g.dtype, device=g.device)
         g_s = torch.empty_like(g_u) if ctx.needs_input_grad[1] else None
-        rc = L.load().inc_burgers_adj(ctypes.byref(ctx.desc), L.ptr(stages), L.ptr(g_traj), L.ptr(g_u),
-                                      L.ptr(g_s), ctx.steps, L.stream())
+        g_nu = _nu_buffer(ctx, 8, b, g_u)
+        rc = L.load().inc_burgers_adj_nu(ctypes.byref(ctx.desc), L.ptr(stages), L.ptr(g_traj), L.ptr(g_u),
+                                         L.ptr(g_s), L.ptr(g_nu), ctx.steps, L.stream())
         L.check(rc, "BurgersSolverTorch.step backward")
-        return g_u, g_s, None, None, None, None, None, None
+        g_nu = _nu_cotangent(ctx.nu, g_nu) if g_nu is not None else None
+        return g_u, g_s, None, None, None, None, None, None, g_nu
 
 
 def _mse_target(target, u, steps):
@@ -171,8 +196,9 @@ class _BurgersRolloutMSE(torch.autograd.Function):
     batch, steps and points, torch.nn.MSELoss's default."""
 
     @staticmethod
-    def forward(ctx, u, s, target, solver, dt, steps, t, need_stages):
+    def forward(ctx, u, s, target, solver, dt, steps, t, need_stages, nu):
         ctx.set_materialize_grads(False)
+        ctx.nu = nu
         u = u.contiguous()
         s = s.contiguous() if s is not None else None
         b, n = u.shape
@@ -201,11 +227,13 @@ class _BurgersRolloutMSE(torch.autograd.Function):
         b, n = ctx.shape
         g_u = torch.empty(b, n, dtype=traj.dtype, device=traj.device)
         g_s = torch.empty_like(g_u) if ctx.needs_input_grad[1] else None
-        rc = L.load().inc_burgers_adj_mse(ctypes.byref(ctx.desc), L.ptr(stages), L.ptr(traj), L.ptr(tgt),
-                                          L.ptr(scale), L.ptr(g_traj), L.ptr(g_u), L.ptr(g_s), ctx.steps,
-                                          L.stream())
+        g_nu = _nu_buffer(ctx, 8, b, g_u)
+        rc = L.load().inc_burgers_adj_mse_nu(ctypes.byref(ctx.desc), L.ptr(stages), L.ptr(traj), L.ptr(tgt),
+                                             L.ptr(scale), L.ptr(g_traj), L.ptr(g_u), L.ptr(g_s), L.ptr(g_nu),
+                                             ctx.steps, L.stream())
         L.check(rc, "BurgersSolverTorch.rollout_mse backward")
-        return g_u, g_s, g_target, None, None, None, None, None
+        g_nu = _nu_cotangent(ctx.nu, g_nu) if g_nu is not None else None
+        return g_u, g_s, g_target, None, None, None, None, None, g_nu
 
 
 class _BurgersIncRollout(torch.autograd.Function):
@@ -341,11 +369,12 @@ class _BurgersAdaptive(torch.autograd.Function):
     gradient, as with the reference's .item()."""
 
     @staticmethod
-    def forward(ctx, u, s, solver, steps, t, want_traj, need_stages):
+    def forward(ctx, u, s, solver, steps, t, want_traj, need_stages, nu):
         u = u.contiguous()
         s = s.contiguous() if s is not None else None
         b, n = u.shape
         desc, keep = solver._desc(u, solver.dt, t)
+        ctx.nu = nu
         ms = solver.max_substeps
         ns = 3 if solver.time_scheme == "SSPRK3" else 1
         out = torch.empty_like(u)
@@ -378,23 +407,25 @@ class _BurgersAdaptive(torch.autograd.Function):
             g_traj[:, -1] = g
         g_u = torch.empty(b, n, dtype=g.dtype, device=g.device)
         g_s = torch.empty_like(g_u) if ctx.needs_input_grad[1] else None
-        rc = L.load().inc_burgers_adaptive_adj(ctypes.byref(ctx.desc), ctx.ms, L.ptr(nsub), L.ptr(dts),
-                                               L.ptr(stages), L.ptr(g_traj), L.ptr(g_u), L.ptr(g_s), ctx.steps,
-                                               L.stream())
+        g_nu = _nu_buffer(ctx, 7, b, g_u)
+        rc = L.load().inc_burgers_adaptive_adj_nu(ctypes.byref(ctx.desc), ctx.ms, L.ptr(nsub), L.ptr(dts),
+                                                  L.ptr(stages), L.ptr(g_traj), L.ptr(g_u), L.ptr(g_s),
+                                                  L.ptr(g_nu), ctx.steps, L.stream())
         L.check(rc, "BurgersSolverTorch.step (adaptive) backward")
-        return g_u, g_s, None, None, None, None, None
+        g_nu = _nu_cotangent(ctx.nu, g_nu) if g_nu is not None else None
+        return g_u, g_s, None, None, None, None, None, g_nu
 
 
 class _BurgersRHS(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u, solver, parts, t):
+    def forward(ctx, u, solver, parts, t, nu):
         u = u.contiguous()
         desc, keep = solver._desc(u, solver.dt, t)
         out = torch.empty_like(u)
         rc = L.load().inc_burgers_rhs(ctypes.byref(desc), parts, L.ptr(u), None, L.ptr(out), L.stream())
         L.check(rc, "BurgersSolverTorch right-hand side")
         ctx.save_for_backward(u)
-        ctx.desc, ctx.keep, ctx.parts = desc, keep, parts
+        ctx.desc, ctx.keep, ctx.parts, ctx.nu = desc, keep, parts, nu
         return out
 
     @staticmethod
@@ -402,10 +433,12 @@ class _BurgersRHS(torch.autograd.Function):
         (u,) = ctx.saved_tensors
         g = g.contiguous()
         g_u = torch.empty_like(u)
-        rc = L.load().inc_burgers_rhs_adj(ctypes.byref(ctx.desc), ctx.parts, L.ptr(u), L.ptr(g), L.ptr(g_u),
-                                          L.stream())
+        g_nu = _nu_buffer(ctx, 4, u.shape[0], u)
+        rc = L.load().inc_burgers_rhs_adj_nu(ctypes.byref(ctx.desc), ctx.parts, L.ptr(u), L.ptr(g), L.ptr(g_u),
+                                             L.ptr(g_nu), L.stream())
         L.check(rc, "BurgersSolverTorch right-hand side backward")
-        return g_u, None, None, None
+        g_nu = _nu_cotangent(ctx.nu, g_nu) if g_nu is not None else None
+        return g_u, None, None, None, g_nu
 
 
 class BurgersSolverTorch:
@@ -509,8 +542,6 @@ class BurgersSolverTorch:
         keep = []
         d.nu, nu_t = self._per_traj("nu", self.nu, b, dtype, dev)
         if nu_t is not None:
-            if isinstance(self.nu, torch.Tensor) and self.nu.requires_grad:
-                raise ValueError("gradients with respect to nu are not supported by the HIP adjoint")
             keep.append(nu_t)
             d.nu_ptr = nu_t.data_ptr()
         if d.source == L.INC_SRC_LINEAR:
@@ -570,7 +601,7 @@ class BurgersSolverTorch:
         if kind not in _SOURCES:
             raise ValueError("Unknown equation type.")
         L.require_gpu(u)
-        return _BurgersRHS.apply(u, self, L.INC_PART_SOURCE, self.t)
+        return _BurgersRHS.apply(u, self, L.INC_PART_SOURCE, self.t, _nu_input(self))
 
     def _apply_corrections(self, source):
         return source + self.correction_s if self.correction_s is not None else source
@@ -579,17 +610,17 @@ class BurgersSolverTorch:
         L.require_gpu(u)
         if self._conv != L.INC_WENO:
             raise ValueError("solver was built with convection_term='FDM'")
-        return _BurgersRHS.apply(u, self, L.INC_PART_CONVECTION, self.t)
+        return _BurgersRHS.apply(u, self, L.INC_PART_CONVECTION, self.t, _nu_input(self))
 
     def calc_nonlinear_FDM(self, u):
         L.require_gpu(u)
         if self._conv != L.INC_FDM:
             raise ValueError("solver was built with convection_term='WENO'")
-        return _BurgersRHS.apply(u, self, L.INC_PART_CONVECTION, self.t)
+        return _BurgersRHS.apply(u, self, L.INC_PART_CONVECTION, self.t, _nu_input(self))
 
     def calc_linear(self, u):
         L.require_gpu(u)
-        return _BurgersRHS.apply(u, self, L.INC_PART_DIFFUSION, self.t)
+        return _BurgersRHS.apply(u, self, L.INC_PART_DIFFUSION, self.t, _nu_input(self))
 
     def _advance(self, u, dt, steps=1, want_traj=False):
         L.require_gpu(u)
@@ -598,8 +629,9 @@ class BurgersSolverTorch:
             self.team = autotune.tune_burgers(u.shape[-1], u.shape[0], u.dtype, self.time_scheme,
                                               self.param.convection_term)
         s = _burgers_correction(self.correction_s, u)
-        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
-        return _BurgersAdvance.apply(u, s, self, dt, steps, self.t, want_traj, need)
+        nu = _nu_input(self)
+        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad) or nu is not None)
+        return _BurgersAdvance.apply(u, s, self, dt, steps, self.t, want_traj, need, nu)
 
     def step_adaptive(self, u):
         """Sub-step to CFL * dx / max|u| over one dt (solver_1d.py:148-179):
@@ -630,8 +662,9 @@ class BurgersSolverTorch:
         ``self.last_substeps``."""
         L.require_gpu(u)
         s = _burgers_correction(self.correction_s, u)
-        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
-        return _BurgersAdaptive.apply(u, s, self, steps, self.t, want_traj, need)
+        nu = _nu_input(self)
+        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad) or nu is not None)
+        return _BurgersAdaptive.apply(u, s, self, steps, self.t, want_traj, need, nu)
 
     def step(self, u):
         """Advance by one dt (solver_1d.py:181-195)."""
@@ -674,6 +707,9 @@ class BurgersSolverTorch:
         L.require_like(u, *model.parameters(), what="correction network parameter")
         if self.adaptive_CFL is not None:
             raise ValueError("rollout_inc runs a fixed time step; adaptive_CFL is set")
+        if _nu_input(self) is not None:
+            raise ValueError("the in-kernel INC adjoint does not differentiate nu; use simulate_rollout's "
+                             "per-step path for gradients with respect to nu")
         cnn = model.fused_desc(u.dtype)
         if cnn is None:
             raise ValueError("the correction network's shape is not one the in-kernel path runs")
@@ -711,6 +747,9 @@ class BurgersSolverTorch:
         L.require_gpu(target)
         if self.adaptive_CFL is not None:
             raise ValueError("rollout_inc_mse runs a fixed time step; adaptive_CFL is set")
+        if _nu_input(self) is not None:
+            raise ValueError("the in-kernel INC adjoint does not differentiate nu; use simulate_rollout's "
+                             "per-step path for gradients with respect to nu")
         if model.fused_desc(u.dtype) is None or not inc_train_supported(model, u.shape[-1], u.dtype):
             raise ValueError("the correction network's shape is not one the in-kernel training adjoint runs")
         steps = target.shape[1]
@@ -737,8 +776,9 @@ class BurgersSolverTorch:
                                               self.param.convection_term)
         steps = target.shape[1]
         s = _burgers_correction(self.correction_s, u)
-        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
-        loss, traj = _BurgersRolloutMSE.apply(u, s, target, self, self.dt, steps, self.t, need)
+        nu = _nu_input(self)
+        need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad) or nu is not None)
+        loss, traj = _BurgersRolloutMSE.apply(u, s, target, self, self.dt, steps, self.t, need, nu)
         for _ in range(steps):
             self.t += self.dt
         return loss, traj

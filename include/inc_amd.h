@@ -97,6 +97,16 @@ int inc_burgers_rhs(const inc_burgers_desc* d, int parts, const void* u,
 /* g_u = (d rhs_parts / du)^T g. */
 int inc_burgers_rhs_adj(const inc_burgers_desc* d, int parts, const void* u,
                         const void* g, void* g_u, void* stream);
+/*
+ * The *_nu adjoints below also return the cotangent of the viscosity, the
+ * gradient autograd gives for a nu tensor that requires grad (calc_linear,
+ * solver_1d.py:144-146: nu * (u @ D2_T)):
+ *   g_nu  [batch]  sum over the right-hand-side evaluations of g . (D2 u),
+ *                  per trajectory (a scalar nu's cotangent is its sum)
+ * Without INC_PART_DIFFUSION in parts it is zero.
+ */
+int inc_burgers_rhs_adj_nu(const inc_burgers_desc* d, int parts, const void* u,
+                           const void* g, void* g_u, void* g_nu, void* stream);
 
 /*
  * Advance `steps` steps of size dt from u (BurgersSolverTorch.step,
@@ -121,6 +131,10 @@ int inc_burgers_fwd(const inc_burgers_desc* d, const void* u, const void* s,
 int inc_burgers_adj(const inc_burgers_desc* d, const void* stages,
                     const void* g_traj, void* g_u, void* g_s, int steps,
                     void* stream);
+/* inc_burgers_adj plus g_nu [batch] (see inc_burgers_rhs_adj_nu). */
+int inc_burgers_adj_nu(const inc_burgers_desc* d, const void* stages,
+                       const void* g_traj, void* g_u, void* g_s, void* g_nu,
+                       int steps, void* stream);
 
 /*
  * Rollout with its MSE to a target trajectory fused in (simulate_rollout,
@@ -144,6 +158,11 @@ int inc_burgers_adj_mse(const inc_burgers_desc* d, const void* stages,
                         const void* traj, const void* target, const void* scale,
                         const void* g_traj, void* g_u, void* g_s, int steps,
                         void* stream);
+/* inc_burgers_adj_mse plus g_nu [batch] (see inc_burgers_rhs_adj_nu). */
+int inc_burgers_adj_mse_nu(const inc_burgers_desc* d, const void* stages,
+                           const void* traj, const void* target, const void* scale,
+                           const void* g_traj, void* g_u, void* g_s, void* g_nu,
+                           int steps, void* stream);
 
 /*
  * INC rollout with the correction network inside the kernel:
@@ -242,6 +261,10 @@ int inc_burgers_adaptive_fwd(const inc_burgers_desc* d, double cfl, int max_sub,
 int inc_burgers_adaptive_adj(const inc_burgers_desc* d, int max_sub, const void* nsub,
                              const void* dts, const void* stages, const void* g_traj,
                              void* g_u, void* g_s, int steps, void* stream);
+/* inc_burgers_adaptive_adj plus g_nu [batch] (see inc_burgers_rhs_adj_nu). */
+int inc_burgers_adaptive_adj_nu(const inc_burgers_desc* d, int max_sub, const void* nsub,
+                                const void* dts, const void* stages, const void* g_traj,
+                                void* g_u, void* g_s, void* g_nu, int steps, void* stream);
 
 /*
  * A ragged batch of Burgers trajectories with different resolution,
