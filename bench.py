@@ -661,6 +661,36 @@ def extra_ks_saturation(b=8192, h=200):
             "fwd_ms": tf * 1e3, "adj_ms": ta * 1e3}
 
 
+def extra_ks_checkpoint(b=B, h=H, ks=(10, 50, 100)):
+    """The headline workload (KS ETDRK4 fwd+adj, B=256, N=256, 1000 steps)
+    with checkpoint-and-recompute every k steps in the fused rollout: time
+    per fwd+adj and the peak memory the rollout and its adjoint allocate,
+    against the adjoint over the stages of all steps."""
+    import torch
+    import inc_amd
+    dev = torch.device("cuda")
+    sv = inc_amd.KSSolverTorch(inc_amd.KSSimParams([L], resolution=N, dt=DT, time_scheme="ETDRK4", device="cuda"))
+    sv.doublestep = False
+    Ld = torch.tensor([L], device=dev)
+    u0 = initial_state(b, N, L, 1006).float().to(dev)
+    cot = torch.randn(b, h, N, device=dev) * 1e-3
+    out = {"workload": f"KS ETDRK4 fwd+adj B={b} N={N} {h} steps fp32, checkpoint every k steps"}
+    for k in (None,) + tuple(ks):
+        def run():
+            u = u0.clone().requires_grad_(True)
+            traj = sv.rollout(u, h, Ld, checkpoint_every=k)
+            torch.autograd.backward(traj, cot)
+        run()
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        t = _timed(run)
+        peak = torch.cuda.max_memory_allocated() - base
+        out["all_stages" if k is None else f"k{k}"] = {"ms": t * 1e3, "trajectory_steps_per_s": b * h / t,
+                                                       "peak_MB": peak / 1e6}
+    return out
+
+
 def extra_ks_mse(b=B, h=H):
     """The headline rollout trained against a target trajectory: MSE loss and
     its gradient w.r.t. the initial state, with the loss fused into the
@@ -827,6 +857,7 @@ def main():
     if not args.no_extras:
         out["batched"] = batched_ragged_inc()
         out["other_workloads"] = {"ks_saturated": extra_ks_saturation(), "ks_mse_fused": extra_ks_mse(),
+                                  "ks_checkpoint": extra_ks_checkpoint(),
                                   "burgers_saturated": extra_burgers_saturation(), "burgers_inc": extra_burgers_inc(),
                                   "burgers_inc_eval": extra_burgers_inc_eval(),
                                   "ks_inc_checkpointed": extra_ks_inc()}

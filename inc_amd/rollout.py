@@ -71,10 +71,14 @@ def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, c
         return empty, (empty.clone() if correction_term else None)
     if fused and correction_term == "INC" and _fusable_inc(model, solver, u_initial):
         return solver.rollout_inc(model, u_initial, mstep)
-    if fused and _fusable(solver, correction_term):
+    # checkpoint_every=k on a fused rollout: its adjoint recomputes the stage
+    # states k steps at a time instead of keeping them for the whole rollout
+    # (fixed time step; the adaptive rollout keeps the step loop for it)
+    adaptive = isinstance(solver, BurgersSolverTorch) and solver.adaptive_CFL is not None
+    if fused and _fusable(solver, correction_term) and not (checkpoint_every and adaptive):
         if isinstance(solver, KSSolverTorch):
-            return solver.rollout(u_initial, mstep, L), None
-        return solver.rollout(u_initial, mstep), None
+            return solver.rollout(u_initial, mstep, L, checkpoint_every=checkpoint_every), None
+        return solver.rollout(u_initial, mstep, checkpoint_every=checkpoint_every), None
 
     model_fn = (lambda u: model(u)) if L is None else (lambda u: model(u, L))
     step_fn = (lambda u: solver.step(u)) if L is None else (lambda u: solver.step(u, L))

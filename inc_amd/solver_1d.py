@@ -103,6 +103,15 @@ def _ks_correction(s, u):
     return s
 
 
+def _checkpoint_k(k):
+    if k is None:
+        return 0
+    k = int(k)
+    if k < 1:
+        raise ValueError(f"checkpoint_every must be a positive number of steps (got {k})")
+    return k
+
+
 def _nu_input(solver):
     """The solver's viscosity when autograd is to differentiate it (a tensor
     that requires grad), else None: passed to the autograd Functions as an
@@ -125,12 +134,50 @@ def _nu_buffer(ctx, index, b, like):
     return torch.empty(b, dtype=like.dtype, device=like.device) if ctx.needs_input_grad[index] else None
 
 
+def _checkpointed_adjoint(g_traj, u0, traj, k, nst, fwd, adj, need_s, need_nu=False):
+    """Reverse sweep of a fused rollout whose forward kept no stage states
+    (checkpoint-and-recompute every k steps): for each segment of k steps,
+    last first, the stages are recomputed from the segment's start state
+    (u0, or the previous segment's last output, which the trajectory holds)
+    into one buffer reused by every segment, then that segment's adjoint
+    runs with the cotangent of its last output increased by the cotangent
+    carried back from the segment after it.  Memory O(B k stages N) instead
+    of O(B steps stages N).
+
+    fwd(k0, start, out, m, stages) and adj(k0, stages, g_seg, g_u, g_s,
+    g_nu, m) launch the kernels for the segment starting at step k0 with m
+    steps.  Returns (g_u0, g_s summed over segments, g_nu summed)."""
+    b, steps, n = g_traj.shape
+    buf = g_traj.new_empty(b * min(k, steps) * nst * n)
+    out = torch.empty_like(u0)
+    g_s = torch.zeros_like(u0) if need_s else None
+    g_s_seg = torch.empty_like(u0) if need_s else None
+    g_nu = u0.new_zeros(b) if need_nu else None
+    g_nu_seg = u0.new_empty(b) if need_nu else None
+    lam = None
+    for k0 in reversed(range(0, steps, k)):
+        m = min(k, steps - k0)
+        start = u0 if k0 == 0 else traj[:, k0 - 1].contiguous()
+        stages = buf[: b * m * nst * n].view(b, m, nst, n)
+        fwd(k0, start, out, m, stages)
+        g_seg = g_traj[:, k0:k0 + m].contiguous()
+        if lam is not None:
+            g_seg[:, -1] += lam
+        lam = torch.empty_like(u0)
+        adj(k0, stages, g_seg, lam, g_s_seg, g_nu_seg, m)
+        if need_s:
+            g_s += g_s_seg
+        if need_nu:
+            g_nu += g_nu_seg
+    return lam, g_s, g_nu
+
+
 class _BurgersAdvance(torch.autograd.Function):
     """`steps` steps of the Burgers map; returns the final state, or the
     [B, steps, N] trajectory when ``want_traj``."""
 
     @staticmethod
-    def forward(ctx, u, s, solver, dt, steps, t, want_traj, need_stages, nu):
+    def forward(ctx, u, s, solver, dt, steps, t, want_traj, need_stages, nu, ckpt):
         u = u.contiguous()
         s = s.contiguous() if s is not None else None
         b, n = u.shape
@@ -139,17 +186,61 @@ class _BurgersAdvance(torch.autograd.Function):
         out = torch.empty_like(u)
         traj = u.new_empty(b, steps, n) if want_traj else None
         ns = 3 if solver.time_scheme == "SSPRK3" else 1
-        stages = u.new_empty(b, steps, ns, n) if need_stages else None
+        # checkpoint every ckpt steps: no stage states now, recomputed per segment in backward
+        ctx.ckpt = ckpt if (ckpt and need_stages and want_traj and ckpt < steps) else 0
+        stages = u.new_empty(b, steps, ns, n) if need_stages and not ctx.ckpt else None
         rc = L.load().inc_burgers_fwd(ctypes.byref(desc), L.ptr(u), L.ptr(s), L.ptr(out), steps, L.ptr(traj),
                                       L.ptr(stages), L.stream())
         L.check(rc, "BurgersSolverTorch.step")
-        ctx.save_for_backward(stages)
+        if ctx.ckpt:
+            ctx.save_for_backward(u, s, traj)
+            ctx.ns, ctx.dt = ns, float(dt)
+            ctx.t = t.detach().reshape(-1).to(u.dtype).clone() if isinstance(t, torch.Tensor) else float(t)
+        else:
+            ctx.save_for_backward(stages)
         ctx.desc, ctx.keep, ctx.steps, ctx.want_traj = desc, keep, steps, want_traj
         ctx.shape = (b, n)
         return traj if want_traj else out
 
     @staticmethod
+    def _checkpointed_backward(ctx, g):
+        u0, s, traj = ctx.saved_tensors
+        lib = L.load()
+        descs = {}
+
+        def seg_desc(k0):
+            # the segment's forcing clock starts k0 steps later
+            if k0 not in descs:
+                d = L.BurgersDesc.from_buffer_copy(ctx.desc)
+                hold = None
+                if isinstance(ctx.t, torch.Tensor):
+                    hold = ctx.t + k0 * ctx.dt
+                    d.t_ptr = hold.data_ptr()
+                else:
+                    d.t0 = ctx.t + k0 * ctx.dt
+                descs.clear()
+                descs[k0] = (d, hold)
+            return descs[k0][0]
+
+        def fwd(k0, start, out, m, stages):
+            rc = lib.inc_burgers_fwd(ctypes.byref(seg_desc(k0)), L.ptr(start), L.ptr(s), L.ptr(out), m, None,
+                                     L.ptr(stages), L.stream())
+            L.check(rc, "BurgersSolverTorch rollout (checkpoint recompute)")
+
+        def adj(k0, stages, g_seg, g_u, g_s, g_nu, m):
+            rc = lib.inc_burgers_adj_nu(ctypes.byref(seg_desc(k0)), L.ptr(stages), L.ptr(g_seg), L.ptr(g_u),
+                                        L.ptr(g_s), L.ptr(g_nu), m, L.stream())
+            L.check(rc, "BurgersSolverTorch rollout backward (checkpointed)")
+
+        g_u, g_s, g_nu = _checkpointed_adjoint(g.contiguous(), u0, traj, ctx.ckpt, ctx.ns, fwd, adj,
+                                               ctx.needs_input_grad[1], ctx.needs_input_grad[8])
+        g_nu = _nu_cotangent(ctx.nu, g_nu) if g_nu is not None else None
+        return g_u, g_s, None, None, None, None, None, None, g_nu, None
+
+    @staticmethod
     def backward(ctx, g):
+        if ctx.ckpt:
+            return _BurgersAdvance._checkpointed_backward(ctx, g)
         (stages,) = ctx.saved_tensors
         if stages is None:
             raise RuntimeError("Burgers step was run without saved stages; enable grad before the step")
@@ -168,7 +259,7 @@ class _BurgersAdvance(torch.autograd.Function):
                                          L.ptr(g_s), L.ptr(g_nu), ctx.steps, L.stream())
         L.check(rc, "BurgersSolverTorch.step backward")
         g_nu = _nu_cotangent(ctx.nu, g_nu) if g_nu is not None else None
-        return g_u, g_s, None, None, None, None, None, None, g_nu
+        return g_u, g_s, None, None, None, None, None, None, g_nu, None
 
 
 def _mse_target(target, u, steps):
@@ -622,7 +713,7 @@ class BurgersSolverTorch:
         L.require_gpu(u)
         return _BurgersRHS.apply(u, self, L.INC_PART_DIFFUSION, self.t, _nu_input(self))
 
-    def _advance(self, u, dt, steps=1, want_traj=False):
+    def _advance(self, u, dt, steps=1, want_traj=False, checkpoint_every=None):
         L.require_gpu(u)
         if self._team_auto:
             self._team_auto = False
@@ -631,7 +722,8 @@ class BurgersSolverTorch:
         s = _burgers_correction(self.correction_s, u)
         nu = _nu_input(self)
         need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad) or nu is not None)
-        return _BurgersAdvance.apply(u, s, self, dt, steps, self.t, want_traj, need, nu)
+        return _BurgersAdvance.apply(u, s, self, dt, steps, self.t, want_traj, need, nu,
+                                     _checkpoint_k(checkpoint_every))
 
     def step_adaptive(self, u):
         """Sub-step to CFL * dx / max|u| over one dt (solver_1d.py:148-179):
@@ -680,17 +772,23 @@ class BurgersSolverTorch:
             self.LOG.info(f"CFL = {torch.max(torch.abs(u)) * self.dt / self.dx:.2f}, time_step: {self.dt}")
         return u_new
 
-    def rollout(self, u, mstep):
+    def rollout(self, u, mstep, checkpoint_every=None):
         """``mstep`` steps in one launch with the correction held fixed;
         returns the [B, mstep, N] trajectory (the fused path of
         simulate_rollout).  With adaptive_CFL this needs adaptive_mode
-        'trajectory' (the batch mode reads max|u| back every sub-step)."""
+        'trajectory' (the batch mode reads max|u| back every sub-step).
+        ``checkpoint_every=k`` (fixed time step): the forward keeps no stage
+        states and the backward recomputes them k steps at a time from the
+        trajectory, so the adjoint's memory is O(k) steps of stages instead
+        of O(mstep)."""
         if self.adaptive_CFL is not None:
             if self.adaptive_mode != "trajectory":
                 raise ValueError("fused rollout with adaptive_CFL needs adaptive_mode='trajectory'")
+            if checkpoint_every:
+                raise ValueError("checkpoint_every needs a fixed time step; adaptive_CFL is set")
             traj = self._advance_adaptive(u, steps=mstep, want_traj=True)
         else:
-            traj = self._advance(u, self.dt, steps=mstep, want_traj=True)
+            traj = self._advance(u, self.dt, steps=mstep, want_traj=True, checkpoint_every=checkpoint_every)
         for _ in range(mstep):
             self.t += self.dt
         return traj
@@ -793,7 +891,7 @@ _KS_NS = {L.INC_ETD1: 1, L.INC_ETRK2: 2, L.INC_ETDRK4: 4}
 
 class _KSAdvance(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u, s, solver, scheme, substeps, steps, domain_size, want_traj, need_stages):
+    def forward(ctx, u, s, solver, scheme, substeps, steps, domain_size, want_traj, need_stages, ckpt):
         u = u.contiguous()
         s = s.contiguous() if s is not None else None
         b, n = u.shape
@@ -801,16 +899,41 @@ class _KSAdvance(torch.autograd.Function):
         out = torch.empty_like(u)
         traj = u.new_empty(b, steps, n) if want_traj else None
         nst = _KS_NS[scheme] * substeps
-        stages = u.new_empty(b, steps, nst, n) if need_stages else None
+        # checkpoint every ckpt steps: no stage states now, recomputed per segment in backward
+        ctx.ckpt = ckpt if (ckpt and need_stages and want_traj and ckpt < steps) else 0
+        stages = u.new_empty(b, steps, nst, n) if need_stages and not ctx.ckpt else None
         rc = L.load().inc_ks_fwd(ctypes.byref(desc), L.ptr(u), L.ptr(s), L.ptr(out), steps, L.ptr(traj),
                                  L.ptr(stages), L.stream())
         L.check(rc, "KSSolverTorch.step")
-        ctx.save_for_backward(stages)
+        if ctx.ckpt:
+            ctx.save_for_backward(u, s, traj)
+            ctx.nst = nst
+        else:
+            ctx.save_for_backward(stages)
         ctx.desc, ctx.keep, ctx.steps, ctx.want_traj, ctx.shape = desc, keep, steps, want_traj, (b, n)
         return traj if want_traj else out
 
     @staticmethod
+    def _checkpointed_backward(ctx, g):
+        u0, s, traj = ctx.saved_tensors
+        lib, d = L.load(), ctypes.byref(ctx.desc)
+
+        def fwd(k0, start, out, m, stages):
+            rc = lib.inc_ks_fwd(d, L.ptr(start), L.ptr(s), L.ptr(out), m, None, L.ptr(stages), L.stream())
+            L.check(rc, "KSSolverTorch rollout (checkpoint recompute)")
+
+        def adj(k0, stages, g_seg, g_u, g_s, g_nu, m):
+            rc = lib.inc_ks_adj(d, L.ptr(stages), L.ptr(g_seg), L.ptr(g_u), L.ptr(g_s), m, L.stream())
+            L.check(rc, "KSSolverTorch rollout backward (checkpointed)")
+
+        g_u, g_s, _ = _checkpointed_adjoint(g.contiguous(), u0, traj, ctx.ckpt, ctx.nst, fwd, adj,
+                                            ctx.needs_input_grad[1])
+        return g_u, g_s, None, None, None, None, None, None, None, None
+
+    @staticmethod
     def backward(ctx, g):
+        if ctx.ckpt:
+            return _KSAdvance._checkpointed_backward(ctx, g)
         (stages,) = ctx.saved_tensors
         if stages is None:
             raise RuntimeError("KS step was run without saved stages; enable grad before the step")
@@ -827,7 +950,7 @@ class _KSAdvance(torch.autograd.Function):
         rc = L.load().inc_ks_adj(ctypes.byref(ctx.desc), L.ptr(stages), L.ptr(g_traj), L.ptr(g_u), L.ptr(g_s),
                                  ctx.steps, L.stream())
         L.check(rc, "KSSolverTorch.step backward")
-        return g_u, g_s, None, None, None, None, None, None, None
+        return g_u, g_s, None, None, None, None, None, None, None, None
 
 
 class _KSRolloutMSE(torch.autograd.Function):
@@ -988,7 +1111,7 @@ class KSSolverTorch:
         d.table_idx = idx.data_ptr() if idx is not None else None
         return d, [table, idx, uniq]
 
-    def _advance(self, u, domain_size, scheme, substeps, steps=1, want_traj=False):
+    def _advance(self, u, domain_size, scheme, substeps, steps=1, want_traj=False, checkpoint_every=None):
         L.require_gpu(u)
         if self._radix_auto:
             self._radix_auto = False
@@ -997,7 +1120,8 @@ class KSSolverTorch:
         if domain_size is None:
             domain_size = self.domain_size
         need = torch.is_grad_enabled() and (u.requires_grad or (s is not None and s.requires_grad))
-        return _KSAdvance.apply(u, s, self, scheme, substeps, steps, domain_size, want_traj, need)
+        return _KSAdvance.apply(u, s, self, scheme, substeps, steps, domain_size, want_traj, need,
+                                _checkpoint_k(checkpoint_every))
 
     def _run(self, u, domain_size, scheme):
         if self.correction_u is not None:
@@ -1024,13 +1148,17 @@ class KSSolverTorch:
     def etdrk4(self, u, domain_size):
         return self._run(u, domain_size, L.INC_ETDRK4)
 
-    def rollout(self, u, mstep, domain_size=None):
+    def rollout(self, u, mstep, domain_size=None, checkpoint_every=None):
         """``mstep`` step() calls in one launch with the correction held
-        fixed; returns the [B, mstep, N] trajectory."""
+        fixed; returns the [B, mstep, N] trajectory.  ``checkpoint_every=k``:
+        the forward keeps no stage states and the backward recomputes them
+        k steps at a time from the trajectory (adjoint memory O(k) steps of
+        stages instead of O(mstep))."""
         if self.correction_u is not None:
             raise ValueError("fused rollout does not apply correction_u; use step()")
         scheme = _KS_SCHEMES[self.time_scheme]
-        return self._advance(u, domain_size, scheme, 2 if self.doublestep else 1, steps=mstep, want_traj=True)
+        return self._advance(u, domain_size, scheme, 2 if self.doublestep else 1, steps=mstep, want_traj=True,
+                             checkpoint_every=checkpoint_every)
 
     def rollout_mse(self, u, target, domain_size=None):
         """Fused rollout over ``target.shape[1]`` step() calls with the MSE
