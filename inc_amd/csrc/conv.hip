@@ -957,18 +957,19 @@ int fill(int64_t n, T v, T* out, hipStream_t st) {
 // matrix-core kernel uses at most 512, the vector-ALU kernel 256).
 constexpr int kIncSlots = 512;
 
-// Workspace of the ragged INC rollout, in elements of the state's type.
-// forward: two activation buffers [C][total], the correction [total] and the
-// scaled last layer (C K + 1); adjoint: the recomputed activations and their
-// GELU derivatives ((L - 1) each), two cotangent buffers [C][total], four
-// [total] vectors, the scaled last layer and every layer's partial sums.
+// Workspace of an INC rollout over `tot` state values (the packed ragged
+// batch, or batch x n), in elements of the state's type.  forward: two
+// activation buffers [C][tot], the correction [tot] and the scaled last
+// layer (C K + 1); adjoint: the recomputed activations and their GELU
+// derivatives ((L - 1) each), two cotangent buffers [C][tot], four [tot]
+// vectors, the scaled last layer and every layer's partial sums.
 struct IncLayout {
   int64_t act[2], s, wl, y[8], dy[8], g[2], lam_s, lam, lam_sol, gx, part[8], total;
 };
 
-IncLayout inc_layout(const inc_ragged_desc* d, const inc_cnn_desc* cnn, bool adj) {
+IncLayout inc_layout(int64_t tot, const inc_cnn_desc* cnn, bool adj) {
   IncLayout L{};
-  const int64_t C = cnn->channels, tot = d->total, K = cnn->kernel;
+  const int64_t C = cnn->channels, K = cnn->kernel;
   int64_t o = 0;
   auto take = [&](int64_t n) {
     const int64_t at = o;
@@ -1000,45 +1001,106 @@ IncLayout inc_layout(const inc_ragged_desc* d, const inc_cnn_desc* cnn, bool adj
   return L;
 }
 
-int inc_check(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* r) {
-  INC_REQUIRE(d && cnn && r, INC_EINVAL, "null descriptor");
-  INC_REQUIRE(d->dtype == INC_F32 || d->dtype == INC_F64, INC_EINVAL, "unsupported dtype %d", d->dtype);
+int cnn_check(const inc_cnn_desc* cnn) {
   INC_REQUIRE(cnn->layers >= 2 && cnn->layers <= 8, INC_EUNSUPPORTED, "the network needs 2..8 layers (got %d)",
               cnn->layers);
   INC_REQUIRE(cnn->channels >= 1 && cnn->channels <= 64, INC_EUNSUPPORTED, "network width must be 1..64");
   INC_REQUIRE(cnn->kernel % 2 == 1 && cnn->kernel <= 9, INC_EUNSUPPORTED, "kernel size must be odd and <= 9");
-  INC_REQUIRE(2 * d->nmin >= cnn->kernel, INC_EUNSUPPORTED, "every trajectory needs >= K/2 points");
   for (int l = 0; l < cnn->layers; ++l) INC_REQUIRE(cnn->weight[l] && cnn->bias[l], INC_EINVAL, "missing layer %d", l);
+  return INC_OK;
+}
+
+int inc_check(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* r) {
+  INC_REQUIRE(d && cnn && r, INC_EINVAL, "null descriptor");
+  INC_REQUIRE(d->dtype == INC_F32 || d->dtype == INC_F64, INC_EINVAL, "unsupported dtype %d", d->dtype);
+  const int rc = cnn_check(cnn);
+  if (rc) return rc;
+  INC_REQUIRE(2 * d->nmin >= cnn->kernel, INC_EUNSUPPORTED, "every trajectory needs >= K/2 points");
   INC_REQUIRE(r->count == d->count, INC_EINVAL, "rows (%d) and trajectories (%d) differ", r->count, d->count);
   return rows_check(d->dtype, INC_ACT_GELU, r, 1, 1, cnn->kernel);
 }
 
-// The network on the state `in` for the rows of the first `jobs` tiles:
-// hidden activations to y[l] (with their GELU derivatives to dy[l] when dy
-// is given), the scaled correction to s.
+int ks_inc_check(const inc_ks_desc* d, const inc_cnn_desc* cnn) {
+  INC_REQUIRE(d && cnn, INC_EINVAL, "null descriptor");
+  INC_REQUIRE(d->dtype == INC_F32 || d->dtype == INC_F64, INC_EINVAL, "unsupported dtype %d", d->dtype);
+  INC_REQUIRE(d->batch >= 0 && d->substeps >= 1, INC_EINVAL, "bad KS descriptor");
+  const int rc = cnn_check(cnn);
+  if (rc) return rc;
+  INC_REQUIRE(2 * d->n >= cnn->kernel, INC_EUNSUPPORTED, "the resolution needs >= K/2 points");
+  return INC_OK;
+}
+
+int ks_stages(const inc_ks_desc* d) {
+  return (d->scheme == INC_ETD1 ? 1 : (d->scheme == INC_ETRK2 ? 2 : 4)) * d->substeps;
+}
+
+// Where the network runs at step k: the ragged rows (the first active[k]
+// tiles of rt, every tile when active is NULL) or a uniform batch of B rows
+// of n points (rt empty).
+struct NetRows {
+  RowTiles rt;
+  int B, n;
+  const int32_t* active;
+  int njobs;
+  int jobs(int k) const { return rt.tile ? (active ? active[k] : njobs) : 1; }
+};
+
+// The network on the state `in` at step k: hidden activations to y[l] (with
+// their GELU derivatives to dy[l] when dy is given), the scaled correction
+// to s.
 template <typename T>
-int inc_network(const inc_cnn_desc* cnn, const T* wl, const T* in, T* const* y, T* const* dy, T* s, RowTiles rt,
+int inc_network(const inc_cnn_desc* cnn, const T* wl, const T* in, T* const* y, T* const* dy, T* s, const NetRows& nr,
                 int jobs, hipStream_t st) {
   const int C = cnn->channels, K = cnn->kernel, nl = cnn->layers;
   const T* x = in;
   for (int l = 0; l < nl - 1; ++l) {
     const int ci = l == 0 ? 1 : C;
-    const int rc = conv_fwd_launch<T, false>(0, ci, C, 0, K, x, cnn->weight[l], cnn->bias[l], y[l], nullptr,
-                                             INC_ACT_GELU, dy ? dy[l] : nullptr, st, rt, jobs);
+    const int rc = conv_fwd_launch<T, false>(nr.B, ci, C, nr.n, K, x, cnn->weight[l], cnn->bias[l], y[l], nullptr,
+                                             INC_ACT_GELU, dy ? dy[l] : nullptr, st, nr.rt, jobs);
     if (rc) return rc;
     x = y[l];
   }
   if (s == nullptr) return INC_OK;  // the adjoint's recomputation needs the hidden layers only
-  return conv_fwd_launch<T, false>(0, C, 1, 0, K, x, wl, wl + C * K, s, nullptr, INC_ACT_NONE, nullptr, st, rt, jobs);
+  return conv_fwd_launch<T, false>(nr.B, C, 1, nr.n, K, x, wl, wl + C * K, s, nullptr, INC_ACT_NONE, nullptr, st,
+                                   nr.rt, jobs);
 }
 
-template <typename T>
-int inc_fwd(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* r, const int32_t* active,
-            const T* u0, int steps, T* traj, T* corr, T* stages, T* work, hipStream_t st) {
-  const IncLayout L = inc_layout(d, cnn, false);
+// The solver step of an INC rollout, one step at a time on step-major
+// buffers: the ragged Burgers batch (its forcing clock at step k) ...
+struct RaggedStep {
+  inc_ragged_desc d;
+  int64_t tot;
+  int ns;
+  int fwd(int k, const void* in, const void* s, void* out, void* stages, hipStream_t st) {
+    d.step0 = k;
+    return inc_burgers_ragged_fwd(&d, in, s, out, 1, nullptr, stages, st);
+  }
+  int adj(int k, const void* stages, const void* lam_in, void* lam_sol, void* lam_s, hipStream_t st) {
+    d.step0 = k;
+    return inc_burgers_ragged_adj(&d, stages, lam_in, lam_sol, lam_s, 1, st);
+  }
+};
+
+// ... or the KS batch ([batch][n] per step, stages [batch][nst][n]).
+struct KsStep {
+  const inc_ks_desc* d;
+  int64_t tot;
+  int ns;
+  int fwd(int, const void* in, const void* s, void* out, void* stages, hipStream_t st) {
+    return inc_ks_fwd(d, in, s, out, 1, nullptr, stages, st);
+  }
+  int adj(int, const void* stages, const void* lam_in, void* lam_sol, void* lam_s, hipStream_t st) {
+    return inc_ks_adj(d, stages, lam_in, lam_sol, lam_s, 1, st);
+  }
+};
+
+template <typename T, typename Step>
+int inc_fwd(Step& sv, const inc_cnn_desc* cnn, const NetRows& nr, const T* u0, int steps, T* traj, T* corr,
+            T* stages, T* work, hipStream_t st) {
+  const int64_t tot = sv.tot;
+  const IncLayout L = inc_layout(tot, cnn, false);
   const int C = cnn->channels, K = cnn->kernel, nl = cnn->layers;
-  const int64_t tot = d->total;
-  const int ns = d->scheme == INC_SSPRK3 ? 3 : 1;
+  const int ns = sv.ns;
   T* wl = work + L.wl;
   int rc = lincomb<T>(int64_t(C) * K, T(cnn->scale), static_cast<const T*>(cnn->weight[nl - 1]), nullptr, nullptr,
                       wl, st);
@@ -1048,31 +1110,29 @@ int inc_fwd(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_ro
   if (rc) return rc;
   T* bufs[8];
   for (int l = 0; l < nl - 1; ++l) bufs[l] = work + L.act[l & 1];
-  const RowTiles rt = row_tiles(r);
-  inc_ragged_desc dk = *d;
   const T* in = u0;
   for (int k = 0; k < steps; ++k) {
-    const int jobs = active ? active[k] : r->njobs;
+    const int jobs = nr.jobs(k);
     T* s = corr ? corr + int64_t(k) * tot : work + L.s;
-    rc = inc_network<T>(cnn, wl, in, bufs, nullptr, s, rt, jobs, st);
+    rc = inc_network<T>(cnn, wl, in, bufs, nullptr, s, nr, jobs, st);
     if (rc) return rc;
-    dk.step0 = k;
     T* out = traj + int64_t(k) * tot;
-    rc = inc_burgers_ragged_fwd(&dk, in, s, out, 1, nullptr, stages ? stages + int64_t(k) * ns * tot : nullptr, st);
+    rc = sv.fwd(k, in, s, out, stages ? stages + int64_t(k) * ns * tot : nullptr, st);
     if (rc) return rc;
     in = out;
   }
   return INC_OK;
 }
 
-template <typename T>
-int inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* r, const int32_t* active,
-            const T* u0, const T* traj, const T* stages, const T* g_traj, T* g_u0, T* g_w, int steps, T* work,
-            hipStream_t st) {
-  const IncLayout L = inc_layout(d, cnn, true);
+// g_corr (optional, [steps][tot]): cotangent of the corrections, added to
+// the solver's cotangent of each step's correction.
+template <typename T, typename Step>
+int inc_adj(Step& sv, const inc_cnn_desc* cnn, const NetRows& nr, const T* u0, const T* traj, const T* stages,
+            const T* g_traj, const T* g_corr, T* g_u0, T* g_w, int steps, T* work, hipStream_t st) {
+  const int64_t tot = sv.tot;
+  const IncLayout L = inc_layout(tot, cnn, true);
   const int C = cnn->channels, K = cnn->kernel, nl = cnn->layers;
-  const int64_t tot = d->total;
-  const int ns = d->scheme == INC_SSPRK3 ? 3 : 1;
+  const int ns = sv.ns;
   T* wl = work + L.wl;
   int rc = lincomb<T>(int64_t(C) * K, T(cnn->scale), static_cast<const T*>(cnn->weight[nl - 1]), nullptr, nullptr,
                       wl, st);
@@ -1094,48 +1154,48 @@ int inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_ro
   T* lam_sol = work + L.lam_sol;
   T* gx = work + L.gx;
   T* g[2] = {work + L.g[0], work + L.g[1]};
-  const RowTiles rt = row_tiles(r);
+  const RowTiles rt = nr.rt;
+  const int B = nr.B, n = nr.n;
   int nparts[8] = {};
   int64_t width[8];
   for (int l = 0; l < nl; ++l) {
     const int ci = l == 0 ? 1 : C, co = l == nl - 1 ? 1 : C;
     width[l] = int64_t(co) * ci * K + co;
   }
-  inc_ragged_desc dk = *d;
   const T* lam_in = g_traj + int64_t(steps - 1) * tot;  // cotangent of the last state
   for (int k = steps - 1; k >= 0; --k) {
-    const int jobs = active ? active[k] : r->njobs;
+    const int jobs = nr.jobs(k);
     const T* in = k == 0 ? u0 : traj + int64_t(k - 1) * tot;
     // the solver step's transpose: cotangents of its input state and of s
-    dk.step0 = k;
-    rc = inc_burgers_ragged_adj(&dk, stages + int64_t(k) * ns * tot, lam_in, lam_sol, lam_s, 1, st);
+    rc = sv.adj(k, stages + int64_t(k) * ns * tot, lam_in, lam_sol, lam_s, st);
+    if (!rc && g_corr) rc = lincomb<T>(tot, T(1), lam_s, g_corr + int64_t(k) * tot, nullptr, lam_s, st);
     if (rc) return rc;
     // the network again at the step's input state, with its GELU derivatives
-    rc = inc_network<T>(cnn, wl, in, y, dy, nullptr, rt, jobs, st);
+    rc = inc_network<T>(cnn, wl, in, y, dy, nullptr, nr, jobs, st);
     if (rc) return rc;
     if (jobs > 0) {
       // last layer (C -> 1): weight gradient of lam_s x y[L-2] (scaled at the end), input cotangent
       // scale W^T lam_s times the GELU derivative
       WgradMode md{rt, jobs, true, false, &nparts[nl - 1]};
-      rc = conv_wgrad<T>(0, C, 1, 0, K, lam_s, y[nl - 2], nullptr, nullptr, work + L.part[nl - 1],
+      rc = conv_wgrad<T>(B, C, 1, n, K, lam_s, y[nl - 2], nullptr, nullptr, work + L.part[nl - 1],
                          kIncSlots * width[nl - 1], st, md);
-      if (!rc) rc = conv_fwd_launch<T, true>(0, 1, C, 0, K, lam_s, wl, nullptr, g[0], dy[nl - 2], INC_ACT_NONE,
+      if (!rc) rc = conv_fwd_launch<T, true>(B, 1, C, n, K, lam_s, wl, nullptr, g[0], dy[nl - 2], INC_ACT_NONE,
                                              nullptr, st, rt, jobs);
       int cur = 0;
       for (int l = nl - 2; l >= 1 && !rc; --l) {
         WgradMode mh{rt, jobs, true, false, &nparts[l]};
-        rc = conv_wgrad<T>(0, C, C, 0, K, g[cur], y[l - 1], nullptr, nullptr, work + L.part[l],
+        rc = conv_wgrad<T>(B, C, C, n, K, g[cur], y[l - 1], nullptr, nullptr, work + L.part[l],
                            kIncSlots * width[l], st, mh);
-        if (!rc) rc = conv_fwd_launch<T, true>(0, C, C, 0, K, g[cur], cnn->weight[l], nullptr, g[cur ^ 1], dy[l - 1],
+        if (!rc) rc = conv_fwd_launch<T, true>(B, C, C, n, K, g[cur], cnn->weight[l], nullptr, g[cur ^ 1], dy[l - 1],
                                                INC_ACT_NONE, nullptr, st, rt, jobs);
         cur ^= 1;
       }
       if (!rc) {
         WgradMode m0{rt, jobs, true, false, &nparts[0]};
-        rc = conv_wgrad<T>(0, 1, C, 0, K, g[cur], in, nullptr, nullptr, work + L.part[0], kIncSlots * width[0], st,
+        rc = conv_wgrad<T>(B, 1, C, n, K, g[cur], in, nullptr, nullptr, work + L.part[0], kIncSlots * width[0], st,
                            m0);
       }
-      if (!rc) rc = conv_fwd_launch<T, true>(0, C, 1, 0, K, g[cur], cnn->weight[0], nullptr, gx, nullptr,
+      if (!rc) rc = conv_fwd_launch<T, true>(B, C, 1, n, K, g[cur], cnn->weight[0], nullptr, gx, nullptr,
                                              INC_ACT_NONE, nullptr, st, rt, jobs);
       if (rc) return rc;
     }
@@ -1315,7 +1375,7 @@ int inc_ragged_inc_workspace(const inc_ragged_desc* d, const inc_cnn_desc* cnn, 
   INC_REQUIRE(d && cnn && elems, INC_EINVAL, "null argument");
   INC_REQUIRE(cnn->layers >= 2 && cnn->layers <= 8 && cnn->channels >= 1 && cnn->kernel >= 1, INC_EINVAL,
               "bad network description");
-  *elems = inc::inc_layout(d, cnn, adjoint != 0).total;
+  *elems = inc::inc_layout(d->total, cnn, adjoint != 0).total;
   return INC_OK;
 }
 
@@ -1327,16 +1387,17 @@ int inc_ragged_inc_fwd(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const 
   if (rc) return rc;
   INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
   INC_REQUIRE(u0 && traj && work, INC_EINVAL, "u0, traj and work are required");
-  INC_REQUIRE(work_elems >= inc::inc_layout(d, cnn, false).total, INC_EINVAL, "workspace too small");
+  INC_REQUIRE(work_elems >= inc::inc_layout(d->total, cnn, false).total, INC_EINVAL, "workspace too small");
   if (d->count == 0) return INC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  inc::RaggedStep sv{*d, d->total, d->scheme == INC_SSPRK3 ? 3 : 1};
+  const inc::NetRows nr{inc::row_tiles(rows), 0, 0, active_tiles, rows->njobs};
   if (d->dtype == INC_F32)
-    return inc::inc_fwd<float>(d, cnn, rows, active_tiles, static_cast<const float*>(u0), steps,
-                               static_cast<float*>(traj), static_cast<float*>(corr), static_cast<float*>(stages),
-                               static_cast<float*>(work), st);
-  return inc::inc_fwd<double>(d, cnn, rows, active_tiles, static_cast<const double*>(u0), steps,
-                              static_cast<double*>(traj), static_cast<double*>(corr), static_cast<double*>(stages),
-                              static_cast<double*>(work), st);
+    return inc::inc_fwd<float>(sv, cnn, nr, static_cast<const float*>(u0), steps, static_cast<float*>(traj),
+                               static_cast<float*>(corr), static_cast<float*>(stages), static_cast<float*>(work), st);
+  return inc::inc_fwd<double>(sv, cnn, nr, static_cast<const double*>(u0), steps, static_cast<double*>(traj),
+                              static_cast<double*>(corr), static_cast<double*>(stages), static_cast<double*>(work),
+                              st);
 }
 
 int inc_ragged_inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* rows,
@@ -1348,17 +1409,74 @@ int inc_ragged_inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const 
   if (rc) return rc;
   INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
   INC_REQUIRE(u0 && traj && stages && g_traj && g_u0 && g_w && work, INC_EINVAL, "missing buffer");
-  INC_REQUIRE(work_elems >= inc::inc_layout(d, cnn, true).total, INC_EINVAL, "workspace too small");
+  INC_REQUIRE(work_elems >= inc::inc_layout(d->total, cnn, true).total, INC_EINVAL, "workspace too small");
   if (d->count == 0) return INC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  inc::RaggedStep sv{*d, d->total, d->scheme == INC_SSPRK3 ? 3 : 1};
+  const inc::NetRows nr{inc::row_tiles(rows), 0, 0, active_tiles, rows->njobs};
   if (d->dtype == INC_F32)
-    return inc::inc_adj<float>(d, cnn, rows, active_tiles, static_cast<const float*>(u0),
-                               static_cast<const float*>(traj), static_cast<const float*>(stages),
-                               static_cast<const float*>(g_traj), static_cast<float*>(g_u0), static_cast<float*>(g_w),
+    return inc::inc_adj<float>(sv, cnn, nr, static_cast<const float*>(u0), static_cast<const float*>(traj),
+                               static_cast<const float*>(stages), static_cast<const float*>(g_traj), nullptr,
+                               static_cast<float*>(g_u0), static_cast<float*>(g_w), steps, static_cast<float*>(work),
+                               st);
+  return inc::inc_adj<double>(sv, cnn, nr, static_cast<const double*>(u0), static_cast<const double*>(traj),
+                              static_cast<const double*>(stages), static_cast<const double*>(g_traj), nullptr,
+                              static_cast<double*>(g_u0), static_cast<double*>(g_w), steps,
+                              static_cast<double*>(work), st);
+}
+
+int inc_ks_inc_workspace(const inc_ks_desc* d, const inc_cnn_desc* cnn, int adjoint, int64_t* elems) {
+  inc::clear_error();
+  INC_REQUIRE(d && cnn && elems, INC_EINVAL, "null argument");
+  INC_REQUIRE(cnn->layers >= 2 && cnn->layers <= 8 && cnn->channels >= 1 && cnn->kernel >= 1, INC_EINVAL,
+              "bad network description");
+  *elems = inc::inc_layout(int64_t(d->batch) * d->n, cnn, adjoint != 0).total;
+  return INC_OK;
+}
+
+int inc_ks_inc_fwd(const inc_ks_desc* d, const inc_cnn_desc* cnn, const void* u0, int steps, void* traj, void* corr,
+                   void* stages, void* work, int64_t work_elems, void* stream) {
+  inc::clear_error();
+  int rc = inc::ks_inc_check(d, cnn);
+  if (rc) return rc;
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(u0 && traj && work, INC_EINVAL, "u0, traj and work are required");
+  const int64_t tot = int64_t(d->batch) * d->n;
+  INC_REQUIRE(work_elems >= inc::inc_layout(tot, cnn, false).total, INC_EINVAL, "workspace too small");
+  if (d->batch == 0) return INC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  inc::KsStep sv{d, tot, inc::ks_stages(d)};
+  const inc::NetRows nr{inc::RowTiles{}, d->batch, d->n, nullptr, 0};
+  if (d->dtype == INC_F32)
+    return inc::inc_fwd<float>(sv, cnn, nr, static_cast<const float*>(u0), steps, static_cast<float*>(traj),
+                               static_cast<float*>(corr), static_cast<float*>(stages), static_cast<float*>(work), st);
+  return inc::inc_fwd<double>(sv, cnn, nr, static_cast<const double*>(u0), steps, static_cast<double*>(traj),
+                              static_cast<double*>(corr), static_cast<double*>(stages), static_cast<double*>(work),
+                              st);
+}
+
+int inc_ks_inc_adj(const inc_ks_desc* d, const inc_cnn_desc* cnn, const void* u0, const void* traj,
+                   const void* stages, const void* g_traj, const void* g_corr, void* g_u0, void* g_w, int steps,
+                   void* work, int64_t work_elems, void* stream) {
+  inc::clear_error();
+  int rc = inc::ks_inc_check(d, cnn);
+  if (rc) return rc;
+  INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
+  INC_REQUIRE(u0 && traj && stages && g_traj && g_u0 && g_w && work, INC_EINVAL, "missing buffer");
+  const int64_t tot = int64_t(d->batch) * d->n;
+  INC_REQUIRE(work_elems >= inc::inc_layout(tot, cnn, true).total, INC_EINVAL, "workspace too small");
+  if (d->batch == 0) return INC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  inc::KsStep sv{d, tot, inc::ks_stages(d)};
+  const inc::NetRows nr{inc::RowTiles{}, d->batch, d->n, nullptr, 0};
+  if (d->dtype == INC_F32)
+    return inc::inc_adj<float>(sv, cnn, nr, static_cast<const float*>(u0), static_cast<const float*>(traj),
+                               static_cast<const float*>(stages), static_cast<const float*>(g_traj),
+                               static_cast<const float*>(g_corr), static_cast<float*>(g_u0), static_cast<float*>(g_w),
                                steps, static_cast<float*>(work), st);
-  return inc::inc_adj<double>(d, cnn, rows, active_tiles, static_cast<const double*>(u0),
-                              static_cast<const double*>(traj), static_cast<const double*>(stages),
-                              static_cast<const double*>(g_traj), static_cast<double*>(g_u0),
+  return inc::inc_adj<double>(sv, cnn, nr, static_cast<const double*>(u0), static_cast<const double*>(traj),
+                              static_cast<const double*>(stages), static_cast<const double*>(g_traj),
+                              static_cast<const double*>(g_corr), static_cast<double*>(g_u0),
                               static_cast<double*>(g_w), steps, static_cast<double*>(work), st);
 }
 

@@ -11,6 +11,9 @@ as the reference does.  Two MI355X additions:
   kernel runs, the correction network is evaluated inside that one launch
   as well, and with a gradient wanted (training) its transpose inside the
   one adjoint launch (float32, width 32, up to 4 layers, N <= 128);
+* with ``correction_term='INC'``, a KS solver and a CorrectionCNN, the
+  rollout is one library call (the network on this library's convolution
+  kernels between the ETD steps) and its gradient one more;
 * ``checkpoint_every=k`` recomputes the INC rollout in segments of k steps
   during the backward pass (segment-start states are the only activations
   kept), for long unrolled horizons.
@@ -38,6 +41,14 @@ def _fusable(solver, correction_term):
     if isinstance(solver, KSSolverTorch):
         return solver.correction_u is None
     return False
+
+
+def _fusable_ks_inc(model, solver, u):
+    """A KS INC rollout the library runs in one call (inc_ks_inc_fwd / _adj):
+    a CorrectionCNN its convolution kernels run, no correction_u."""
+    return (isinstance(solver, KSSolverTorch) and isinstance(model, CorrectionCNN) and u.is_cuda
+            and solver.correction_u is None and model.ragged_desc(u.dtype) is not None
+            and 2 * u.shape[-1] >= model.convs[0].k)
 
 
 def _fusable_inc(model, solver, u):
@@ -71,6 +82,11 @@ def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, c
         return empty, (empty.clone() if correction_term else None)
     if fused and correction_term == "INC" and _fusable_inc(model, solver, u_initial):
         return solver.rollout_inc(model, u_initial, mstep)
+    # KS INC with a CorrectionCNN: one call per direction; its adjoint keeps
+    # no activations (the network is evaluated again per step), so it needs
+    # no checkpoint segments
+    if fused and correction_term == "INC" and _fusable_ks_inc(model, solver, u_initial):
+        return solver.rollout_inc(model, u_initial, mstep, L)
     # checkpoint_every=k on a fused rollout: its adjoint recomputes the stage
     # states k steps at a time instead of keeping them for the whole rollout
     # (fixed time step; the adaptive rollout keeps the step loop for it)

@@ -474,6 +474,33 @@ int inc_ragged_inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const 
                        const void* g_traj, void* g_u0, void* g_w, int steps, void* work, int64_t work_elems,
                        void* stream);
 
+/*
+ * INC training rollout of a KS batch in one call: simulate_rollout(model,
+ * solver, u0, steps, 'INC', L) (trainer/trainer_1d.py:154-183) with a
+ * KSSolverTorch (solvers/solver_1d.py:198-288) and the network of
+ * inc_ragged_inc_fwd (inc_cnn_desc: circular Conv1d layers with GELU between
+ * them, s = scale * net(u)).  Step k evaluates the network on the batch's
+ * state (one convolution launch per layer over batch x n) and runs one
+ * inc_ks_fwd step with s as its correction.  Step-major buffers:
+ *   traj   [steps, batch, n]           states after each step
+ *   corr   [steps, batch, n] or NULL   corrections of each step
+ *   stages [steps, batch, nst, n]      stage states (nst as for inc_ks_fwd),
+ *                                      or NULL when no adjoint follows
+ * inc_ks_inc_adj is its adjoint, as inc_ragged_inc_adj: from u0, traj,
+ * stages, the trajectory cotangent g_traj [steps, batch, n] and the
+ * corrections' cotangent g_corr [steps, batch, n] (or NULL) it writes g_u0
+ * [batch, n] and the parameter gradient g_w [P] = [w0, b0, w1, b1, ...]
+ * summed over the batch and the steps in a fixed order.  The network is
+ * evaluated again at every step's input state.  work: workspace of at least
+ * what inc_ks_inc_workspace reports (adjoint = 0 forward, 1 adjoint).
+ */
+int inc_ks_inc_workspace(const inc_ks_desc* d, const inc_cnn_desc* cnn, int adjoint, int64_t* elems);
+int inc_ks_inc_fwd(const inc_ks_desc* d, const inc_cnn_desc* cnn, const void* u0, int steps, void* traj, void* corr,
+                   void* stages, void* work, int64_t work_elems, void* stream);
+int inc_ks_inc_adj(const inc_ks_desc* d, const inc_cnn_desc* cnn, const void* u0, const void* traj,
+                   const void* stages, const void* g_traj, const void* g_corr, void* g_u0, void* g_w, int steps,
+                   void* work, int64_t work_elems, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
