@@ -352,6 +352,28 @@ def extra_ks_inc(step_path=True):
                          "flops_per_launch": flops}}
 
 
+def extra_resnet(shapes=((512, 128), (2048, 256))):
+    """The reference's ResNet1D correction network (6 residual blocks, 32
+    features) forward + backward, on this library's convolutions against
+    the same module run by torch (torch.nn.Conv1d)."""
+    import torch
+    from inc_amd import ResNet1D
+    out = {"workload": "ResNet1D(32 features, 6 blocks) forward+backward, fp32"}
+    for b, n in shapes:
+        torch.manual_seed(0)
+        hip = ResNet1D().cuda()
+        ref = ResNet1D(backend="torch").cuda()
+        ref.load_state_dict(hip.state_dict())
+        u = torch.randn(b, n, device="cuda")
+
+        def step(m):
+            m(u, 2 * math.pi).square().sum().backward()
+        t_hip = _timed(lambda: step(hip), reps=10)
+        t_ref = _timed(lambda: step(ref), reps=10)
+        out[f"B{b}_N{n}"] = {"ms": t_hip * 1e3, "torch_conv1d_ms": t_ref * 1e3, "speedup": t_ref / t_hip}
+    return out
+
+
 MFMA_F32_PEAK_TFS = 157.3  # dense fp32 matrix peak
 
 
@@ -871,7 +893,7 @@ def main():
                                   "ks_checkpoint": extra_ks_checkpoint(),
                                   "burgers_saturated": extra_burgers_saturation(), "burgers_inc": extra_burgers_inc(),
                                   "burgers_inc_eval": extra_burgers_inc_eval(),
-                                  "ks_inc": extra_ks_inc()}
+                                  "ks_inc": extra_ks_inc(), "resnet1d": extra_resnet()}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_baselines"] = cpu_baselines()

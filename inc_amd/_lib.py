@@ -20,7 +20,7 @@ INC_SRC_NONE, INC_SRC_LINEAR, INC_SRC_LEARNING = 0, 1, 2
 INC_PART_CONVECTION, INC_PART_DIFFUSION, INC_PART_SOURCE = 1, 2, 4
 INC_ETD1, INC_ETRK2, INC_ETDRK4 = 0, 1, 2
 INC_KS_TABLE_STRIDE = 8
-INC_ACT_NONE, INC_ACT_GELU = 0, 1
+INC_ACT_NONE, INC_ACT_GELU, INC_ACT_RELU = 0, 1, 2
 
 _vp = ctypes.c_void_p
 

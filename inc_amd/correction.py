@@ -236,3 +236,103 @@ class CorrectionCNN(nn.Module):
                     x = self.act(x)
             s = x.squeeze(1)
         return s if self.scale == 1.0 else self.scale * s
+
+
+class _CircConv1dAct(torch.autograd.Function):
+    """act(conv(x) + b) as one autograd node: the convolution's epilogue
+    writes act'(v) beside the output (none for act = NONE), and the backward
+    multiplies the output's gradient by it before the convolution's input
+    and weight gradients."""
+
+    @staticmethod
+    def forward(ctx, x, w, b, act):
+        x, w = x.contiguous(), w.contiguous()
+        b = b.contiguous() if b is not None else None
+        y, dy = _conv_fwd(x, w, b, act, need_dy=any(ctx.needs_input_grad[:3]))
+        ctx.save_for_backward(x, w, dy)
+        ctx.has_bias = b is not None
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        x, w, dy = ctx.saved_tensors
+        g = g.contiguous()
+        if dy is not None:
+            g = g * dy
+        gx = gw = gb = None
+        if ctx.needs_input_grad[0]:
+            gx = _conv_bwd_data(g, w, x.shape[1])
+        if ctx.needs_input_grad[1] or (ctx.has_bias and ctx.needs_input_grad[2]):
+            gw, gb = _conv_bwd_weight(g, x, w, ctx.has_bias)
+        return gx, gw, gb, None
+
+
+def _conv_act(x, conv, act):
+    """A circular nn.Conv1d module's weights through this library's kernels
+    ('same' periodic padding: padding = K // 2, as the modules are built)."""
+    _lib.require_like(x, conv.weight, conv.bias, what="Conv1d weight/bias")
+    return _CircConv1dAct.apply(x, conv.weight, conv.bias, act)
+
+
+def get_grid(x, L):
+    """Grid coordinates [B, D] of models/models_1d/utils.py:3-30: linspace(0,
+    L, D) for a number L, L_b * linspace(0, 1, D) for a tensor of one L per
+    trajectory."""
+    b, d = x.shape[:2]
+    if isinstance(L, (float, int)):
+        return torch.linspace(0, L, d, dtype=x.dtype, device=x.device).expand(b, d)
+    if isinstance(L, torch.Tensor):
+        return L.to(dtype=x.dtype, device=x.device).reshape(b, 1) * torch.linspace(0, 1, d, dtype=x.dtype,
+                                                                                   device=x.device)
+    raise ValueError("L must be a float, int, or a torch.Tensor")
+
+
+class _ResidualBlock1D(nn.Module):
+    """relu(conv2(relu(conv1(x))) + shortcut(x)), 5-tap circular convolutions
+    (models/models_1d/resnet.py:5-20, the same submodules and parameter
+    names)."""
+
+    def __init__(self, in_channels, out_channels):
+        super().__init__()
+        self.conv1 = nn.Conv1d(in_channels, out_channels, kernel_size=5, padding=2, padding_mode="circular")
+        self.conv2 = nn.Conv1d(out_channels, out_channels, kernel_size=5, padding=2, padding_mode="circular")
+        self.relu = nn.ReLU(inplace=True)
+        self.shortcut = nn.Conv1d(in_channels, out_channels, 1) if in_channels != out_channels else nn.Identity()
+
+
+class ResNet1D(nn.Module):
+    """The reference's ResNet1D correction network (models/models_1d/resnet.py:22-52):
+    the state and a grid channel -> circular 5-tap conv + ReLU -> residual
+    blocks -> 1x1 conv, returning [B, D].  The module tree, parameter names
+    and initialisation (torch's nn.Conv1d defaults, created in the same
+    order) are the reference's, so its state dicts load and a seeded
+    construction gives the same weights.  With ``backend='hip'`` (default)
+    every convolution runs in this library's kernels, ReLU fused into the
+    convolution that produces it; ``backend='torch'`` runs the modules
+    themselves."""
+
+    def __init__(self, L=16, in_channels=1 + 1, out_channels=1, init_features=32, num_blocks=6, backend="hip"):
+        super().__init__()
+        self.L = L
+        self.backend = backend
+        f = init_features
+        self.initial = nn.Sequential(nn.Conv1d(in_channels, f, kernel_size=5, padding=2, padding_mode="circular"),
+                                     nn.ReLU(inplace=True))
+        self.blocks = nn.Sequential(*[_ResidualBlock1D(f, f) for _ in range(num_blocks)])
+        self.final = nn.Conv1d(f, out_channels, kernel_size=1)
+
+    def forward(self, x, L=None):
+        L = L if L is not None else self.L
+        h = torch.stack((x, get_grid(x, L)), 1)  # [B, 2, D]
+        if self.backend != "hip":
+            h = self.initial(h)
+            for blk in self.blocks:
+                r = blk.shortcut(h)
+                h = blk.relu(blk.conv2(blk.relu(blk.conv1(h))) + r)
+            return self.final(h).squeeze(1)
+        relu, none = _lib.INC_ACT_RELU, _lib.INC_ACT_NONE
+        h = _conv_act(h, self.initial[0], relu)
+        for blk in self.blocks:
+            r = h if isinstance(blk.shortcut, nn.Identity) else _conv_act(h, blk.shortcut, none)
+            h = torch.relu(_conv_act(_conv_act(h, blk.conv1, relu), blk.conv2, none) + r)
+        return _conv_act(h, self.final, none).squeeze(1)

@@ -90,6 +90,17 @@ __device__ __forceinline__ T gelu_value(T v) {
   return v * (T(0.5) * (T(1) + erf(v * T(0.70710678118654752440))));
 }
 
+// ReLU as torch.relu: NaN stays NaN; the derivative is 1 where the output is
+// positive, 0 elsewhere (NaN and 0 included), as relu's backward masks it.
+template <typename T>
+__device__ __forceinline__ T relu(T v) {
+  return v < T(0) ? T(0) : v;
+}
+template <typename T>
+__device__ __forceinline__ T relu_grad(T v) {
+  return v > T(0) ? T(1) : T(0);
+}
+
 // float32: Phi(-|v|) = erfc(z) / 2 with z = |v| / sqrt 2 and
 // erfc(z) = exp(-z^2) t Q(t), t = 1 / (1 + p z), Q a degree-9 polynomial
 // least-squares fitted (relative weights) to erfc(z) exp(z^2) / t over
@@ -210,6 +221,9 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int Nu, c
           } else {
             v = gelu_value(v);
           }
+        } else if (act == INC_ACT_RELU) {
+          if (dy) dy[row + i] = relu_grad(v);
+          v = relu(v);
         }
         y[row + i] = v;
       }
@@ -741,6 +755,9 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
             } else {
               v = gelu_value(v);
             }
+          } else if (act == INC_ACT_RELU) {
+            if (dy) dy[at] = relu_grad(v);
+            v = relu(v);
           }
           y[at] = v;
         }
@@ -941,7 +958,8 @@ int conv_wgrad(int B, int Ci, int Co, int N, int K, const void* g, const void* x
 
 int conv_check(int dtype, int act, int B, int Ci, int Co, int N, int K) {
   INC_REQUIRE(dtype == INC_F32 || dtype == INC_F64, INC_EINVAL, "unsupported dtype %d", dtype);
-  INC_REQUIRE(act == INC_ACT_NONE || act == INC_ACT_GELU, INC_EINVAL, "unknown activation %d", act);
+  INC_REQUIRE(act == INC_ACT_NONE || act == INC_ACT_GELU || act == INC_ACT_RELU, INC_EINVAL, "unknown activation %d",
+              act);
   INC_REQUIRE(B >= 0 && Ci >= 1 && Co >= 1 && N >= 1, INC_EINVAL, "conv1d: bad sizes");
   INC_REQUIRE(K % 2 == 1 && K <= 9 && K <= N, INC_EUNSUPPORTED, "conv1d kernel size must be odd, <= 9 and <= n");
   return INC_OK;
@@ -950,7 +968,8 @@ int conv_check(int dtype, int act, int B, int Ci, int Co, int N, int K) {
 int rows_check(int dtype, int act, const inc_conv_rows* r, int Ci, int Co, int K) {
   INC_REQUIRE(r != nullptr, INC_EINVAL, "null rows descriptor");
   INC_REQUIRE(dtype == INC_F32 || dtype == INC_F64, INC_EINVAL, "unsupported dtype %d", dtype);
-  INC_REQUIRE(act == INC_ACT_NONE || act == INC_ACT_GELU, INC_EINVAL, "unknown activation %d", act);
+  INC_REQUIRE(act == INC_ACT_NONE || act == INC_ACT_GELU || act == INC_ACT_RELU, INC_EINVAL, "unknown activation %d",
+              act);
   INC_REQUIRE(r->count >= 0 && r->njobs >= 0 && Ci >= 1 && Co >= 1, INC_EINVAL, "conv1d rows: bad sizes");
   INC_REQUIRE(K % 2 == 1 && K <= 9, INC_EUNSUPPORTED, "conv1d kernel size must be odd and <= 9");
   INC_REQUIRE(r->njobs == 0 || (r->tile && r->n && r->offset), INC_EINVAL, "conv1d rows: tile, n, offset missing");

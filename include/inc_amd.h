@@ -400,11 +400,13 @@ int inc_conv1d_bwd_weight_noalloc(int dtype, int B, int Ci, int Co, int n, int K
  * with v = conv(x) + bias, y = act(v) and dy = act'(v) [B, Co, n].  y feeds the
  * next layer (and its weight gradient); dy is what the input gradient of that
  * next layer multiplies by (inc_conv1d_bwd_data_mul).  act: INC_ACT_NONE (dy
- * unused, the plain inc_conv1d_fwd) or INC_ACT_GELU (exact erf form, as
- * torch.nn.GELU()).  dy may be NULL when no gradient is wanted: y alone is
- * written, without the derivative's exp and store.
+ * unused, the plain inc_conv1d_fwd), INC_ACT_GELU (exact erf form, as
+ * torch.nn.GELU()) or INC_ACT_RELU (torch.relu, dy = 1 where y > 0 else 0:
+ * ResNet1D's and UNet1D's activation, models/models_1d/resnet.py:5-20).  dy
+ * may be NULL when no gradient is wanted: y alone is written, without the
+ * derivative's evaluation and store.
  */
-enum { INC_ACT_NONE = 0, INC_ACT_GELU = 1 };
+enum { INC_ACT_NONE = 0, INC_ACT_GELU = 1, INC_ACT_RELU = 2 };
 int inc_conv1d_fwd_act(int dtype, int act, int B, int Ci, int Co, int n, int K, const void* x,
                        const void* w, const void* bias, void* y, void* dy, void* stream);
 /* gx = (conv^T g) * m elementwise, m [B, Ci, n] (NULL: the plain inc_conv1d_bwd_data). */

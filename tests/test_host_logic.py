@@ -94,3 +94,24 @@ def test_ks_initial_conditions_structure_and_batch_norm():
         pass
     else:
         raise AssertionError("mismatched alpha / domain_size lengths must raise ValueError")
+
+
+def test_resnet1d_module_tree_is_the_reference_s():
+    """ResNet1D keeps models/models_1d/resnet.py:22-52's parameter names (its
+    state dicts load) and builds the same weights from a seed in either
+    backend; the torch backend runs on the CPU."""
+    from inc_amd.correction import ResNet1D
+    keys = (["initial.0.weight", "initial.0.bias"]
+            + [f"blocks.{i}.{c}.{p}" for i in range(6) for c in ("conv1", "conv2") for p in ("weight", "bias")]
+            + ["final.weight", "final.bias"])
+    assert list(ResNet1D().state_dict().keys()) == keys
+    torch.manual_seed(3)
+    a = ResNet1D()
+    torch.manual_seed(3)
+    b = ResNet1D(backend="torch")
+    assert all(torch.equal(p, q) for p, q in zip(a.parameters(), b.parameters()))
+    assert b(torch.randn(2, 32, dtype=torch.float32), 16.0).shape == (2, 32)
+    # a block whose widths differ gets the reference's 1x1 shortcut convolution
+    from inc_amd.correction import _ResidualBlock1D
+    assert list(_ResidualBlock1D(8, 16).state_dict().keys()) == [
+        "conv1.weight", "conv1.bias", "conv2.weight", "conv2.bias", "shortcut.weight", "shortcut.bias"]
