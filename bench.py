@@ -352,17 +352,20 @@ def extra_ks_inc(step_path=True):
                          "flops_per_launch": flops}}
 
 
-def extra_resnet(shapes=((512, 128), (2048, 256))):
-    """The reference's ResNet1D correction network (6 residual blocks, 32
-    features) forward + backward, on this library's convolutions against
-    the same module run by torch (torch.nn.Conv1d)."""
+def extra_reference_net(kind, shapes=((512, 128), (2048, 256))):
+    """One of the reference's correction networks (ResNet1D: 6 residual
+    blocks, 32 features; UNet1D: 32 initial features) forward + backward, on
+    this library's kernels against the same module run by torch
+    (torch.nn.Conv1d, MaxPool1d, ConvTranspose1d, cat)."""
     import torch
-    from inc_amd import ResNet1D
-    out = {"workload": "ResNet1D(32 features, 6 blocks) forward+backward, fp32"}
+    from inc_amd import ResNet1D, UNet1D
+    cls = {"resnet1d": ResNet1D, "unet1d": UNet1D}[kind]
+    name = "ResNet1D(32 features, 6 blocks)" if kind == "resnet1d" else "UNet1D(32 initial features)"
+    out = {"workload": f"{name} forward+backward, fp32"}
     for b, n in shapes:
         torch.manual_seed(0)
-        hip = ResNet1D().cuda()
-        ref = ResNet1D(backend="torch").cuda()
+        hip = cls().cuda()
+        ref = cls(backend="torch").cuda()
         ref.load_state_dict(hip.state_dict())
         u = torch.randn(b, n, device="cuda")
 
@@ -370,7 +373,7 @@ def extra_resnet(shapes=((512, 128), (2048, 256))):
             m(u, 2 * math.pi).square().sum().backward()
         t_hip = _timed(lambda: step(hip), reps=10)
         t_ref = _timed(lambda: step(ref), reps=10)
-        out[f"B{b}_N{n}"] = {"ms": t_hip * 1e3, "torch_conv1d_ms": t_ref * 1e3, "speedup": t_ref / t_hip}
+        out[f"B{b}_N{n}"] = {"ms": t_hip * 1e3, "torch_ms": t_ref * 1e3, "speedup": t_ref / t_hip}
     return out
 
 
@@ -893,7 +896,8 @@ def main():
                                   "ks_checkpoint": extra_ks_checkpoint(),
                                   "burgers_saturated": extra_burgers_saturation(), "burgers_inc": extra_burgers_inc(),
                                   "burgers_inc_eval": extra_burgers_inc_eval(),
-                                  "ks_inc": extra_ks_inc(), "resnet1d": extra_resnet()}
+                                  "ks_inc": extra_ks_inc(), "resnet1d": extra_reference_net("resnet1d"),
+                                  "unet1d": extra_reference_net("unet1d")}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_baselines"] = cpu_baselines()

@@ -6,10 +6,10 @@ for gfx950 (libinc_amd.so, C ABI in include/inc_amd.h).
 """
 from .batched import RaggedBurgers
 from .config_1d import BGSimParams, KSSimParams
-from .correction import CorrectionCNN, ResNet1D
+from .correction import CorrectionCNN, ResNet1D, UNet1D
 from .rollout import simulate_rollout, simulate_rollout_mse
 from .solver_1d import BurgersSolverTorch, KSSolverTorch, weno_reconstruction
 
 __all__ = ["BurgersSolverTorch", "KSSolverTorch", "weno_reconstruction", "simulate_rollout",
-           "simulate_rollout_mse", "BGSimParams", "KSSimParams", "RaggedBurgers", "CorrectionCNN", "ResNet1D"]
+           "simulate_rollout_mse", "BGSimParams", "KSSimParams", "RaggedBurgers", "CorrectionCNN", "ResNet1D", "UNet1D"]
 __version__ = "0.1.0"

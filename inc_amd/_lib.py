@@ -135,6 +135,10 @@ SIGNATURES = {
                                      + [_vp] * 5),
     "inc_conv1d_rows_bwd_weight": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ConvRows)] + [ctypes.c_int] * 3
                                    + [_vp] * 5 + [ctypes.c_int64, _vp]),
+    "inc_maxpool2_fwd": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, _vp, _vp, _vp]),
+    "inc_maxpool2_bwd": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    "inc_upsample2_cat": (ctypes.c_int, [ctypes.c_int] * 5 + [_vp, _vp, _vp, _vp]),
+    "inc_upsample2_cat_bwd": (ctypes.c_int, [ctypes.c_int] * 5 + [_vp, _vp, _vp, _vp]),
     "inc_ragged_inc_workspace": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), ctypes.POINTER(CnnDesc), ctypes.c_int,
                                                 ctypes.POINTER(ctypes.c_int64)]),
     "inc_ragged_inc_fwd": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), ctypes.POINTER(CnnDesc),

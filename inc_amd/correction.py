@@ -336,3 +336,154 @@ class ResNet1D(nn.Module):
             r = h if isinstance(blk.shortcut, nn.Identity) else _conv_act(h, blk.shortcut, none)
             h = torch.relu(_conv_act(_conv_act(h, blk.conv1, relu), blk.conv2, none) + r)
         return _conv_act(h, self.final, none).squeeze(1)
+
+
+def _require_even(n, what):
+    if n % 2:
+        raise ValueError(f"{what}: the library's 2-to-1 max pooling takes an even number of points (got {n})")
+
+
+class _MaxPool2(torch.autograd.Function):
+    """torch.nn.MaxPool1d(2, 2) on [B, C, n] (n even) in this library's
+    kernels; the backward recomputes each pair's pick from the saved input."""
+
+    @staticmethod
+    def forward(ctx, x):
+        x = x.contiguous()
+        b, c, n = x.shape
+        _require_even(n, "max pooling")
+        y = x.new_empty(b, c, n // 2)
+        rc = _lib.load().inc_maxpool2_fwd(_lib.dtype_code(x.dtype), b * c, n, _lib.ptr(x), _lib.ptr(y), _lib.stream())
+        _lib.check(rc, "max pooling")
+        ctx.save_for_backward(x)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        (x,) = ctx.saved_tensors
+        gy = gy.contiguous()
+        b, c, n = x.shape
+        gx = torch.empty_like(x)
+        rc = _lib.load().inc_maxpool2_bwd(_lib.dtype_code(x.dtype), b * c, n, _lib.ptr(x), _lib.ptr(gy),
+                                          _lib.ptr(gx), _lib.stream())
+        _lib.check(rc, "max pooling gradient")
+        return gx
+
+
+def max_pool2(x):
+    """torch.nn.MaxPool1d(kernel_size=2, stride=2) of x [B, C, n], n even."""
+    _lib.require_gpu(x)
+    return _MaxPool2.apply(x)
+
+
+class _UpsampleCat(torch.autograd.Function):
+    """torch.cat((up, skip), 1) where up [B, C, 2n] interleaves the two
+    channel halves of y2 [B, 2C, n] (up[:, c, 2i + t] = y2[:, t C + c, i])."""
+
+    @staticmethod
+    def forward(ctx, y2, skip):
+        y2, skip = y2.contiguous(), skip.contiguous()
+        b, c2, n = y2.shape
+        c, cs = c2 // 2, skip.shape[1]
+        out = y2.new_empty(b, c + cs, 2 * n)
+        rc = _lib.load().inc_upsample2_cat(_lib.dtype_code(y2.dtype), b, c, n, cs, _lib.ptr(y2), _lib.ptr(skip),
+                                           _lib.ptr(out), _lib.stream())
+        _lib.check(rc, "upsampling concatenation")
+        ctx.dims = (b, c, n, cs)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        b, c, n, cs = ctx.dims
+        g = g.contiguous()
+        gy2 = g.new_empty(b, 2 * c, n)
+        gskip = g.new_empty(b, cs, 2 * n) if ctx.needs_input_grad[1] else None
+        rc = _lib.load().inc_upsample2_cat_bwd(_lib.dtype_code(g.dtype), b, c, n, cs, _lib.ptr(g), _lib.ptr(gy2),
+                                               _lib.ptr(gskip), _lib.stream())
+        _lib.check(rc, "upsampling concatenation gradient")
+        return gy2, gskip
+
+
+def upconv2_cat(x, up, skip):
+    """torch.cat((up(x), skip), 1) for up = nn.ConvTranspose1d(Ci, C, 2, stride=2):
+    the transposed convolution as a 1x1 convolution with 2C outputs
+    (W'[t C + c] = W[:, c, t], the bias repeated) on the matrix cores, then
+    one pass that interleaves its halves and appends skip's channels."""
+    ci, c, k = up.weight.shape
+    if k != 2 or up.stride != (2,) or up.padding != (0,) or up.output_padding != (0,) or up.dilation != (1,) \
+            or up.groups != 1:
+        raise ValueError("upconv2_cat takes ConvTranspose1d(kernel_size=2, stride=2) without padding or groups")
+    if skip.shape[0] != x.shape[0] or skip.shape[2] != 2 * x.shape[2]:
+        raise RuntimeError(f"Sizes of tensors must match except in dimension 1: upsampled {2 * x.shape[2]} points, "
+                           f"skip connection {skip.shape[2]}")
+    _lib.require_like(x, up.weight, up.bias, skip, what="ConvTranspose1d weight/bias or skip connection")
+    w2 = up.weight.permute(2, 1, 0).reshape(2 * c, ci, 1)
+    b2 = up.bias.repeat(2) if up.bias is not None else None
+    return _UpsampleCat.apply(_CircConv1dAct.apply(x, w2, b2, _lib.INC_ACT_NONE), skip)
+
+
+def _unet_block(in_channels, features):
+    """conv5 -> ReLU -> conv5 -> ReLU, circular padding (unet.py:44-53)."""
+    return nn.Sequential(
+        nn.Conv1d(in_channels=in_channels, out_channels=features, kernel_size=5, padding=2, padding_mode="circular"),
+        nn.ReLU(inplace=True),
+        nn.Conv1d(in_channels=features, out_channels=features, kernel_size=5, padding=2, padding_mode="circular"),
+        nn.ReLU(inplace=True),
+    )
+
+
+class UNet1D(nn.Module):
+    """The reference's UNet1D correction network (models/models_1d/unet.py:5-53),
+    the trainer's default model type (scripts/Train_1D.py:82): the state and
+    a grid channel -> two encoder blocks with 2-to-1 max pooling -> a
+    bottleneck block -> two decoder blocks, each after a kernel-2 stride-2
+    transposed convolution concatenated with the encoder output of the same
+    resolution -> 1x1 convolution, returning [B, D].  Blocks are two
+    circular 5-tap convolutions with ReLU.  The module tree, parameter names
+    and initialisation order are the reference's, so its state dicts load
+    and a seeded construction gives the same weights.  With
+    ``backend='hip'`` (default) every convolution runs in this library's
+    kernels with its ReLU fused, and pooling, upsampling and concatenation
+    run in the library's resolution-change kernels; ``backend='torch'`` runs
+    the modules themselves.  D must be a multiple of 4 (the reference's
+    concatenations fail otherwise)."""
+
+    def __init__(self, L=16, in_channels=1 + 1, out_channels=1, init_features=32, backend="hip"):
+        super().__init__()
+        f = init_features
+        self.L = L
+        self.backend = backend
+        self.encoder1 = _unet_block(in_channels, f)
+        self.pool1 = nn.MaxPool1d(kernel_size=2, stride=2)
+        self.encoder2 = _unet_block(f, f * 2)
+        self.pool2 = nn.MaxPool1d(kernel_size=2, stride=2)
+        self.bottleneck = _unet_block(f * 2, f * 4)
+        self.upconv2 = nn.ConvTranspose1d(f * 4, f * 2, kernel_size=2, stride=2)
+        self.decoder2 = _unet_block(f * 4, f * 2)
+        self.upconv1 = nn.ConvTranspose1d(f * 2, f, kernel_size=2, stride=2)
+        self.decoder1 = _unet_block(f * 2, f)
+        self.conv = nn.Conv1d(in_channels=f, out_channels=out_channels, kernel_size=1)
+
+    def forward(self, x, L=None):
+        L = L if L is not None else self.L
+        h = torch.stack((x, get_grid(x, L)), 1)  # [B, 2, D]
+        if self.backend != "hip":
+            enc1 = self.encoder1(h)
+            enc2 = self.encoder2(self.pool1(enc1))
+            bottleneck = self.bottleneck(self.pool2(enc2))
+            dec2 = self.decoder2(torch.cat((self.upconv2(bottleneck), enc2), dim=1))
+            dec1 = self.decoder1(torch.cat((self.upconv1(dec2), enc1), dim=1))
+            return self.conv(dec1).squeeze(1)
+        if x.shape[-1] % 4:
+            raise RuntimeError(f"UNet1D needs a multiple of 4 points for its skip concatenations (got {x.shape[-1]})")
+        relu = _lib.INC_ACT_RELU
+
+        def block(t, seq):
+            return _conv_act(_conv_act(t, seq[0], relu), seq[2], relu)
+
+        enc1 = block(h, self.encoder1)
+        enc2 = block(max_pool2(enc1), self.encoder2)
+        bottleneck = block(max_pool2(enc2), self.bottleneck)
+        dec2 = block(upconv2_cat(bottleneck, self.upconv2, enc2), self.decoder2)
+        dec1 = block(upconv2_cat(dec2, self.upconv1, enc1), self.decoder1)
+        return _conv_act(dec1, self.conv, _lib.INC_ACT_NONE).squeeze(1)

@@ -60,6 +60,14 @@ struct WgradMode {
   int* nparts;
 };
 
+// One block of a weight gradient's channels: output channels [o0, o0 + Co)
+// of the gC channels of g and input channels [c0, c0 + Ci) of the xC
+// channels of x, where the kernel's Co and Ci are the block's sizes.  The
+// whole gradient is the block {Co, Ci, 0, 0}.
+struct ChanBlock {
+  int gC, xC, o0, c0;
+};
+
 struct JobPos {
   int b, i0, n;
   int64_t base;  // first point of the row (element offset C * base)
@@ -239,7 +247,8 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int Nu, c
 template <typename T, int K, int MAXP>
 __global__ void __launch_bounds__(256) conv_wgrad_kernel(int njobs, int Ci, int Co, int Nu, int TI,
                                                         const T* __restrict__ g, const T* __restrict__ x,
-                                                        T* __restrict__ part, int accumulate, RowTiles rt) {
+                                                        T* __restrict__ part, int accumulate, RowTiles rt,
+                                                        ChanBlock cb) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int P = K / 2;
   const int GW = TI | 1;                   // gradient row stride (odd)
@@ -262,8 +271,8 @@ __global__ void __launch_bounds__(256) conv_wgrad_kernel(int njobs, int Ci, int 
     const JobPos jp = job_pos(rt, job, tiles, TI, Nu);
     const int N = jp.n, i0 = jp.i0;
     __syncthreads();
-    const T* gb = g + jp.base * Co;
-    const T* xb = x + jp.base * Ci;
+    const T* gb = g + jp.base * cb.gC + size_t(cb.o0) * N;
+    const T* xb = x + jp.base * cb.xC + size_t(cb.c0) * N;
     for (int e = tid; e < Co2 * TI; e += nthr) {
       int o = e / TI, j = e - o * TI;
       gs[o * GW + j] = (o < Co && i0 + j < N) ? gb[size_t(o) * N + i0 + j] : T(0);
@@ -342,7 +351,8 @@ __global__ void __launch_bounds__(256) conv_wgrad_kernel(int njobs, int Ci, int 
 // eight loads are in flight rather than one.
 template <typename T>
 __global__ void __launch_bounds__(256) reduce_parts_kernel(int nparts, int width, const T* __restrict__ part,
-                                                          T* __restrict__ dw, T* __restrict__ db, int nw) {
+                                                          T* __restrict__ dw, T* __restrict__ db, int nw, int rowlen,
+                                                          int rowstride) {
   __shared__ T red[16][17];
   const int ex = threadIdx.x % 16, pl = threadIdx.x / 16;
   const int e = blockIdx.x * 16 + ex;
@@ -362,7 +372,7 @@ __global__ void __launch_bounds__(256) reduce_parts_kernel(int nparts, int width
   if (pl == 0 && e < width) {
     T t = red[0][ex];
     for (int q = 1; q < 16; ++q) t += red[q][ex];
-    if (e < nw) dw[e] = t;
+    if (e < nw) dw[size_t(e / rowlen) * rowstride + e % rowlen] = t;
     else if (db) db[e - nw] = t;
   }
 }
@@ -386,7 +396,7 @@ template <int MT, int NT>
 __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci, int Co, int Nu, int K, int WT, int WP,
                                                              const float* __restrict__ g,
                                                              const float* __restrict__ x, float* __restrict__ part,
-                                                             int accumulate, RowTiles rt) {
+                                                             int accumulate, RowTiles rt, ChanBlock cb) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int P = K / 2;
   constexpr int GS = kWgTP + 1;             // odd row strides: the 16 rows of a fragment
@@ -423,8 +433,8 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
   for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
     const JobPos jp = job_pos(rt, job, tiles, kWgTP, Nu);
     const int N = jp.n, i0 = jp.i0;
-    const float* gb = g + jp.base * Co;
-    const float* xb = x + jp.base * Ci;
+    const float* gb = g + jp.base * cb.gC + size_t(cb.o0) * N;
+    const float* xb = x + jp.base * cb.xC + size_t(cb.c0) * N;
     // Stage the job's tiles with every load of a thread in flight before its
     // LDS writes (the staging is latency-bound otherwise): all MT * 8
     // gradient values, then the input rows eight at a time.
@@ -506,7 +516,7 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
 // MFMA weight gradient when the shape fits (Co <= 64, Ci*K + 1 <= 512, LDS);
 // returns INC_EUNSUPPORTED (without touching the error text) otherwise.
 int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const float* x, float* dw, float* db,
-                    float* part, int64_t work_elems, hipStream_t st, const WgradMode& md) {
+                    float* part, int64_t work_elems, hipStream_t st, const WgradMode& md, const ChanBlock& cb) {
   const int MT = (Co + 15) / 16;
   const int ncw = Ci * K, ntile = (ncw + 1 + 15) / 16;
   const int WT = ntile >= 4 ? 4 : (ntile >= 2 ? 2 : 1);
@@ -527,7 +537,7 @@ int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const f
   const int accumulate = md.accumulate ? 1 : 0;
 #define INC_WGM(M, T)                                                                                                 \
   hipLaunchKernelGGL((conv_wgrad_mfma_kernel<M, T>), grid, block, lds, st, jobs, Ci, Co, N, K, WT, WP, g, x, part, \
-                     accumulate, md.rt)
+                     accumulate, md.rt, cb)
 #define INC_WGM_T(M)                  \
   switch (NT) {                       \
     case 1: INC_WGM(M, 1); break;     \
@@ -548,7 +558,7 @@ int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const f
   int rc = check_launch("conv1d weight gradient (matrix cores)");
   if (rc || !md.reduce) return rc;
   hipLaunchKernelGGL(reduce_parts_kernel<float>, dim3((width + 15) / 16), dim3(256), 0, st, nblk * WP, width, part,
-                     dw, db, Co * ncw);
+                     dw, db, Co * ncw, ncw, cb.xC * K);
   return check_launch("conv1d weight gradient reduction");
 }
 
@@ -898,13 +908,13 @@ int conv_fwd_launch(int B, int Ci, int Co, int N, int K, const void* x, const vo
 
 template <typename T, int K>
 int wgrad_launch_k(int njobs, int Ci, int Co, int N, int TI, int nblk, size_t lds, const T* g, const T* x, T* part,
-                   hipStream_t st, int accumulate, RowTiles rt) {
+                   hipStream_t st, int accumulate, RowTiles rt, const ChanBlock& cb) {
   const int nblk2 = ((Co + 1) / 2) * ((Ci + 1) / 2);
   const int maxp = (nblk2 + 255) / 256;
   dim3 grid(nblk), block(256);
 #define INC_WG(MP)                                                                                            \
   hipLaunchKernelGGL((conv_wgrad_kernel<T, K, MP>), grid, block, lds, st, njobs, Ci, Co, N, TI, g, x, part, \
-                     accumulate, rt)
+                     accumulate, rt, cb)
   if (maxp <= 1) INC_WG(1);
   else if (maxp <= 2) INC_WG(2);
   else if (maxp <= 3) INC_WG(3);
@@ -918,12 +928,12 @@ int wgrad_launch_k(int njobs, int Ci, int Co, int N, int TI, int nblk, size_t ld
 }
 
 template <typename T>
-int conv_wgrad(int B, int Ci, int Co, int N, int K, const void* g, const void* x, void* dw, void* db, void* work,
-               int64_t work_elems, hipStream_t st, const WgradMode& md = WgradMode{RowTiles{}, 0, false, true, nullptr}) {
+int conv_wgrad_block(int B, int Ci, int Co, int N, int K, const void* g, const void* x, void* dw, void* db, void* work,
+                     int64_t work_elems, hipStream_t st, const WgradMode& md, const ChanBlock& cb) {
   if constexpr (std::is_same<T, float>::value) {
     const int rc = conv_wgrad_mfma(B, Ci, Co, N, K, static_cast<const float*>(g), static_cast<const float*>(x),
                                    static_cast<float*>(dw), static_cast<float*>(db), static_cast<float*>(work),
-                                   work_elems, st, md);
+                                   work_elems, st, md, cb);
     if (rc != INC_EUNSUPPORTED) return rc;
   }
   int TI = md.rt.tile ? kRowTile : (N < 128 ? N : 128);
@@ -943,17 +953,41 @@ int conv_wgrad(int B, int Ci, int Co, int N, int K, const void* g, const void* x
   const int acc = md.accumulate ? 1 : 0;
   int rc;
   switch (K) {
-    case 1: rc = wgrad_launch_k<T, 1>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt); break;
-    case 3: rc = wgrad_launch_k<T, 3>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt); break;
-    case 5: rc = wgrad_launch_k<T, 5>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt); break;
-    case 7: rc = wgrad_launch_k<T, 7>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt); break;
-    case 9: rc = wgrad_launch_k<T, 9>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt); break;
+    case 1: rc = wgrad_launch_k<T, 1>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt, cb); break;
+    case 3: rc = wgrad_launch_k<T, 3>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt, cb); break;
+    case 5: rc = wgrad_launch_k<T, 5>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt, cb); break;
+    case 7: rc = wgrad_launch_k<T, 7>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt, cb); break;
+    case 9: rc = wgrad_launch_k<T, 9>(jobs, Ci, Co, N, TI, nblk, lds, gp, xp, part, st, acc, md.rt, cb); break;
     default: set_error("conv1d kernel size must be 1, 3, 5, 7 or 9 (got %d)", K); return INC_EUNSUPPORTED;
   }
   if (rc || !md.reduce) return rc;
   hipLaunchKernelGGL(reduce_parts_kernel<T>, dim3((width + 15) / 16), dim3(256), 0, st, nblk, width, part,
-                     static_cast<T*>(dw), static_cast<T*>(db), Co * Ci * K);
+                     static_cast<T*>(dw), static_cast<T*>(db), Co * Ci * K, Ci * K, cb.xC * K);
   return check_launch("conv1d weight gradient reduction");
+}
+
+// Weight and bias gradients; a layer with more channels than one launch
+// holds (accumulators in registers, tiles in LDS) runs as blocks of at most
+// 64 output x 64 input channels, each reduced into its part of dw (the
+// blocks are ordered on the stream, so they share the workspace; the bias
+// gradient comes from the blocks of the first input channels).
+template <typename T>
+int conv_wgrad(int B, int Ci, int Co, int N, int K, const void* g, const void* x, void* dw, void* db, void* work,
+               int64_t work_elems, hipStream_t st, const WgradMode& md = WgradMode{RowTiles{}, 0, false, true, nullptr}) {
+  int rc = conv_wgrad_block<T>(B, Ci, Co, N, K, g, x, dw, db, work, work_elems, st, md, ChanBlock{Co, Ci, 0, 0});
+  if (rc != INC_EUNSUPPORTED || md.accumulate || !md.reduce || (Co <= 64 && Ci <= 64)) return rc;
+  constexpr int kB = 64;
+  T* dwp = static_cast<T*>(dw);
+  T* dbp = static_cast<T*>(db);
+  for (int o0 = 0; o0 < Co; o0 += kB)
+    for (int c0 = 0; c0 < Ci; c0 += kB) {
+      rc = conv_wgrad_block<T>(B, std::min(kB, Ci - c0), std::min(kB, Co - o0), N, K, g, x,
+                               dwp + (size_t(o0) * Ci + c0) * K, c0 == 0 && dbp ? dbp + o0 : nullptr, work, work_elems,
+                               st, md, ChanBlock{Co, Ci, o0, c0});
+      if (rc) return rc;
+    }
+  clear_error();
+  return INC_OK;
 }
 
 int conv_check(int dtype, int act, int B, int Ci, int Co, int N, int K) {
@@ -1273,7 +1307,7 @@ int inc_adj(Step& sv, const inc_cnn_desc* cnn, const NetRows& nr, const T* u0, c
     const int nw = int(width[l]) - (l == nl - 1 ? 1 : C);
     if (nparts[l] > 0) {
       hipLaunchKernelGGL(reduce_parts_kernel<T>, dim3(unsigned((width[l] + 15) / 16)), dim3(256), 0, st, nparts[l],
-                         int(width[l]), work + L.part[l], g_w + at, g_w + at + nw, nw);
+                         int(width[l]), work + L.part[l], g_w + at, g_w + at + nw, nw, nw, nw);
       rc = check_launch("ragged INC weight-gradient reduction");
     } else {
       rc = fill<T>(width[l], T(0), g_w + at, st);

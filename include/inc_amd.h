@@ -442,6 +442,34 @@ int inc_conv1d_rows_bwd_weight(int dtype, const inc_conv_rows* r, int Ci, int Co
                                void* gw, void* gb, void* work, int64_t work_elems, void* stream);
 
 /*
+ * Resolution changes of the UNet1D correction network
+ * (models/models_1d/unet.py:5-53).
+ *
+ * inc_maxpool2_fwd is torch.nn.MaxPool1d(kernel_size=2, stride=2)
+ * (unet.py:11,13) over `rows` rows of n points (n even): y [rows, n/2],
+ * y[r][i] = x[r][2i + 1] if x[r][2i + 1] > x[r][2i] or it is NaN, else
+ * x[r][2i] (torch's window scan: the first element wins ties).
+ * inc_maxpool2_bwd routes gy to the element the forward picked, recomputed
+ * from x; the other element of each pair gets 0.
+ */
+int inc_maxpool2_fwd(int dtype, int64_t rows, int n, const void* x, void* y, void* stream);
+int inc_maxpool2_bwd(int dtype, int64_t rows, int n, const void* x, const void* gy, void* gx, void* stream);
+/*
+ * torch.cat((ConvTranspose1d(k=2, s=2)(x), skip), dim=1) (unet.py:16-21,
+ * :37-42) from y2, the transposed convolution computed as a 1x1 convolution
+ * with 2C output channels (W'[t*C + c][ci] = W[ci][c][t], bias repeated):
+ *   out [B, C + Cs, 2n]: out[b][c][2i + t] = y2[b][t*C + c][i] (c < C),
+ *                        out[b][C + c][j] = skip[b][c][j]
+ * y2 [B, 2C, n]; skip [B, Cs, 2n] (Cs = 0: no skip connection).
+ * inc_upsample2_cat_bwd splits the gradient g [B, C + Cs, 2n] back into
+ * gy2 [B, 2C, n] and gskip [B, Cs, 2n] (NULL: not wanted).
+ */
+int inc_upsample2_cat(int dtype, int B, int C, int n, int Cs, const void* y2, const void* skip, void* out,
+                      void* stream);
+int inc_upsample2_cat_bwd(int dtype, int B, int C, int n, int Cs, const void* g, void* gy2, void* gskip,
+                          void* stream);
+
+/*
  * INC training rollout of a ragged batch: simulate_rollout(model, solver, u0,
  * steps, 'INC') (trainer/trainer_1d.py:154-183) over the uneven batch of
  * inc_burgers_ragged_fwd (one solver per batch in trainer_1d.py:338-358),

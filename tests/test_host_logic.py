@@ -115,3 +115,50 @@ def test_resnet1d_module_tree_is_the_reference_s():
     from inc_amd.correction import _ResidualBlock1D
     assert list(_ResidualBlock1D(8, 16).state_dict().keys()) == [
         "conv1.weight", "conv1.bias", "conv2.weight", "conv2.bias", "shortcut.weight", "shortcut.bias"]
+
+
+def test_unet1d_module_tree_is_the_reference_s():
+    """UNet1D keeps models/models_1d/unet.py:5-53's module tree, parameter
+    names and shapes (its state dicts load) and builds the same weights from
+    a seed in either backend; the torch backend runs on the CPU.  The HIP
+    backend refuses host tensors and point counts that are not a multiple of
+    4 before any launch."""
+    from inc_amd.correction import UNet1D
+    f = 32
+    blocks = {"encoder1": (2, f), "encoder2": (f, 2 * f), "bottleneck": (2 * f, 4 * f),
+              "decoder2": (4 * f, 2 * f), "decoder1": (2 * f, f)}
+    shapes = {}
+    for name in ("encoder1", "encoder2", "bottleneck"):
+        ci, co = blocks[name]
+        shapes.update({f"{name}.0.weight": (co, ci, 5), f"{name}.0.bias": (co,),
+                       f"{name}.2.weight": (co, co, 5), f"{name}.2.bias": (co,)})
+    shapes.update({"upconv2.weight": (4 * f, 2 * f, 2), "upconv2.bias": (2 * f,)})
+    ci, co = blocks["decoder2"]
+    shapes.update({"decoder2.0.weight": (co, ci, 5), "decoder2.0.bias": (co,), "decoder2.2.weight": (co, co, 5),
+                   "decoder2.2.bias": (co,)})
+    shapes.update({"upconv1.weight": (2 * f, f, 2), "upconv1.bias": (f,)})
+    ci, co = blocks["decoder1"]
+    shapes.update({"decoder1.0.weight": (co, ci, 5), "decoder1.0.bias": (co,), "decoder1.2.weight": (co, co, 5),
+                   "decoder1.2.bias": (co,)})
+    shapes.update({"conv.weight": (1, f, 1), "conv.bias": (1,)})
+    sd = UNet1D().state_dict()
+    assert list(sd.keys()) == list(shapes.keys())
+    assert all(tuple(sd[k].shape) == v for k, v in shapes.items())
+    torch.manual_seed(3)
+    a = UNet1D()
+    torch.manual_seed(3)
+    b = UNet1D(backend="torch")
+    assert all(torch.equal(p, q) for p, q in zip(a.parameters(), b.parameters()))
+    assert b(torch.randn(2, 32, dtype=torch.float32), 16.0).shape == (2, 32)
+    try:
+        a(torch.randn(2, 30), 16.0)
+    except RuntimeError as e:
+        assert "multiple of 4" in str(e)
+    else:
+        raise AssertionError("30 points must raise")
+    try:
+        a(torch.randn(2, 32), 16.0)
+    except ValueError as e:
+        assert "GPU" in str(e)
+    else:
+        raise AssertionError("host tensors must raise")
