@@ -68,6 +68,18 @@ struct ChanBlock {
   int gC, xC, o0, c0;
 };
 
+// One channel block of a matrix-core convolution: the kernel's Ci input
+// channels are channels [c0, c0 + Ci) of the xC of x, its Co outputs are
+// channels [o0, o0 + Co) of the yC of y (and of mul, dy); the weights are
+// indexed in the full layout (flip: w[c][o] with yC per row, else w[o][c]
+// with xC).  phase: 0 the whole reduction (bias, mul, activation in the
+// epilogue); over several input-channel blocks 1 writes the first partial
+// sum plus the bias, 2 adds a middle block's, 3 adds the last and applies
+// mul and the activation.
+struct ConvBlock {
+  int xC, c0, yC, o0, phase;
+};
+
 struct JobPos {
   int b, i0, n;
   int64_t base;  // first point of the row (element offset C * base)
@@ -585,7 +597,7 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
                                                             const float* __restrict__ w,
                                                             const float* __restrict__ bias, float* __restrict__ y,
                                                             const float* __restrict__ mul, int act,
-                                                            float* __restrict__ dy, RowTiles rt) {
+                                                            float* __restrict__ dy, RowTiles rt, ConvBlock cb) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int TI = 64 * PW;
   const int P = K / 2;
@@ -605,7 +617,7 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
     for (int u = 0; u < 2; ++u) {
       const int p = p0 + u * 256, o = p / Ci, c = p - o * Ci;
       const bool ok = p < CT * 16 * Ci && o < Co;
-      const float* src = w + (flip ? size_t(c) * Co + o : size_t(o) * Ci + c) * K;
+      const float* src = w + (flip ? size_t(cb.c0 + c) * cb.yC + cb.o0 + o : size_t(cb.o0 + o) * cb.xC + cb.c0 + c) * K;
 #pragma unroll
       for (int k = 0; k < 9; ++k) v[u][k] = (ok && k < K) ? src[flip ? K - 1 - k : k] : 0.f;
     }
@@ -648,7 +660,7 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
   auto fetch = [&](int job, v4f* v) __attribute__((always_inline)) {
     const JobPos jp = job_pos(rt, job, tiles, TI, Nu);
     const int N = jp.n, i0 = jp.i0;
-    const float* xb = x + jp.base * Ci;
+    const float* xb = x + jp.base * cb.xC + size_t(cb.c0) * N;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       // unconditional loads (items past the tile read quad c4n - 1 and are
@@ -671,7 +683,7 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
   for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
     const JobPos jp = job_pos(rt, job, tiles, TI, Nu);
     const int N = jp.n, i0 = jp.i0;
-    const size_t ybase = size_t(jp.base) * Co;
+    const size_t ybase = size_t(jp.base) * cb.yC + size_t(cb.o0) * N;
     __syncthreads();  // the weights are staged and the previous tile consumed
     stage(nx);
     __syncthreads();
@@ -745,7 +757,7 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int o = min(m * 16 + 4 * q + r, Co - 1), i = min(i0 + p0 + 16 * t + lr, N - 1);
-          mv[m][t][r] = mul ? mul[ybase + size_t(o) * N + i] : 1.f;
+          mv[m][t][r] = mul && cb.phase == 0 ? mul[ybase + size_t(o) * N + i] : 1.f;
         }
 #pragma unroll
     for (int m = 0; m < CT; ++m)
@@ -756,7 +768,17 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
           const int o = m * 16 + 4 * q + r, i = i0 + p0 + 16 * t + lr;
           if (o >= Co || i >= N) continue;
           const size_t at = ybase + size_t(o) * N + i;
-          float v = (acc[m][t][r] + (bias ? bias[o] : 0.f)) * mv[m][t][r];
+          if (cb.phase == 1) {  // first input-channel block: the partial sum and the bias
+            y[at] = acc[m][t][r] + (bias ? bias[o] : 0.f);
+            continue;
+          }
+          if (cb.phase == 2) {  // a middle block: add to the partial sum
+            y[at] += acc[m][t][r];
+            continue;
+          }
+          float v = cb.phase == 3 ? y[at] + acc[m][t][r] : acc[m][t][r] + (bias ? bias[o] : 0.f);
+          if (cb.phase == 3 && mul) v *= mul[at];
+          v *= mv[m][t][r];
           if (act == INC_ACT_GELU) {
             if (dy) {
               float gv;
@@ -812,9 +834,9 @@ int blocks_per_cu(const void* fn, size_t lds) {
   return per_cu;
 }
 
-int conv_fwd_mfma(int B, int Ci, int Co, int N, int K, bool flip, const float* x, const float* w,
-                  const float* bias, float* y, const float* mul, int act, float* dy, hipStream_t st,
-                  RowTiles rt, int rjobs) {
+int conv_fwd_mfma_block(int B, int Ci, int Co, int N, int K, bool flip, const float* x, const float* w,
+                        const float* bias, float* y, const float* mul, int act, float* dy, hipStream_t st,
+                        RowTiles rt, int rjobs, ConvBlock cb) {
   if (Ci % 16 != 0 || Ci > 64 || Co < 16 || Co > 64) return INC_EUNSUPPORTED;
   const int CT = (Co + 15) / 16;
   auto lds_of = [&](int pw) {  // staged weights and the input tile (with its spare row)
@@ -842,9 +864,36 @@ int conv_fwd_mfma(int B, int Ci, int Co, int N, int K, bool flip, const float* x
   const int nblk = int(std::min<int64_t>(jobs, int64_t(cu_count()) * per_cu));
   int fl = flip ? 1 : 0;
   int nj = int(jobs);
-  void* args[] = {&nj, &Ci, &Co, &N, &K, &fl, &x, &w, &bias, &y, &mul, &act, &dy, &rt};
+  void* args[] = {&nj, &Ci, &Co, &N, &K, &fl, &x, &w, &bias, &y, &mul, &act, &dy, &rt, &cb};
   INC_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(256), args, lds, st));
   return check_launch("conv1d (matrix cores)");
+}
+
+// Matrix-core forward (or input gradient).  A layer of more than 64 input
+// or output channels runs as channel blocks: output blocks of equal size
+// (<= 64, >= 16) are independent launches; input blocks of 64 (Ci a
+// multiple of 16) accumulate into y in stream order, the last one applying
+// mul and the activation.  INC_EUNSUPPORTED when a block does not fit.
+int conv_fwd_mfma(int B, int Ci, int Co, int N, int K, bool flip, const float* x, const float* w,
+                  const float* bias, float* y, const float* mul, int act, float* dy, hipStream_t st,
+                  RowTiles rt, int rjobs) {
+  if (Ci <= 64 && Co <= 64)
+    return conv_fwd_mfma_block(B, Ci, Co, N, K, flip, x, w, bias, y, mul, act, dy, st, rt, rjobs,
+                               ConvBlock{Ci, 0, Co, 0, 0});
+  if (Ci % 16 != 0 || Co < 16 || rt.tile) return INC_EUNSUPPORTED;
+  const int nob = (Co + 63) / 64, cob = (Co + nob - 1) / nob;
+  const int nib = (Ci + 63) / 64;
+  for (int o0 = 0; o0 < Co; o0 += cob) {
+    const int co = std::min(cob, Co - o0);
+    for (int ib = 0; ib < nib; ++ib) {
+      const int c0 = 64 * ib, ci = std::min(64, Ci - c0);
+      const int phase = nib == 1 ? 0 : (ib == 0 ? 1 : (ib == nib - 1 ? 3 : 2));
+      const int rc = conv_fwd_mfma_block(B, ci, co, N, K, flip, x, w, bias ? bias + o0 : nullptr, y, mul, act, dy, st,
+                                         rt, rjobs, ConvBlock{Ci, c0, Co, o0, phase});
+      if (rc) return rc;
+    }
+  }
+  return INC_OK;
 }
 
 template <typename T, int K, bool FLIP>
