@@ -352,7 +352,7 @@ def extra_ks_inc(step_path=True):
                          "flops_per_launch": flops}}
 
 
-def extra_reference_net(kind, shapes=((512, 128), (2048, 256))):
+def extra_reference_net(kind, shapes=((512, 64), (512, 128), (2048, 256))):
     """One of the reference's correction networks (ResNet1D: 6 residual
     blocks, 32 features; UNet1D: 32 initial features) forward + backward, on
     this library's kernels against the same module run by torch

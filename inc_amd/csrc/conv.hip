@@ -75,9 +75,12 @@ struct ChanBlock {
 // with xC).  phase: 0 the whole reduction (bias, mul, activation in the
 // epilogue); over several input-channel blocks 1 writes the first partial
 // sum plus the bias, 2 adds a middle block's, 3 adds the last and applies
-// mul and the activation.
+// mul and the activation.  rpj > 1 (n = 16 or 32, the packed kernel): a
+// job is rpj = 64 / n consecutive batch rows of `rows`, each staged with its
+// own periodic halo, so no position of the 64-position tile is idle.
 struct ConvBlock {
   int xC, c0, yC, o0, phase;
+  int rpj, rows;  // packed rows (rows per job, batch rows); rpj = 1: one row per job
 };
 
 struct JobPos {
@@ -591,7 +594,10 @@ int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const f
 // used in turn).
 // flip: the input-gradient operator, as in conv_fwd_kernel; the epilogue is
 // conv_fwd_kernel's (bias, mul, GELU with its derivative).
-template <int CT, int PW>
+// PACK (PW = 1, n = 16 or 32): a job is cb.rpj = 64 / n whole batch rows,
+// staged row after row, each as n + K - 1 rows of xs with its own halo;
+// position p of the tile is point p % n of row p / n.
+template <int CT, int PW, bool PACK = false>
 __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, int Co, int Nu, int K, int flip,
                                                             const float* __restrict__ x,
                                                             const float* __restrict__ w,
@@ -600,8 +606,10 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
                                                             float* __restrict__ dy, RowTiles rt, ConvBlock cb) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int TI = 64 * PW;
+  static_assert(!PACK || PW == 1, "packed rows fill a 64-position tile");
   const int P = K / 2;
-  const int R = Ci * K, RS = R + 4, CS = Ci + 4, XW = TI + K - 1;
+  const int XR = Nu + K - 1;  // staged rows per packed batch row
+  const int R = Ci * K, RS = R + 4, CS = Ci + 4, XW = PACK ? cb.rpj * XR : TI + K - 1;
   float* ws = reinterpret_cast<float*>(smem_raw);  // [CT*16][RS]
   float* xs = ws + CT * 16 * RS;                   // [XW + 1][CS] (row XW spare)
   const int tid = threadIdx.x, lane = tid & 63;
@@ -639,7 +647,8 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
   // loads x[b, 4c4 .. 4c4+3, pos(j)] into registers and staged as one
   // 16-byte LDS write.  The next job's tile is fetched before this job's
   // MFMAs and staged after them, so its HBM latency overlaps the matrix work.
-  constexpr int U = 4 * PW + 1;  // items <= 16 * (64 PW + 8) <= 256 U (Ci <= 64, K <= 9)
+  // items <= 16 * (64 PW + 8) <= 256 U (Ci <= 64, K <= 9); packed: 16 * (64 + 4 * 8)
+  constexpr int U = PACK ? 6 : 4 * PW + 1;
   // A thread's items are the same in every job: quad and row of item u,
   // computed once (the global channel offset clamped to the last quad for
   // items past the tile, which are loaded and not staged)
@@ -657,7 +666,28 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
       }
     }
   }
+  // packed: each item's batch-row offset within the job and its point, the
+  // same in every job
+  int prow[PACK ? U : 1], ppos[PACK ? U : 1];
+  if constexpr (PACK) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      prow[u] = jr[u] / XR;
+      int pos = jr[u] - prow[u] * XR - P;  // in [-P, Nu + P)
+      ppos[u] = pos < 0 ? pos + Nu : (pos >= Nu ? pos - Nu : pos);
+    }
+  }
   auto fetch = [&](int job, v4f* v) __attribute__((always_inline)) {
+    if constexpr (PACK) {
+      const int b0 = job * cb.rpj;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int b = min(b0 + prow[u], cb.rows - 1);  // rows past the batch are loaded, never stored
+        const float* s = x + (size_t(b) * cb.xC + cb.c0 + 4 * min(cq[u], c4n - 1)) * Nu + ppos[u];
+        v[u] = v4f{s[0], s[Nu], s[2 * Nu], s[3 * Nu]};
+      }
+      return;
+    }
     const JobPos jp = job_pos(rt, job, tiles, TI, Nu);
     const int N = jp.n, i0 = jp.i0;
     const float* xb = x + jp.base * cb.xC + size_t(cb.c0) * N;
@@ -706,7 +736,7 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
       }
     };
     const float* wl = ws + lr * RS + 4 * q;
-    const float* xl = xs + (p0 + lr) * CS + 4 * q;
+    const float* xl = xs + (PACK ? ((p0 + lr) / Nu) * XR + (p0 + lr) % Nu : p0 + lr) * CS + 4 * q;
     auto load = [&](int s, v4f* fa, v4f* fb) __attribute__((always_inline)) {
 #pragma unroll
       for (int m = 0; m < CT; ++m) fa[m] = *reinterpret_cast<const v4f*>(wl + m * 16 * RS + 16 * s);
@@ -757,7 +787,12 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int o = min(m * 16 + 4 * q + r, Co - 1), i = min(i0 + p0 + 16 * t + lr, N - 1);
-          mv[m][t][r] = mul && cb.phase == 0 ? mul[ybase + size_t(o) * N + i] : 1.f;
+          size_t at = ybase + size_t(o) * N + i;
+          if constexpr (PACK) {
+            const int pr = (p0 + lr) / Nu, b = min(job * cb.rpj + pr, cb.rows - 1);
+            at = (size_t(b) * cb.yC + cb.o0 + o) * Nu + (p0 + lr) % Nu;
+          }
+          mv[m][t][r] = mul && cb.phase == 0 ? mul[at] : 1.f;
         }
 #pragma unroll
     for (int m = 0; m < CT; ++m)
@@ -766,8 +801,14 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int o = m * 16 + 4 * q + r, i = i0 + p0 + 16 * t + lr;
-          if (o >= Co || i >= N) continue;
-          const size_t at = ybase + size_t(o) * N + i;
+          size_t at = ybase + size_t(o) * N + i;
+          if constexpr (PACK) {
+            const int b = job * cb.rpj + (p0 + lr) / Nu;
+            if (o >= Co || b >= cb.rows) continue;
+            at = (size_t(b) * cb.yC + cb.o0 + o) * Nu + (p0 + lr) % Nu;
+          } else if (o >= Co || i >= N) {
+            continue;
+          }
           if (cb.phase == 1) {  // first input-channel block: the partial sum and the bias
             y[at] = acc[m][t][r] + (bias ? bias[o] : 0.f);
             continue;
@@ -839,16 +880,29 @@ int conv_fwd_mfma_block(int B, int Ci, int Co, int N, int K, bool flip, const fl
                         RowTiles rt, int rjobs, ConvBlock cb) {
   if (Ci % 16 != 0 || Ci > 64 || Co < 16 || Co > 64) return INC_EUNSUPPORTED;
   const int CT = (Co + 15) / 16;
+  // rows of 16 or 32 points: 64 / n whole rows per 64-position tile
+  const bool pack = !rt.tile && (N == 16 || N == 32);
+  cb.rpj = pack ? 64 / N : 1;
+  cb.rows = B;
   auto lds_of = [&](int pw) {  // staged weights and the input tile (with its spare row)
-    return (size_t(CT) * 16 * (Ci * K + 4) + size_t(64 * pw + K) * (Ci + 4)) * sizeof(float);
+    const size_t xrows = pack ? size_t(cb.rpj) * (N + K - 1) + 1 : size_t(64 * pw + K);
+    return (size_t(CT) * 16 * (Ci * K + 4) + xrows * (Ci + 4)) * sizeof(float);
   };
   // ragged rows come as kRowTile = 128-position tiles: two 16-position groups per wavefront
   const int PW = rt.tile ? 2 : ((N > 64 && lds_of(2) <= 80 * 1024) ? 2 : 1);
   const size_t lds = lds_of(PW);
   if (lds > 150 * 1024) return INC_EUNSUPPORTED;
-  const int64_t jobs = rt.tile ? rjobs : int64_t(B) * ((N + 64 * PW - 1) / (64 * PW));
+  const int64_t jobs = rt.tile ? rjobs
+                               : (pack ? (int64_t(B) + cb.rpj - 1) / cb.rpj : int64_t(B) * ((N + 64 * PW - 1) / (64 * PW)));
   const void* fn;
-  switch (CT * 2 + PW - 1) {
+  if (pack) {
+    switch (CT) {
+      case 1: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<1, 1, true>); break;
+      case 2: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<2, 1, true>); break;
+      case 3: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<3, 1, true>); break;
+      default: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<4, 1, true>); break;
+    }
+  } else switch (CT * 2 + PW - 1) {
     case 2: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<1, 1>); break;
     case 3: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<1, 2>); break;
     case 4: fn = reinterpret_cast<const void*>(&conv_fwd_mfma_kernel<2, 1>); break;
@@ -879,7 +933,7 @@ int conv_fwd_mfma(int B, int Ci, int Co, int N, int K, bool flip, const float* x
                   RowTiles rt, int rjobs) {
   if (Ci <= 64 && Co <= 64)
     return conv_fwd_mfma_block(B, Ci, Co, N, K, flip, x, w, bias, y, mul, act, dy, st, rt, rjobs,
-                               ConvBlock{Ci, 0, Co, 0, 0});
+                               ConvBlock{Ci, 0, Co, 0, 0, 1, B});
   if (Ci % 16 != 0 || Co < 16 || rt.tile) return INC_EUNSUPPORTED;
   const int nob = (Co + 63) / 64, cob = (Co + nob - 1) / nob;
   const int nib = (Ci + 63) / 64;
@@ -889,7 +943,7 @@ int conv_fwd_mfma(int B, int Ci, int Co, int N, int K, bool flip, const float* x
       const int c0 = 64 * ib, ci = std::min(64, Ci - c0);
       const int phase = nib == 1 ? 0 : (ib == 0 ? 1 : (ib == nib - 1 ? 3 : 2));
       const int rc = conv_fwd_mfma_block(B, ci, co, N, K, flip, x, w, bias ? bias + o0 : nullptr, y, mul, act, dy, st,
-                                         rt, rjobs, ConvBlock{Ci, c0, Co, o0, phase});
+                                         rt, rjobs, ConvBlock{Ci, c0, Co, o0, phase, 1, B});
       if (rc) return rc;
     }
   }

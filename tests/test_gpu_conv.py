@@ -32,11 +32,14 @@ def ref_conv(x, w, b):
 # takes the vector-ALU kernel.  Layers of more than 64 channels run as
 # channel blocks: input blocks of 64 accumulated in turn (80 = 64 + 16,
 # 144 = 64 + 64 + 16), equal output blocks (72 = 2 x 36, 144 = 3 x 48), and
-# weight gradients in 64 x 64 blocks.
+# weight gradients in 64 x 64 blocks.  Rows of 16 or 32 points are packed
+# 64 / n to a tile (a partial last job: B = 5 at n = 32, B = 7 and B = 1 at
+# n = 16).
 SHAPES = [(3, 1, 32, 128, 5), (3, 32, 32, 128, 5), (2, 32, 1, 128, 5), (2, 48, 48, 100, 7), (5, 16, 8, 64, 3),
           (1, 8, 8, 9, 9), (4, 32, 32, 1024, 5), (2, 48, 1, 512, 7), (3, 1, 48, 130, 7), (1, 1, 32, 60, 5),
           (1, 64, 64, 256, 9), (2, 48, 48, 512, 7), (2, 32, 24, 40, 5), (1, 16, 16, 16, 9), (3, 32, 16, 64, 1),
-          (2, 64, 64, 300, 3), (2, 80, 72, 96, 3), (2, 128, 128, 64, 5), (3, 16, 144, 70, 5), (2, 144, 16, 50, 1)]
+          (2, 64, 64, 300, 3), (2, 80, 72, 96, 3), (2, 128, 128, 64, 5), (3, 16, 144, 70, 5), (2, 144, 16, 50, 1),
+          (5, 32, 48, 32, 5), (7, 64, 64, 16, 5), (3, 128, 96, 32, 3)]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
@@ -62,7 +65,7 @@ def test_conv_forward_and_gradients(cuda, shape, dtype):
 
 
 @pytest.mark.parametrize("shape", [(3, 32, 32, 128, 5), (2, 48, 1, 100, 7), (2, 16, 48, 512, 7),
-                                   (2, 96, 80, 128, 5)],
+                                   (2, 96, 80, 128, 5), (3, 48, 32, 32, 7), (5, 16, 80, 16, 5)],
                          ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_conv_gelu_epilogue_and_scaled_input_gradient(cuda, shape, dtype):
