@@ -63,9 +63,12 @@ struct WgradMode {
 // One block of a weight gradient's channels: output channels [o0, o0 + Co)
 // of the gC channels of g and input channels [c0, c0 + Ci) of the xC
 // channels of x, where the kernel's Co and Ci are the block's sizes.  The
-// whole gradient is the block {Co, Ci, 0, 0}.
+// whole gradient is the block {Co, Ci, 0, 0}.  rpj > 1 (the packed weight
+// gradient, n = 16, 32 or 64): a job is rpj = 128 / n consecutive batch rows
+// of `rows`.
 struct ChanBlock {
   int gC, xC, o0, c0;
+  int rpj, rows;  // packed rows (rows per job, batch rows); rpj = 1: one row per job
 };
 
 // One channel block of a matrix-core convolution: the kernel's Ci input
@@ -407,7 +410,10 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kWgTP = 128;  // positions per job
 static_assert(kWgTP == kRowTile, "ragged tiles are one job's positions");
 
-template <int MT, int NT>
+// PACK (n = 16, 32 or 64): a job is cb.rpj = 128 / n whole batch rows; the
+// gradient tile is the rows side by side, the input staged row after row
+// with its own halo ([c][r (n + K - 1) + i + k]).
+template <int MT, int NT, bool PACK = false>
 __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci, int Co, int Nu, int K, int WT, int WP,
                                                              const float* __restrict__ g,
                                                              const float* __restrict__ x, float* __restrict__ part,
@@ -415,7 +421,9 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int P = K / 2;
   constexpr int GS = kWgTP + 1;             // odd row strides: the 16 rows of a fragment
-  const int XS = (kWgTP + K - 1) | 1;       // read hit distinct banks
+  const int XR = Nu + K - 1;                // staged points per packed row
+  const int lgn = 31 - __builtin_clz(Nu);   // packed: n is a power of two
+  const int XS = ((PACK ? cb.rpj * XR : kWgTP + K - 1)) | 1;  // read hit distinct banks
   float* gs = reinterpret_cast<float*>(smem_raw);  // [MT*16][GS]
   float* xs = gs + MT * 16 * GS;                   // [Ci][XS], then a row of ones and one of zeros
   const int tid = threadIdx.x, lane = tid & 63;
@@ -423,7 +431,7 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
   // every B-fragment read is an unconditional LDS read (a conditional read
   // makes the MFMAs wait for every read in flight); written once, ordered by
   // the first job's barriers
-  for (int e = tid; e < 2 * kWgTP; e += 256) xs[Ci * XS + e] = e < kWgTP ? 1.f : 0.f;
+  for (int e = tid; e < 2 * XS; e += 256) xs[Ci * XS + e] = e < XS ? 1.f : 0.f;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int tphase = wave % WT, pphase = wave / WT;
   const bool active = pphase < WP;
@@ -436,7 +444,7 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int tile = tphase + t * WT, col = tile * 16 + lr;
-    off[t] = tile >= ntile || col > ncw ? Ci * XS + kWgTP : (col == ncw ? Ci * XS : (col / K) * XS + col % K);
+    off[t] = tile >= ntile || col > ncw ? Ci * XS + XS : (col == ncw ? Ci * XS : (col / K) * XS + col % K);
   }
   v4f acc[MT][NT];
 #pragma unroll
@@ -444,7 +452,7 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[m][t] = v4f{0.f, 0.f, 0.f, 0.f};
   const int tiles = (Nu + kWgTP - 1) / kWgTP;
-  const int xw = kWgTP + K - 1;
+  const int xw = PACK ? cb.rpj * XR : kWgTP + K - 1;
   for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
     const JobPos jp = job_pos(rt, job, tiles, kWgTP, Nu);
     const int N = jp.n, i0 = jp.i0;
@@ -457,7 +465,12 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
 #pragma unroll
     for (int q = 0; q < MT * 8; ++q) {
       const int e = tid + q * 256, o = e / kWgTP, j = e % kWgTP;
-      gv[q] = (o < Co && i0 + j < N) ? gb[size_t(o) * N + i0 + j] : 0.f;
+      if constexpr (PACK) {
+        const int b = job * cb.rpj + (j >> lgn);
+        gv[q] = (o < Co && b < cb.rows) ? g[(size_t(b) * cb.gC + cb.o0 + o) * Nu + (j & (Nu - 1))] : 0.f;
+      } else {
+        gv[q] = (o < Co && i0 + j < N) ? gb[size_t(o) * N + i0 + j] : 0.f;
+      }
     }
     __syncthreads();  // the previous job's fragments are consumed
 #pragma unroll
@@ -472,10 +485,18 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
         const int e = e0 + q * 256;
         if (e < Ci * xw) {
           const int c = e / xw, j = e - c * xw;
-          int pos = i0 - P + j;  // in [-P, N + kWgTP + K): one wrap unless N is small
-          pos = pos < 0 ? pos + N : (pos >= N ? pos - N : pos);
-          if (pos >= N) pos %= N;
-          xv[q] = xb[size_t(c) * N + pos];
+          if constexpr (PACK) {
+            const int r = j / XR;
+            int pos = j - r * XR - P;  // in [-P, n + P)
+            pos = pos < 0 ? pos + Nu : (pos >= Nu ? pos - Nu : pos);
+            const int b = min(job * cb.rpj + r, cb.rows - 1);  // past the batch: the gradient rows are 0
+            xv[q] = x[(size_t(b) * cb.xC + cb.c0 + c) * Nu + pos];
+          } else {
+            int pos = i0 - P + j;  // in [-P, N + kWgTP + K): one wrap unless N is small
+            pos = pos < 0 ? pos + N : (pos >= N ? pos - N : pos);
+            if (pos >= N) pos %= N;
+            xv[q] = xb[size_t(c) * N + pos];
+          }
         }
       }
 #pragma unroll
@@ -490,10 +511,11 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
       float a[MT], bv[NT], an[MT], bn[NT];
       auto load = [&](int s, float* fa, float* fb) {
         const int p = 4 * s + lk;
+        const int px = PACK ? p + (p >> lgn) * (K - 1) : p;  // position of the packed row's point
 #pragma unroll
         for (int m = 0; m < MT; ++m) fa[m] = gs[(m * 16 + lr) * GS + p];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) fb[t] = xs[off[t] + p];
+        for (int t = 0; t < NT; ++t) fb[t] = xs[off[t] + px];
       };
       load(pphase, a, bv);
       for (int s = pphase; s < kWgTP / 4; s += WP) {
@@ -531,28 +553,36 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
 // MFMA weight gradient when the shape fits (Co <= 64, Ci*K + 1 <= 512, LDS);
 // returns INC_EUNSUPPORTED (without touching the error text) otherwise.
 int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const float* x, float* dw, float* db,
-                    float* part, int64_t work_elems, hipStream_t st, const WgradMode& md, const ChanBlock& cb) {
+                    float* part, int64_t work_elems, hipStream_t st, const WgradMode& md, ChanBlock cb) {
   const int MT = (Co + 15) / 16;
   const int ncw = Ci * K, ntile = (ncw + 1 + 15) / 16;
   const int WT = ntile >= 4 ? 4 : (ntile >= 2 ? 2 : 1);
   const int ntw = (ntile + WT - 1) / WT;
   const int NT = ntw <= 4 ? ntw : (ntw <= 6 ? 6 : 8);
-  const size_t lds =
-      (size_t(MT) * 16 * (kWgTP + 1) + size_t(Ci) * ((kWgTP + K - 1) | 1) + 2 * kWgTP) * sizeof(float);
+  // rows of 16, 32 or 64 points: 128 / n whole rows per job
+  const bool pack = !md.rt.tile && (N == 16 || N == 32 || N == 64);
+  cb.rpj = pack ? kWgTP / N : 1;
+  cb.rows = B;
+  const int XS = (pack ? cb.rpj * (N + K - 1) : kWgTP + K - 1) | 1;
+  const size_t lds = (size_t(MT) * 16 * (kWgTP + 1) + size_t(Ci) * XS + 2 * size_t(XS)) * sizeof(float);
   if (MT > 4 || ntw > 8 || lds > 150 * 1024) return INC_EUNSUPPORTED;
   const int width = Co * ncw + Co;
   const int64_t slots = std::min<int64_t>(512, work_elems / width);  // two blocks per CU
   if (slots < 1) return INC_EUNSUPPORTED;
   const int WP = int(std::min<int64_t>(4 / WT, slots));
-  const int jobs = md.rt.tile ? md.njobs : B * ((N + kWgTP - 1) / kWgTP);
+  const int jobs = md.rt.tile ? md.njobs : (pack ? (B + cb.rpj - 1) / cb.rpj : B * ((N + kWgTP - 1) / kWgTP));
   const int nblk = int(std::min<int64_t>(jobs, slots / WP));
   if (md.nparts) *md.nparts = std::max(*md.nparts, nblk * WP);
   if (nblk < 1) return INC_OK;  // no active rows: the partial sums stay as they are
   dim3 grid(nblk), block(256);
   const int accumulate = md.accumulate ? 1 : 0;
-#define INC_WGM(M, T)                                                                                                 \
-  hipLaunchKernelGGL((conv_wgrad_mfma_kernel<M, T>), grid, block, lds, st, jobs, Ci, Co, N, K, WT, WP, g, x, part, \
-                     accumulate, md.rt, cb)
+#define INC_WGM(M, T)                                                                                         \
+  if (pack)                                                                                                   \
+    hipLaunchKernelGGL((conv_wgrad_mfma_kernel<M, T, true>), grid, block, lds, st, jobs, Ci, Co, N, K, WT, WP, \
+                       g, x, part, accumulate, md.rt, cb);                                                   \
+  else                                                                                                        \
+    hipLaunchKernelGGL((conv_wgrad_mfma_kernel<M, T>), grid, block, lds, st, jobs, Ci, Co, N, K, WT, WP, g, x, \
+                       part, accumulate, md.rt, cb)
 #define INC_WGM_T(M)                  \
   switch (NT) {                       \
     case 1: INC_WGM(M, 1); break;     \
