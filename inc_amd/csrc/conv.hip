@@ -809,61 +809,81 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
     // of a lane's values in flight at once (clamped addresses, unused past
     // the edges).  Staging the tile through LDS for whole-row stores cost
     // two barriers per job and measured 4-7% slower.
-    float mv[CT][PW][4];
-#pragma unroll
-    for (int m = 0; m < CT; ++m)
-#pragma unroll
-      for (int t = 0; t < PW; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int o = min(m * 16 + 4 * q + r, Co - 1), i = min(i0 + p0 + 16 * t + lr, N - 1);
-          size_t at = ybase + size_t(o) * N + i;
-          if constexpr (PACK) {
-            const int pr = (p0 + lr) / Nu, b = min(job * cb.rpj + pr, cb.rows - 1);
-            at = (size_t(b) * cb.yC + cb.o0 + o) * Nu + (p0 + lr) % Nu;
-          }
-          mv[m][t][r] = mul && cb.phase == 0 ? mul[at] : 1.f;
+    auto store_act = [&](size_t at, float v) __attribute__((always_inline)) {
+      if (act == INC_ACT_GELU) {
+        if (dy) {
+          float gv;
+          dy[at] = gelu_and_grad(v, gv);
+          v = gv;
+        } else {
+          v = gelu_value(v);
         }
+      } else if (act == INC_ACT_RELU) {
+        if (dy) dy[at] = relu_grad(v);
+        v = relu(v);
+      }
+      y[at] = v;
+    };
+    // element (m, t, r) of this lane: its address, or false past the edges
+    auto addr = [&](int m, int t, int r, size_t& at) __attribute__((always_inline)) {
+      const int o = m * 16 + 4 * q + r, i = i0 + p0 + 16 * t + lr;
+      if constexpr (PACK) {
+        const int b = job * cb.rpj + (p0 + lr) / Nu;
+        at = (size_t(b) * cb.yC + cb.o0 + o) * Nu + (p0 + lr) % Nu;
+        return o < Co && b < cb.rows;
+      } else {
+        at = ybase + size_t(o) * N + i;
+        return o < Co && i < N;
+      }
+    };
+    if (cb.phase == 0) {  // the whole reduction in this launch
+      float mv[CT][PW][4];
 #pragma unroll
-    for (int m = 0; m < CT; ++m)
+      for (int m = 0; m < CT; ++m)
 #pragma unroll
-      for (int t = 0; t < PW; ++t)
+        for (int t = 0; t < PW; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int o = m * 16 + 4 * q + r, i = i0 + p0 + 16 * t + lr;
-          size_t at = ybase + size_t(o) * N + i;
-          if constexpr (PACK) {
-            const int b = job * cb.rpj + (p0 + lr) / Nu;
-            if (o >= Co || b >= cb.rows) continue;
-            at = (size_t(b) * cb.yC + cb.o0 + o) * Nu + (p0 + lr) % Nu;
-          } else if (o >= Co || i >= N) {
-            continue;
-          }
-          if (cb.phase == 1) {  // first input-channel block: the partial sum and the bias
-            y[at] = acc[m][t][r] + (bias ? bias[o] : 0.f);
-            continue;
-          }
-          if (cb.phase == 2) {  // a middle block: add to the partial sum
-            y[at] += acc[m][t][r];
-            continue;
-          }
-          float v = cb.phase == 3 ? y[at] + acc[m][t][r] : acc[m][t][r] + (bias ? bias[o] : 0.f);
-          if (cb.phase == 3 && mul) v *= mul[at];
-          v *= mv[m][t][r];
-          if (act == INC_ACT_GELU) {
-            if (dy) {
-              float gv;
-              dy[at] = gelu_and_grad(v, gv);
-              v = gv;
-            } else {
-              v = gelu_value(v);
+          for (int r = 0; r < 4; ++r) {
+            const int o = min(m * 16 + 4 * q + r, Co - 1), i = min(i0 + p0 + 16 * t + lr, N - 1);
+            size_t at = ybase + size_t(o) * N + i;
+            if constexpr (PACK) {
+              const int b = min(job * cb.rpj + (p0 + lr) / Nu, cb.rows - 1);
+              at = (size_t(b) * cb.yC + cb.o0 + o) * Nu + (p0 + lr) % Nu;
             }
-          } else if (act == INC_ACT_RELU) {
-            if (dy) dy[at] = relu_grad(v);
-            v = relu(v);
+            mv[m][t][r] = mul ? mul[at] : 1.f;
           }
-          y[at] = v;
-        }
+#pragma unroll
+      for (int m = 0; m < CT; ++m)
+#pragma unroll
+        for (int t = 0; t < PW; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            size_t at;
+            if (!addr(m, t, r, at)) continue;
+            const int o = m * 16 + 4 * q + r;
+            store_act(at, (acc[m][t][r] + (bias ? bias[o] : 0.f)) * mv[m][t][r]);
+          }
+    } else {  // one input-channel block of several, accumulated into y in stream order
+#pragma unroll
+      for (int m = 0; m < CT; ++m)
+#pragma unroll
+        for (int t = 0; t < PW; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            size_t at;
+            if (!addr(m, t, r, at)) continue;
+            const int o = m * 16 + 4 * q + r;
+            if (cb.phase == 1) {  // the first block: its partial sum and the bias
+              y[at] = acc[m][t][r] + (bias ? bias[o] : 0.f);
+            } else if (cb.phase == 2) {  // a middle block
+              y[at] += acc[m][t][r];
+            } else {  // the last block: the sum, the multiplier and the activation
+              float v = y[at] + acc[m][t][r];
+              if (mul) v *= mul[at];
+              store_act(at, v);
+            }
+          }
+    }
   }
 }
 
