@@ -1080,9 +1080,180 @@ int wgrad_launch_k(int njobs, int Ci, int Co, int N, int TI, int nblk, size_t ld
   return check_launch("conv1d weight gradient");
 }
 
+// ---- weight gradient of a layer with one channel on one side ----------------
+// The network's first layer (1 -> C) and last layer (C -> 1): C*K + C (or
+// C*K + 1) entries summed over every position, a streaming reduction that
+// reads each gradient and input value once (HBM-bound), without matrix-core
+// tiles or LDS staging.  Wavefront w of the 8 owns channels [w*CPW, w*CPW +
+// CPW) of the wide side; its 64 lanes take consecutive positions of a (row,
+// 128-position) job, so every load is coalesced (the K shifted input reads
+// of a lane hit the lines its neighbours load):
+//   ONE_IN (Ci = 1): acc[c][k] += g[c][i] x[(i + k - P) mod n], bias[c] += g[c][i]
+//   else   (Co = 1): acc[c][k] += g[i] x[c][(i + k - P) mod n], bias += g[i]
+// A lane keeps its sums over all of the block's jobs; at the end the 64
+// lanes of a wavefront are summed by a shuffle tree (a fixed order) and the
+// block writes, or adds to, its partial row as the other weight-gradient
+// kernels do.
+constexpr int kThinWaves = 8;
+
+template <typename T, int CPW, bool ONE_IN>
+__global__ void __launch_bounds__(64 * kThinWaves) conv_wgrad_thin_kernel(int njobs, int C, int Nu, int K,
+                                                                       const T* __restrict__ g,
+                                                                       const T* __restrict__ x,
+                                                                       T* __restrict__ part, int accumulate,
+                                                                       RowTiles rt) {
+  const int lane = threadIdx.x & 63;
+  const int c0 = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)) * CPW;
+  const int P = K / 2;
+  T acc[CPW][9], bacc[CPW];
+#pragma unroll
+  for (int cc = 0; cc < CPW; ++cc) {
+    bacc[cc] = T(0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[cc][k] = T(0);
+  }
+  constexpr int H = kRowTile / 64;  // positions per lane and job
+  const int tiles = (Nu + kRowTile - 1) / kRowTile;
+  for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
+    const JobPos jp = job_pos(rt, job, tiles, kRowTile, Nu);
+    const int N = jp.n;
+    const T* gb = g + jp.base * (ONE_IN ? C : 1);
+    const T* xb = x + jp.base * (ONE_IN ? 1 : C);
+    // The one-channel side is read as a K-point window per position, the
+    // wide side once per channel and position.  Written out, dW[c][k] sums
+    // g[c][i] x[i + k - P] (ONE_IN) or x[c][j] g[j - k + P] (with j = i + k -
+    // P, every position once per row), so the window is x at i + k - P or
+    // g at i - k + P.  Every load is unconditional (positions past the row
+    // read its last point and are masked), so all of them are in flight at
+    // once.
+    int ic[H];
+    bool ok[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const int i = jp.i0 + 64 * h + lane;
+      ok[h] = i < N;
+      ic[h] = ok[h] ? i : N - 1;
+    }
+    T wv[H][9], cv[H][CPW];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        int q = ONE_IN ? ic[h] + k - P : ic[h] - k + P;  // in [-P, N + P): one wrap (K <= N)
+        q = q < 0 ? q + N : (q >= N ? q - N : q);
+        wv[h][k] = k < K ? (ONE_IN ? xb : gb)[q] : T(0);
+      }
+#pragma unroll
+      for (int cc = 0; cc < CPW; ++cc)
+        cv[h][cc] = c0 + cc < C ? (ONE_IN ? gb : xb)[size_t(c0 + cc) * N + ic[h]] : T(0);
+    }
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      if constexpr (!ONE_IN) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) wv[h][k] = ok[h] ? wv[h][k] : T(0);
+      }
+      if (ONE_IN) {
+#pragma unroll
+        for (int cc = 0; cc < CPW; ++cc) {
+          const T v = ok[h] ? cv[h][cc] : T(0);
+          bacc[cc] += v;
+#pragma unroll
+          for (int k = 0; k < 9; ++k)
+            if (k < K) acc[cc][k] += v * wv[h][k];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+          if (k == P) bacc[0] += wv[h][k];  // g[i]
+#pragma unroll
+        for (int cc = 0; cc < CPW; ++cc)
+#pragma unroll
+          for (int k = 0; k < 9; ++k)
+            if (k < K) acc[cc][k] += cv[h][cc] * wv[h][k];
+      }
+    }
+  }
+  const int width = C * K + (ONE_IN ? C : 1);
+  T* out = part + size_t(blockIdx.x) * width;
+#pragma unroll
+  for (int cc = 0; cc < CPW; ++cc) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      if (k < K) acc[cc][k] = wave_sum(acc[cc][k], kWave);
+    bacc[cc] = wave_sum(bacc[cc], kWave);
+  }
+  if (lane != 0) return;
+#pragma unroll
+  for (int cc = 0; cc < CPW; ++cc) {
+    const int c = c0 + cc;
+    if (c >= C) break;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      if (k < K) {
+        T* dst = out + c * K + k;
+        *dst = accumulate ? *dst + acc[cc][k] : acc[cc][k];
+      }
+    if (ONE_IN) {
+      T* dst = out + C * K + c;
+      *dst = accumulate ? *dst + bacc[cc] : bacc[cc];
+    }
+  }
+  if (!ONE_IN && c0 == 0) {
+    T* dst = out + C * K;
+    *dst = accumulate ? *dst + bacc[0] : bacc[0];
+  }
+}
+
+// Weight gradient of a 1 -> C or C -> 1 layer (C <= 64) on the streaming
+// kernel above.  INC_EUNSUPPORTED for other shapes.
+template <typename T>
+int conv_wgrad_thin(int B, int Ci, int Co, int N, int K, const T* g, const T* x, T* dw, T* db, T* part,
+                    int64_t work_elems, hipStream_t st, const WgradMode& md) {
+  const bool one_in = Ci == 1;
+  const int C = one_in ? Co : Ci;
+  if (!(Ci == 1 || Co == 1) || C > 8 * kThinWaves) return INC_EUNSUPPORTED;
+  const int width = Co * Ci * K + Co;
+  const int jobs = md.rt.tile ? md.njobs : B * ((N + kRowTile - 1) / kRowTile);
+  const int64_t slots = std::min<int64_t>(512, work_elems / width);
+  INC_REQUIRE(jobs < 1 || slots >= 1, INC_EINVAL, "conv1d weight gradient workspace too small (need >= %d elements)",
+              width);
+  const int nblk = int(std::min<int64_t>(jobs, slots));
+  if (md.nparts) *md.nparts = std::max(*md.nparts, nblk);
+  if (nblk < 1) return INC_OK;
+  const int cpw = (C + kThinWaves - 1) / kThinWaves;
+  const int acc = md.accumulate ? 1 : 0;
+  dim3 grid(nblk), block(64 * kThinWaves);
+#define INC_WT(CPW)                                                                                              \
+  if (one_in)                                                                                                    \
+    hipLaunchKernelGGL((conv_wgrad_thin_kernel<T, CPW, true>), grid, block, 0, st, jobs, C, N, K, g, x, part, acc, \
+                       md.rt);                                                                                   \
+  else                                                                                                           \
+    hipLaunchKernelGGL((conv_wgrad_thin_kernel<T, CPW, false>), grid, block, 0, st, jobs, C, N, K, g, x, part,    \
+                       acc, md.rt)
+  if (cpw <= 1) INC_WT(1);
+  else if (cpw <= 2) INC_WT(2);
+  else if (cpw <= 3) INC_WT(3);
+  else if (cpw <= 4) INC_WT(4);
+  else if (cpw <= 6) INC_WT(6);
+  else INC_WT(8);
+#undef INC_WT
+  int rc = check_launch("conv1d weight gradient (one-channel side)");
+  if (rc || !md.reduce) return rc;
+  hipLaunchKernelGGL(reduce_parts_kernel<T>, dim3((width + 15) / 16), dim3(256), 0, st, nblk, width, part, dw, db,
+                     Co * Ci * K, Ci * K, Ci * K);
+  return check_launch("conv1d weight gradient reduction");
+}
+
 template <typename T>
 int conv_wgrad_block(int B, int Ci, int Co, int N, int K, const void* g, const void* x, void* dw, void* db, void* work,
                      int64_t work_elems, hipStream_t st, const WgradMode& md, const ChanBlock& cb) {
+  if (cb.o0 == 0 && cb.c0 == 0 && cb.gC == Co && cb.xC == Ci) {
+    const int rc = conv_wgrad_thin<T>(B, Ci, Co, N, K, static_cast<const T*>(g), static_cast<const T*>(x),
+                                      static_cast<T*>(dw), static_cast<T*>(db), static_cast<T*>(work), work_elems, st,
+                                      md);
+    if (rc != INC_EUNSUPPORTED) return rc;
+  }
   if constexpr (std::is_same<T, float>::value) {
     const int rc = conv_wgrad_mfma(B, Ci, Co, N, K, static_cast<const float*>(g), static_cast<const float*>(x),
                                    static_cast<float*>(dw), static_cast<float*>(db), static_cast<float*>(work),
