@@ -209,3 +209,44 @@ def test_unet_as_burgers_inc_correction_against_oracle(cuda):
     assert rel_l2(u.grad, uo.grad) < 1e-10
     for p, q in zip(net.parameters(), cpu.parameters()):
         assert rel_l2(p.grad, q.grad) < 1e-10
+
+
+@pytest.mark.parametrize("net_cls", ["UNet1D", "ResNet1D"])
+def test_reference_nets_as_ks_inc_correction_against_oracle(cuda, net_cls):
+    """The reference's KS training configuration (trainer_1d.py:361-378: ETD1,
+    the double step, per-trajectory domain sizes passed to the network as L,
+    64 points): the reference's correction networks on the library's kernels
+    as the INC correction of a KS rollout, against the FP64 oracle driven by
+    a CPU float64 copy of the network."""
+    from inc_amd import KSSimParams, KSSolverTorch, ResNet1D
+    from oracle import KSOracle, ks_params
+    cls = {"UNet1D": UNet1D, "ResNet1D": ResNet1D}[net_cls]
+    b, n, H = 4, 64, 5
+    Ls = [22.0, 30.0, 40.0, 64.0]
+    torch.manual_seed(12)
+    cpu = cls(init_features=8, backend="torch").double()
+    with torch.no_grad():
+        (cpu.conv if net_cls == "UNet1D" else cpu.final).weight.mul_(0.05)
+    net = cls(init_features=8).to(cuda, torch.float64)
+    net.load_state_dict(cpu.state_dict())
+    gen = torch.Generator().manual_seed(13)
+    u0 = torch.zeros(b, n, dtype=torch.float64)
+    for i, L in enumerate(Ls):
+        x = torch.arange(n, dtype=torch.float64) * (L / n)
+        for k in range(1, 5):
+            u0[i] += 0.5 * torch.randn(1, generator=gen, dtype=torch.float64) * torch.cos(
+                2 * math.pi * k * x / L + 2 * math.pi * torch.rand(1, generator=gen, dtype=torch.float64))
+    target = 0.3 * torch.randn(b, H, n, generator=torch.Generator().manual_seed(14), dtype=torch.float64)
+    so = KSOracle(ks_params(Ls, resolution=n, dt=0.1, time_scheme="ETD1"))
+    uo = u0.clone().requires_grad_(True)
+    ref, ref_c = oracle_rollout(cpu, so, uo, H, "INC", L=so.domain_size)
+    ((ref - target) ** 2).mean().backward()
+    sv = KSSolverTorch(KSSimParams(Ls, resolution=n, dt=0.1, time_scheme="ETD1", device="cuda"))
+    Ld = torch.tensor(Ls, dtype=torch.float64, device=cuda)
+    u = u0.to(cuda).requires_grad_(True)
+    traj, corr = inc_amd.simulate_rollout(net, sv, u, H, "INC", L=Ld)
+    ((traj - target.to(cuda)) ** 2).mean().backward()
+    assert rel_l2(traj.detach(), ref.detach()) < 1e-11 and rel_l2(corr.detach(), ref_c.detach()) < 1e-11
+    assert rel_l2(u.grad, uo.grad) < 1e-10
+    for p, q in zip(net.parameters(), cpu.parameters()):
+        assert rel_l2(p.grad, q.grad) < 1e-10
