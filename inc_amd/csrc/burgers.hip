@@ -62,10 +62,12 @@ __device__ __forceinline__ T lin_b(const T* f) {
 
 template <typename T>
 struct WenoParts {
-  T q[3], a[3], b[3], d[3], w[3], F;
+  T q[3], a[3], b[3], r[3], w[3], F;  // r_k = 1 / (eps + beta_k)
 };
 
 // Nonlinear weights and reconstructed value for one 5-tap stencil.
+// alpha_k = gamma_k / d_k^2 is formed from r_k = 1/d_k (one reciprocal per
+// candidate, which the VJP reuses).
 template <int D, typename T>
 __device__ __forceinline__ void weno_parts(const T* f, WenoParts<T>& o) {
   o.q[0] = lin_q<D, 0>(f); o.q[1] = lin_q<D, 1>(f); o.q[2] = lin_q<D, 2>(f);
@@ -75,8 +77,8 @@ __device__ __forceinline__ void weno_parts(const T* f, WenoParts<T>& o) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     T beta = T(13.0 / 12.0) * o.a[k] * o.a[k] + T(0.25) * o.b[k] * o.b[k];
-    o.d[k] = T(kEps) + beta;
-    al[k] = T(kGamma[k]) * rcp(o.d[k] * o.d[k]);
+    o.r[k] = rcp(T(kEps) + beta);
+    al[k] = T(kGamma[k]) * (o.r[k] * o.r[k]);
   }
   T inv = rcp(al[0] + al[1] + al[2]);
   o.w[0] = al[0] * inv; o.w[1] = al[1] * inv; o.w[2] = al[2] * inv;
@@ -95,6 +97,8 @@ __device__ __forceinline__ T weno_value(const T* f) {
 //   dF/df_m = sum_k w_k dq_k/df_m + sum_k c_k dbeta_k/df_m,
 //   c_k = -2 w_k (q_k - F) / d_k,
 //   dbeta_k/df_m = 13/6 a_k da_k/df_m + 1/2 b_k db_k/df_m.
+// With e_k = gbar w_k (q_k - F) / d_k the beta terms' weights are
+// gbar c_k 13/6 a_k = -13/3 e_k a_k and gbar c_k 1/2 b_k = -e_k b_k.
 template <int D, typename T>
 __device__ __forceinline__ void weno_vjp(const T* f, T gbar, T* df) {
   WenoParts<T> o;
@@ -102,10 +106,10 @@ __device__ __forceinline__ void weno_vjp(const T* f, T gbar, T* df) {
   T pw[3], pa[3], pb[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    T ck = T(-2) * o.w[k] * (o.q[k] - o.F) * rcp(o.d[k]);
     pw[k] = gbar * o.w[k];
-    pa[k] = gbar * ck * T(13.0 / 6.0) * o.a[k];
-    pb[k] = gbar * ck * T(0.5) * o.b[k];
+    const T e = pw[k] * ((o.q[k] - o.F) * o.r[k]);
+    pa[k] = e * (T(-13.0 / 3.0) * o.a[k]);
+    pb[k] = -e * o.b[k];
   }
 #pragma unroll
   for (int m = 0; m < 5; ++m) {
