@@ -117,6 +117,8 @@ SIGNATURES = {
     "inc_burgers_ragged_fwd": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp,
                                               _vp]),
     "inc_burgers_ragged_adj": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "inc_burgers_cfl": (ctypes.c_int, [ctypes.POINTER(BurgersDesc), _vp, _vp, _vp]),
+    "inc_burgers_ragged_cfl": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), _vp, _vp, _vp]),
     "inc_ks_tables": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                      _vp, ctypes.c_int, _vp, _vp]),
     "inc_ks_fwd": (ctypes.c_int, [ctypes.POINTER(KSDesc), _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),

@@ -249,6 +249,31 @@ class RaggedBurgers:
                     m[h:, self.offsets[i]:self.offsets[i] + self.n[i]] = 0
         return m
 
+    def stability_numbers(self, u):
+        """Per-trajectory (Courant, viscous) numbers of each trajectory's step
+        from the packed state u: max |u| dt_b / dx_b and nu_b dt_b / dx_b^2,
+        [count, 2] in the user's order, on the GPU (inc_burgers_ragged_cfl).
+        A NaN in a trajectory makes its Courant number NaN."""
+        _lib.require_gpu(u)
+        if u.numel() != self.total or u.dtype != self.dtype:
+            raise ValueError(f"packed state must be [{self.total}] of {self.dtype}")
+        u = u.detach().contiguous()
+        d = self._desc()
+        out = u.new_empty(self.count, 2)
+        rc = _lib.load().inc_burgers_ragged_cfl(ctypes.byref(d), _lib.ptr(u), _lib.ptr(out), _lib.stream())
+        _lib.check(rc, "ragged stability numbers")
+        res = torch.empty_like(out)
+        res[torch.tensor(self.perm, device=out.device)] = out
+        return res
+
+    def unstable(self, u, courant=1.0, viscous=None):
+        """Bool [count] (user order): trajectories whose step from u leaves the
+        stable region (see BurgersSolverTorch.unstable; the viscous limit is
+        the batch's time scheme's)."""
+        c = self.stability_numbers(u)
+        vl = (0.5 if self.scheme == _lib.INC_EULER else 2.5127 / 4) if viscous is None else viscous
+        return ~((c[:, 0] <= courant) & (c[:, 1] <= vl))
+
     def _fusable(self, model, u0):
         from .correction import CorrectionCNN
         return (isinstance(model, CorrectionCNN) and u0.is_cuda and model.ragged_desc(u0.dtype) is not None

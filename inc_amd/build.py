@@ -15,7 +15,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libinc_amd.so")
-SOURCES = ["common.cpp", "burgers.hip", "ks.hip", "conv.hip", "unet.hip"]
+SOURCES = ["common.cpp", "burgers.hip", "ks.hip", "conv.hip", "unet.hip", "cfl.hip"]
 ARCH = os.environ.get("INC_AMD_ARCH", "gfx950")
 
 

@@ -772,6 +772,35 @@ class BurgersSolverTorch:
             self.LOG.info(f"CFL = {torch.max(torch.abs(u)) * self.dt / self.dx:.2f}, time_step: {self.dt}")
         return u_new
 
+    # Largest nu dt / dx^2 with a stable step on the 3-point Laplacian: Euler's
+    # 1 + z needs z >= -2, SSP-RK3's 1 + z + z^2/2 + z^3/6 needs z >= -2.5127,
+    # with z = -4 nu dt / dx^2 at the highest mode.
+    VISCOUS_LIMIT = {"Euler": 0.5, "SSPRK3": 2.5127 / 4}
+
+    def stability_numbers(self, u, dt=None):
+        """Per-trajectory (Courant, viscous) numbers of a step of size dt (the
+        solver's by default) from u: max_i |u_i| dt / dx and nu dt / dx^2,
+        [batch, 2] on the GPU with no host synchronisation (inc_burgers_cfl).
+        A NaN in a trajectory makes its Courant number NaN.  The reference
+        reads the batch-wide max |u| back to the host to size its adaptive
+        sub-steps (solver_1d.py:158-161)."""
+        L.require_gpu(u)
+        u = u.detach().contiguous()
+        d, keep = self._desc(u, self.dt if dt is None else dt, 0.0)
+        out = u.new_empty(u.shape[0], 2)
+        rc = L.load().inc_burgers_cfl(ctypes.byref(d), L.ptr(u), L.ptr(out), L.stream())
+        L.check(rc, "Burgers stability numbers")
+        return out
+
+    def unstable(self, u, courant=1.0, viscous=None, dt=None):
+        """Bool [batch]: the trajectories whose step from u leaves the stable
+        region: Courant number above ``courant``, viscous number above
+        ``viscous`` (by default the time scheme's limit, VISCOUS_LIMIT), or a
+        NaN in the state.  On the GPU, no host synchronisation."""
+        c = self.stability_numbers(u, dt)
+        vl = self.VISCOUS_LIMIT[self.time_scheme] if viscous is None else viscous
+        return ~((c[:, 0] <= courant) & (c[:, 1] <= vl))
+
     def rollout(self, u, mstep, checkpoint_every=None):
         """``mstep`` steps in one launch with the correction held fixed;
         returns the [B, mstep, N] trajectory (the fused path of

@@ -13,8 +13,13 @@
  *    float32 (INC_F32) or float64 (INC_F64) for every floating array of a
  *    call, except the KS domain sizes, which are always float64.
  *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = the
- *    null stream), allocates nothing and only enqueues kernels on it (no
- *    host synchronisation).  Capture into a hipGraph is not validated.
+ *    null stream), allocates nothing and enqueues kernels on it with no host
+ *    synchronisation.  Besides kernel launches it makes only device queries
+ *    whose results are cached (the CU count, and the resident blocks of a
+ *    matrix-core convolution at its LDS size), so a call made once outside a
+ *    hipGraph capture can be captured afterwards: the ragged and KS INC
+ *    rollouts (inc_ragged_inc_fwd/_adj) and the KS rollout with its adjoint
+ *    are validated under capture and replay.
  *  - Return value: INC_OK (0) or an error code; inc_last_error() returns a
  *    thread-local message for the last failing call of the calling thread.
  *  - Calls that overlap on one stream are ordered; calls on different
@@ -302,6 +307,19 @@ typedef struct inc_ragged_desc {
 int inc_burgers_ragged_fwd(const inc_ragged_desc* d, const void* u, const void* s, void* u_out, int steps,
                            void* traj, void* stages, void* stream);
 /* Adjoint of inc_burgers_ragged_fwd: g_traj [steps, total] -> g_u [total], g_s [total] (or NULL). */
+/*
+ * Stability numbers of one Burgers step from the state u, per trajectory:
+ * cfl [batch, 2] (the state's dtype) = (max_i |u_i| dt / dx, nu dt / dx^2),
+ * computed on the GPU with no host synchronisation.  The reference reads the
+ * batch-wide max |u| back to the host to size its adaptive sub-steps
+ * (solvers/solver_1d.py:158-161); a fixed-step caller uses these to flag the
+ * trajectories whose step leaves the stable region.  A NaN in a trajectory
+ * makes its Courant number NaN.  inc_burgers_ragged_cfl does the same for a
+ * ragged batch ([count, 2] in the packed order, each trajectory with its own
+ * n, dt, dx and nu).
+ */
+int inc_burgers_cfl(const inc_burgers_desc* d, const void* u, void* cfl, void* stream);
+int inc_burgers_ragged_cfl(const inc_ragged_desc* d, const void* u, void* cfl, void* stream);
 int inc_burgers_ragged_adj(const inc_ragged_desc* d, const void* stages, const void* g_traj, void* g_u, void* g_s,
                            int steps, void* stream);
 

@@ -56,6 +56,15 @@ int main() {
                  INC_EINVAL &&
              has("target"),
          "mse target");
+  expect(inc_burgers_cfl(&d, nullptr, nullptr, nullptr) == INC_EINVAL && has("missing buffer"),
+         "stability numbers need buffers");
+  d.dx = 0.0;
+  expect(inc_burgers_cfl(&d, nullptr, nullptr, nullptr) == INC_EINVAL && has("bad descriptor"),
+         "stability numbers grid spacing");
+  d.dx = 0.1;
+  d.batch = 0;
+  expect(inc_burgers_cfl(&d, nullptr, nullptr, nullptr) == INC_OK, "empty stability numbers");
+  d.batch = 1;
 
   // the network inside the Burgers rollout
   inc_cnn_desc cn{};
@@ -94,6 +103,8 @@ int main() {
   expect(inc_burgers_ragged_fwd(&r, nullptr, nullptr, nullptr, 1, nullptr, nullptr, nullptr) == INC_EINVAL &&
              has("8 points"),
          "ragged minimum size");
+  expect(inc_burgers_ragged_cfl(&r, nullptr, nullptr, nullptr) == INC_EINVAL && has("missing buffer"),
+         "ragged stability numbers need buffers");
 
   // workspace sizes: host arithmetic only
   r.total = 2048 * 400;

@@ -23,7 +23,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "inc_amd", "csrc")
-SOURCES = ["common.cpp", "burgers.hip", "ks.hip", "conv.hip", "unet.hip"]
+SOURCES = ["common.cpp", "burgers.hip", "ks.hip", "conv.hip", "unet.hip", "cfl.hip"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 
