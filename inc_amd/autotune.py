@@ -86,12 +86,19 @@ def _time_ms(fn, reps=3):
 
 
 def ks_radix_candidates(n, dtype):
+    """FFT radices of the register transform (power-of-two n in [16, 1024]);
+    other resolutions run the mixed-radix transform, which has no radix."""
+    if not (16 <= n <= 1024 and n & (n - 1) == 0):
+        return []
     radices = (2, 4) if dtype == torch.float64 else (2, 4, 8)
     return [r for r in radices if r <= n and n // r <= 256]
 
 
 def tune_ks(n, batch, dtype=torch.float32, scheme="ETDRK4", L=64.0, dt=0.25, steps=16, record=None):
-    """Best FFT radix for a KS batch (forward + adjoint of ``steps`` steps)."""
+    """Best FFT radix for a KS batch (forward + adjoint of ``steps`` steps);
+    0 (no choice) for the resolutions of the mixed-radix transform."""
+    if not ks_radix_candidates(n, dtype):
+        return 0
     key = f"{_arch()}|ks|{scheme}|n{n}|b{batch}|{str(dtype).split('.')[-1]}"
     hit = lookup(key)
     if hit is not None:

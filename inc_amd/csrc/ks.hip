@@ -23,6 +23,9 @@
 //  * The radix trades lanes for registers: R = 2 spreads a pair over twice
 //    the lanes of R = 4 (half the instructions per lane), which wins while
 //    the batch leaves SIMDs idle; R = 4 wins once every SIMD has work.
+//  * Resolutions that are not powers of two in [16, 1024] run the same
+//    scheme kernels over a mixed-radix transform through LDS (MixX below);
+//    the kernels are written once over the transform policy.
 //  * Spectral quantities are kept scaled by 1/N, so the unnormalised inverse
 //    transform returns physical values directly.
 //  * The state stays in registers across all steps of a launch; HBM sees
@@ -428,7 +431,7 @@ struct FftCfg {
 // Unnormalised DFT (SIGN = -1) from the physical strided layout to the
 // transform's spectral layout, or inverse DFT without 1/N (SIGN = +1) back.
 template <typename T, int N, int R, int SIGN>
-__device__ __forceinline__ void fft(cpx<T>* v, const cpx<T>* tw, XBuf<T>& xb, int j) {
+__device__ __forceinline__ void reg_transform(cpx<T>* v, const cpx<T>* tw, XBuf<T>& xb, int j) {
   if constexpr (SIGN < 0) reg_fft_fwd<T, N, R, 0>(v, tw, xb, j);
   else reg_fft_inv<T, N, R, RPlan<N, R>::S - 1>(v, tw, xb, j);
 }
@@ -454,6 +457,9 @@ struct KArgs {
   const T* target;
   T* loss_parts;
   const T* loss_scale;  // device scalar
+  // the mixed-radix transform (any n): points per trajectory and its passes
+  int n;
+  uint64_t plan;
 };
 
 constexpr int kTableStride = INC_KS_TABLE_STRIDE;
@@ -483,17 +489,220 @@ __host__ __device__ constexpr int stages_per_sub(int scheme) {
   return scheme == INC_ETD1 ? 1 : (scheme == INC_ETRK2 ? 2 : 4);
 }
 
-template <typename T, int N, int R>
+// ---- transform policies ------------------------------------------------------------
+// A policy X gives teams of X::TL lanes (X::TPB teams per block, X::LDS
+// complex LDS words per team).  Lane j of a team holds the physical points
+// j + m*TL, m < X::R, of a trajectory pair in registers (the strided
+// layout); x.fft<-1>() is the unnormalised DFT from there to a spectral
+// layout in which register m of lane j holds mode x.spec(m), x.fft<+1>()
+// the inverse DFT without 1/N back.  With X::kMasked, points and modes at
+// j + m*TL >= n are padding: zero, never stored.
+
+// Power-of-two N in [16, 1024]: the register-transpose FFT above.
+template <typename T, int N_, int R_>
+struct RegX {
+  static constexpr int R = R_, TL = N_ / R_, TPB = TL >= 64 ? 1 : 64 / TL;
+  static constexpr int LDS = FftCfg<T, N_, R_>::LDS_TEAM;
+  static constexpr bool kMasked = false;
+  cpx<T> tw[FftCfg<T, N_, R_>::NTW];
+  XBuf<T> xb;
+  int j;
+  __device__ __forceinline__ RegX(cpx<T>* lds, int lane, const KArgs<T>& a) : xb{lds, 0}, j(lane) {
+    load_reg_twiddles<T, N_, R_>(a.twiddle, j, tw);
+  }
+  __device__ __forceinline__ static constexpr int n() { return N_; }
+  __device__ __forceinline__ int spec(int m) const { return spec_index<N_, R_>(m, j); }
+  template <int SIGN>
+  __device__ __forceinline__ void fft(cpx<T>* v) {
+    reg_transform<T, N_, R_, SIGN>(v, tw, xb, j);
+  }
+};
+
+// Mixed-radix transform for any n: a self-sorting (Stockham) decimation in
+// time through LDS.  After the passes of radices p_1 .. p_s (L = p_1 ...
+// p_s, M = n / L) the buffer holds X_s[k * M + r] = DFT_L of the points
+// r + M * i, i < L, at frequency k.  Radix p turns (L, M) into (L p, M / p):
+//   X'[(k + L t) * M' + r] = sum_q w_p^{q t} (w_n^{q k M'} X[(k p + q) M' + r])
+// with M' = M / p, k < L, t < p, r < M'.  Radices 2, 3, 4 and 5 run as
+// butterflies (one item reads p points); any other prime computes each
+// output as its own p-term sum.  The passes come from the host as 4-bit
+// codes (plan_radix), first pass in the low bits; twiddles w_n^i sit in
+// LDS, and the output is in natural order.
+__device__ __forceinline__ int plan_radix(int code, int rest) {
+  switch (code) {
+    case 1: return 2;
+    case 2: return 3;
+    case 3: return 4;
+    case 4: return 5;
+    case 5: return 7;
+    case 6: return 11;
+    case 7: return 13;
+    case 8: return 17;
+    case 9: return 19;
+    case 10: return 23;
+    case 11: return 29;
+    case 12: return 31;
+    default: return rest;  // code 15: the one prime factor above 31
+  }
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft3(cpx<T>* x) {
+  const cpx<T> t1 = x[1] + x[2], t2 = x[0] - T(0.5) * t1;
+  const cpx<T> t3 = T(0.86602540378443865) * mul_i<SIGN>(x[1] - x[2]);
+  x[0] = x[0] + t1;
+  x[1] = t2 + t3;
+  x[2] = t2 - t3;
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft5(cpx<T>* x) {
+  constexpr T c1 = T(0.30901699437494742), c2 = T(-0.80901699437494742);
+  constexpr T s1 = T(0.95105651629515357), s2 = T(0.58778525229247313);
+  const cpx<T> s14 = x[1] + x[4], d14 = x[1] - x[4], s23 = x[2] + x[3], d23 = x[2] - x[3];
+  const cpx<T> ra = x[0] + c1 * s14 + c2 * s23, rb = x[0] + c2 * s14 + c1 * s23;
+  const cpx<T> ia = mul_i<SIGN>(s1 * d14 + s2 * d23), ib = mul_i<SIGN>(s2 * d14 - s1 * d23);
+  x[0] = x[0] + s14 + s23;
+  x[1] = ra + ia;
+  x[4] = ra - ia;
+  x[2] = rb + ib;
+  x[3] = rb - ib;
+}
+
+template <int P, int SIGN, typename T>
+__device__ __forceinline__ void dft_mix(cpx<T>* x) {
+  if constexpr (P == 2) dft2<SIGN>(x);
+  else if constexpr (P == 3) dft3<SIGN>(x);
+  else if constexpr (P == 4) dft4<SIGN>(x);
+  else dft5<SIGN>(x);
+}
+
+// Quotient and remainder of 0 <= a < 2^16 by 1 <= d <= 1024 from rd, the
+// float reciprocal of d within 1 ulp: (a + 0.5) rd is within 2^-22 (a + 0.5)
+// / d of the exact (a + 0.5) / d, whose fraction stays 0.5/d away from the
+// integers, and one correction step absorbs anything left.
+__device__ __forceinline__ int div_small(int a, int d, float rd, int& rem) {
+  int q = int((float(a) + 0.5f) * rd);
+  int r = a - q * d;
+  if (r < 0) {
+    --q;
+    r += d;
+  } else if (r >= d) {
+    ++q;
+    r -= d;
+  }
+  rem = r;
+  return q;
+}
+
+template <int P, int SIGN, typename T>
+__device__ __forceinline__ void mix_pass(const cpx<T>* src, cpx<T>* dst, const cpx<T>* tw, int n, int L, int mp,
+                                         int j, int tl) {
+  const float rmp = __builtin_amdgcn_rcpf(float(mp));
+  for (int b = j; b < n / P; b += tl) {
+    int r;
+    const int k = div_small(b, mp, rmp, r);
+    cpx<T> v[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) v[q] = src[(k * P + q) * mp + r];
+#pragma unroll
+    for (int q = 1; q < P; ++q) {
+      const cpx<T> w = tw[q * k * mp];  // q k M' < n
+      v[q] = SIGN < 0 ? cmul(v[q], w) : cmulc(v[q], w);
+    }
+    dft_mix<P, SIGN>(v);
+#pragma unroll
+    for (int t = 0; t < P; ++t) dst[(k + L * t) * mp + r] = v[t];
+  }
+}
+
+// radix p of no butterfly: output K * M' + r = sum_q X[(k p + q) M' + r] w_n^{q M' K}, k = K mod L
+template <int SIGN, typename T>
+__device__ __forceinline__ void mix_pass_any(const cpx<T>* src, cpx<T>* dst, const cpx<T>* tw, int n, int L,
+                                             int mp, int p, int j, int tl) {
+  const float rmp = __builtin_amdgcn_rcpf(float(mp)), rl = __builtin_amdgcn_rcpf(float(L));
+  for (int o = j; o < n; o += tl) {
+    int r, k;
+    const int K = div_small(o, mp, rmp, r);
+    div_small(K, L, rl, k);
+    const int step = K * mp;  // < n
+    const cpx<T>* in = src + k * p * mp + r;
+    cpx<T> acc{T(0), T(0)};
+    int idx = 0;
+    for (int q = 0; q < p; ++q) {
+      const cpx<T> w = tw[idx], v = in[q * mp];
+      acc = acc + (SIGN < 0 ? cmul(v, w) : cmulc(v, w));
+      idx += step;
+      if (idx >= n) idx -= n;
+    }
+    dst[o] = acc;
+  }
+}
+
+template <typename T, int TL_, int R_>
+struct MixX {
+  static constexpr int R = R_, TL = TL_, TPB = 1, NMAX = TL_ * R_;
+  static constexpr int LDS = 3 * NMAX;  // two pass buffers and the twiddles
+  static constexpr bool kMasked = true;
+  cpx<T>* buf;
+  int j, nn;
+  uint64_t plan;
+  __device__ __forceinline__ MixX(cpx<T>* lds, int lane, const KArgs<T>& a)
+      : buf(lds), j(lane), nn(a.n), plan(a.plan) {
+    cpx<T>* tw = buf + 2 * NMAX;
+    for (int i = j; i < nn; i += TL) tw[i] = cpx<T>{a.twiddle[2 * i], a.twiddle[2 * i + 1]};
+    __syncthreads();
+  }
+  __device__ __forceinline__ int n() const { return nn; }
+  __device__ __forceinline__ int spec(int m) const { return j + m * TL; }
+  template <int SIGN>
+  __device__ __forceinline__ void fft(cpx<T>* v) {
+    cpx<T>* src = buf;
+    cpx<T>* dst = buf + NMAX;
+    const cpx<T>* tw = buf + 2 * NMAX;
+#pragma unroll
+    for (int m = 0; m < R; ++m)
+      if (j + m * TL < nn) src[j + m * TL] = v[m];
+    __syncthreads();
+    int L = 1;
+    for (uint64_t c = plan; c != 0; c >>= 4) {
+      const int p = plan_radix(int(c & 15), nn / L), mp = nn / (L * p);
+      switch (p) {
+        case 2: mix_pass<2, SIGN>(src, dst, tw, nn, L, mp, j, TL); break;
+        case 3: mix_pass<3, SIGN>(src, dst, tw, nn, L, mp, j, TL); break;
+        case 4: mix_pass<4, SIGN>(src, dst, tw, nn, L, mp, j, TL); break;
+        case 5: mix_pass<5, SIGN>(src, dst, tw, nn, L, mp, j, TL); break;
+        default: mix_pass_any<SIGN>(src, dst, tw, nn, L, mp, p, j, TL);
+      }
+      __syncthreads();
+      cpx<T>* t = src;
+      src = dst;
+      dst = t;
+      L *= p;
+    }
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[m] = j + m * TL < nn ? src[j + m * TL] : cpx<T>{T(0), T(0)};
+    __syncthreads();  // the next transform writes the buffers again
+  }
+};
+
+template <typename T, int R>
 struct Coef {
   T c[7][R];  // E, c1, c2, c3, c4, c5 (scheme-specific), g/N signed
 };
 
-template <typename T, int N, int R>
-__device__ __forceinline__ void load_coef(const T* tab, int j, Coef<T, N, R>& co) {
+template <class X, typename T>
+__device__ __forceinline__ void load_coef(const X& x, const T* tab, Coef<T, X::R>& co) {
+  const int N = x.n();
   const T invn = T(1) / T(N);
 #pragma unroll
-  for (int m = 0; m < R; ++m) {
-    int k = spec_index<N, R>(m, j);
+  for (int m = 0; m < X::R; ++m) {
+    int k = x.spec(m);
+    if (X::kMasked && k >= N) {
+#pragma unroll
+      for (int c = 0; c < 7; ++c) co.c[c][m] = T(0);
+      continue;
+    }
     int af = k <= N / 2 ? k : N - k;
     const T* row = tab + af * kTableStride;
 #pragma unroll
@@ -504,16 +713,19 @@ __device__ __forceinline__ void load_coef(const T* tab, int j, Coef<T, N, R>& co
 }
 
 // Pair of rows (b0, b1) in HBM <-> packed strided registers.
-template <typename T, int N, int R>
-__device__ __forceinline__ void load_pair(const T* base, size_t stride_b, int b0, int b1, int j,
-                                          cpx<T>* v) {
-  constexpr int TL = N / R;
+template <class X, typename T>
+__device__ __forceinline__ void load_pair(const X& x, const T* base, size_t stride_b, int b0, int b1, cpx<T>* v) {
+  constexpr int TL = X::TL;
   const T* r0 = base + size_t(b0) * stride_b;
   const T* r1 = b1 >= 0 ? base + size_t(b1) * stride_b : nullptr;
 #pragma unroll
-  for (int m = 0; m < R; ++m) {
-    v[m].x = r0[j + m * TL];
-    v[m].y = r1 ? r1[j + m * TL] : T(0);
+  for (int m = 0; m < X::R; ++m) {
+    if (X::kMasked && x.j + m * TL >= x.n()) {
+      v[m] = cpx<T>{T(0), T(0)};
+      continue;
+    }
+    v[m].x = r0[x.j + m * TL];
+    v[m].y = r1 ? r1[x.j + m * TL] : T(0);
   }
 }
 
@@ -521,57 +733,60 @@ __device__ __forceinline__ void load_pair(const T* base, size_t stride_b, int b0
 // has no second trajectory; the caller ignores that half).  A load whose
 // value merges from a branch is copied at the join, and the copy waits for
 // it: this form keeps a prefetch in flight.
-template <typename T, int N, int R>
-__device__ __forceinline__ void load_pair_both(const T* base, size_t stride_b, int b0, int b1, int j,
+template <class X, typename T>
+__device__ __forceinline__ void load_pair_both(const X& x, const T* base, size_t stride_b, int b0, int b1,
                                                cpx<T>* v) {
-  constexpr int TL = N / R;
+  constexpr int TL = X::TL;
   const T* r0 = base + size_t(b0) * stride_b;
   const T* r1 = base + size_t(b1 >= 0 ? b1 : b0) * stride_b;
 #pragma unroll
-  for (int m = 0; m < R; ++m) {
-    v[m].x = r0[j + m * TL];
-    v[m].y = r1[j + m * TL];
+  for (int m = 0; m < X::R; ++m) {
+    if (X::kMasked && x.j + m * TL >= x.n()) {
+      v[m] = cpx<T>{T(0), T(0)};
+      continue;
+    }
+    v[m].x = r0[x.j + m * TL];
+    v[m].y = r1[x.j + m * TL];
   }
 }
 
-template <typename T, int N, int R>
-__device__ __forceinline__ void store_pair(T* base, size_t stride_b, int b0, int b1, int j,
-                                           const cpx<T>* v) {
-  constexpr int TL = N / R;
+template <class X, typename T>
+__device__ __forceinline__ void store_pair(const X& x, T* base, size_t stride_b, int b0, int b1, const cpx<T>* v) {
+  constexpr int TL = X::TL;
   T* r0 = base + size_t(b0) * stride_b;
 #pragma unroll
-  for (int m = 0; m < R; ++m) r0[j + m * TL] = v[m].x;
+  for (int m = 0; m < X::R; ++m)
+    if (!X::kMasked || x.j + m * TL < x.n()) r0[x.j + m * TL] = v[m].x;
   if (b1 >= 0) {
     T* r1 = base + size_t(b1) * stride_b;
 #pragma unroll
-    for (int m = 0; m < R; ++m) r1[j + m * TL] = v[m].y;
+    for (int m = 0; m < X::R; ++m)
+      if (!X::kMasked || x.j + m * TL < x.n()) r1[x.j + m * TL] = v[m].y;
   }
 }
 
 // nonlinear term in the scaled convention: G~ * F(z^2) with G~ = -i g/N
-template <typename T, int N, int R>
-__device__ __forceinline__ void square_fft_g(cpx<T>* z, const Coef<T, N, R>& co, const cpx<T>* tw,
-                                             XBuf<T>& lds, int j) {
+template <class X, typename T>
+__device__ __forceinline__ void square_fft_g(X& x, cpx<T>* z, const Coef<T, X::R>& co) {
 #pragma unroll
-  for (int m = 0; m < R; ++m) z[m] = cpx<T>{z[m].x * z[m].x, z[m].y * z[m].y};
-  fft<T, N, R, -1>(z, tw, lds, j);
+  for (int m = 0; m < X::R; ++m) z[m] = cpx<T>{z[m].x * z[m].x, z[m].y * z[m].y};
+  x.template fft<-1>(z);
 #pragma unroll
-  for (int m = 0; m < R; ++m) {
+  for (int m = 0; m < X::R; ++m) {
     T g = co.c[6][m];
     z[m] = cpx<T>{g * z[m].y, -g * z[m].x};
   }
 }
 
 // w-bar = Re F^H(conj(G~) X): multiply by +i g/N, inverse transform
-template <typename T, int N, int R>
-__device__ __forceinline__ void adj_g_ifft(const cpx<T>* X, cpx<T>* y, const Coef<T, N, R>& co,
-                                           const cpx<T>* tw, XBuf<T>& lds, int j) {
+template <class X, typename T>
+__device__ __forceinline__ void adj_g_ifft(X& x, const cpx<T>* Xs, cpx<T>* y, const Coef<T, X::R>& co) {
 #pragma unroll
-  for (int m = 0; m < R; ++m) {
+  for (int m = 0; m < X::R; ++m) {
     T g = co.c[6][m];
-    y[m] = cpx<T>{-g * X[m].y, g * X[m].x};
+    y[m] = cpx<T>{-g * Xs[m].y, g * Xs[m].x};
   }
-  fft<T, N, R, +1>(y, tw, lds, j);
+  x.template fft<+1>(y);
 }
 
 template <typename T>
@@ -586,34 +801,31 @@ __device__ __forceinline__ int pair_rows(const KArgs<T>& a, int p, int& b0, int&
   return a.table_idx ? a.table_idx[b0] : 0;
 }
 
-template <typename T, int N, int R, bool LOSS>
+template <typename T, class X, bool LOSS>
 __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
-  using FC = FftCfg<T, N, R>;
-  constexpr int TL = N / R;
-  constexpr int TPB = TL >= 64 ? 1 : 64 / TL;
-  __shared__ cpx<T> lds_all[FC::LDS_TEAM > 0 ? TPB * FC::LDS_TEAM : 1];
+  constexpr int R = X::R, TL = X::TL, TPB = X::TPB;
+  __shared__ cpx<T> lds_all[X::LDS > 0 ? TPB * X::LDS : 1];
   const int team = threadIdx.x / TL, j = threadIdx.x % TL;
   const int p = blockIdx.x * TPB + team;
   if (p >= a.npairs) return;
-  XBuf<T> lds{lds_all + team * FC::LDS_TEAM, 0};
   int b0, b1;
   const int tab = pair_rows(a, p, b0, b1);
-  cpx<T> tw[FC::NTW];
-  load_reg_twiddles<T, N, R>(a.twiddle, j, tw);
-  Coef<T, N, R> co;
-  load_coef<T, N, R>(a.table + size_t(tab) * (N / 2 + 1) * kTableStride, j, co);
+  X x(lds_all + team * X::LDS, j, a);
+  const int N = x.n();
+  Coef<T, R> co;
+  load_coef(x, a.table + size_t(tab) * (N / 2 + 1) * kTableStride, co);
   const T invn = T(1) / T(N);
   const int ns = stages_per_sub(a.scheme);
   const int nst = ns * a.substeps;
 
   cpx<T> V[R], S[R], z[R];
-  load_pair<T, N, R>(a.u, N, b0, b1, j, V);
-  fft<T, N, R, -1>(V, tw, lds, j);
+  load_pair(x, a.u, N, b0, b1, V);
+  x.template fft<-1>(V);
 #pragma unroll
   for (int m = 0; m < R; ++m) V[m] = invn * V[m];
   if (a.s) {
-    load_pair<T, N, R>(a.s, N, b0, b1, j, S);
-    fft<T, N, R, -1>(S, tw, lds, j);
+    load_pair(x, a.s, N, b0, b1, S);
+    x.template fft<-1>(S);
 #pragma unroll
     for (int m = 0; m < R; ++m) S[m] = invn * S[m];
   } else {
@@ -621,8 +833,7 @@ __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
     for (int m = 0; m < R; ++m) S[m] = cpx<T>{T(0), T(0)};
   }
   auto stage_out = [&](int step, int st, const cpx<T>* zz) {
-    if (a.stages) store_pair<T, N, R>(a.stages + (size_t(step) * nst + st) * N, size_t(a.steps) * nst * N,
-                                      b0, b1, j, zz);
+    if (a.stages) store_pair(x, a.stages + (size_t(step) * nst + st) * N, size_t(a.steps) * nst * N, b0, b1, zz);
   };
   // output u^{step+1} (physical): trajectory row and, with a target, the
   // squared-error partial sums of the pair's two trajectories.  The target
@@ -631,7 +842,7 @@ __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
   // its HBM latency stays off the step's chain of transforms.
   T l0 = T(0), l1 = T(0);
   cpx<T> tg[R];
-  if constexpr (LOSS) load_pair_both<T, N, R>(a.target, size_t(a.steps) * N, b0, b1, j, tg);
+  if constexpr (LOSS) load_pair_both(x, a.target, size_t(a.steps) * N, b0, b1, tg);
   // (the squared errors come before the trajectory stores, held there by a
   // scheduling barrier: the wait for the target loads would otherwise also
   // wait for those stores)
@@ -644,13 +855,13 @@ __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
         l1 += dy * dy;
       }
       __builtin_amdgcn_sched_barrier(0);
-      store_pair<T, N, R>(a.traj + size_t(step) * N, size_t(a.steps) * N, b0, b1, j, zz);  // always given
+      store_pair(x, a.traj + size_t(step) * N, size_t(a.steps) * N, b0, b1, zz);  // always given
     } else if (a.traj) {
-      store_pair<T, N, R>(a.traj + size_t(step) * N, size_t(a.steps) * N, b0, b1, j, zz);
+      store_pair(x, a.traj + size_t(step) * N, size_t(a.steps) * N, b0, b1, zz);
     }
     if constexpr (LOSS) {
       const int nxt = step + 1 < a.steps ? step + 1 : step;
-      load_pair_both<T, N, R>(a.target + size_t(nxt) * N, size_t(a.steps) * N, b0, b1, j, tg);
+      load_pair_both(x, a.target + size_t(nxt) * N, size_t(a.steps) * N, b0, b1, tg);
     }
   };
 
@@ -658,34 +869,34 @@ __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
     for (int sub = 0; sub < a.substeps; ++sub) {
 #pragma unroll
       for (int m = 0; m < R; ++m) z[m] = V[m];
-      fft<T, N, R, +1>(z, tw, lds, j);  // physical state
+      x.template fft<+1>(z);  // physical state
       if (sub == 0 && step > 0) state_out(step - 1, z);
       int st = sub * ns;
       stage_out(step, st++, z);
       if (a.scheme == INC_ETD1) {
         // v' = E v + phi1 (N(v) + s)   (solver_1d.py:249-255)
-        square_fft_g<T, N, R>(z, co, tw, lds, j);
+        square_fft_g(x, z, co);
 #pragma unroll
         for (int m = 0; m < R; ++m) V[m] = co.c[0][m] * V[m] + co.c[1][m] * (z[m] + S[m]);
       } else if (a.scheme == INC_ETRK2) {
         // solver_1d.py:269-278
         cpx<T> N1[R], M[R];
-        square_fft_g<T, N, R>(z, co, tw, lds, j);
+        square_fft_g(x, z, co);
 #pragma unroll
         for (int m = 0; m < R; ++m) {
           N1[m] = z[m];
           M[m] = co.c[0][m] * V[m] + co.c[1][m] * (N1[m] + S[m]);
           z[m] = M[m];
         }
-        fft<T, N, R, +1>(z, tw, lds, j);
+        x.template fft<+1>(z);
         stage_out(step, st++, z);
-        square_fft_g<T, N, R>(z, co, tw, lds, j);
+        square_fft_g(x, z, co);
 #pragma unroll
         for (int m = 0; m < R; ++m) V[m] = M[m] + co.c[2][m] * (z[m] - N1[m]);
       } else {
         // ETDRK4: E, E2, Q, f1, f2, f3 in c[0..5]
         cpx<T> Nv[R], A[R], acc[R];
-        square_fft_g<T, N, R>(z, co, tw, lds, j);
+        square_fft_g(x, z, co);
 #pragma unroll
         for (int m = 0; m < R; ++m) {
           Nv[m] = z[m] + S[m];
@@ -693,27 +904,27 @@ __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
           acc[m] = co.c[0][m] * V[m] + co.c[3][m] * Nv[m];
           z[m] = A[m];
         }
-        fft<T, N, R, +1>(z, tw, lds, j);
+        x.template fft<+1>(z);
         stage_out(step, st++, z);
-        square_fft_g<T, N, R>(z, co, tw, lds, j);
+        square_fft_g(x, z, co);
 #pragma unroll
         for (int m = 0; m < R; ++m) {
           cpx<T> Na = z[m] + S[m];
           acc[m] = acc[m] + (T(2) * co.c[4][m]) * Na;
           z[m] = co.c[1][m] * V[m] + co.c[2][m] * Na;  // b
         }
-        fft<T, N, R, +1>(z, tw, lds, j);
+        x.template fft<+1>(z);
         stage_out(step, st++, z);
-        square_fft_g<T, N, R>(z, co, tw, lds, j);
+        square_fft_g(x, z, co);
 #pragma unroll
         for (int m = 0; m < R; ++m) {
           cpx<T> Nb = z[m] + S[m];
           acc[m] = acc[m] + (T(2) * co.c[4][m]) * Nb;
           z[m] = co.c[1][m] * A[m] + co.c[2][m] * (T(2) * Nb - Nv[m]);  // c
         }
-        fft<T, N, R, +1>(z, tw, lds, j);
+        x.template fft<+1>(z);
         stage_out(step, st++, z);
-        square_fft_g<T, N, R>(z, co, tw, lds, j);
+        square_fft_g(x, z, co);
 #pragma unroll
         for (int m = 0; m < R; ++m) V[m] = acc[m] + co.c[5][m] * (z[m] + S[m]);
       }
@@ -721,28 +932,25 @@ __global__ void __launch_bounds__(256) ks_fwd_kernel(KArgs<T> a) {
   }
 #pragma unroll
   for (int m = 0; m < R; ++m) z[m] = V[m];
-  fft<T, N, R, +1>(z, tw, lds, j);
+  x.template fft<+1>(z);
   state_out(a.steps - 1, z);
-  store_pair<T, N, R>(a.out, N, b0, b1, j, z);
+  store_pair(x, a.out, N, b0, b1, z);
   if constexpr (LOSS) write_loss_parts<TL>(l0, l1, j, b0, b1, a.loss_parts);
 }
 
-template <typename T, int N, int R, bool LOSS>
+template <typename T, class X, bool LOSS>
 __global__ void __launch_bounds__(256) ks_adj_kernel(KArgs<T> a) {
-  using FC = FftCfg<T, N, R>;
-  constexpr int TL = N / R;
-  constexpr int TPB = TL >= 64 ? 1 : 64 / TL;
-  __shared__ cpx<T> lds_all[FC::LDS_TEAM > 0 ? TPB * FC::LDS_TEAM : 1];
+  constexpr int R = X::R, TL = X::TL, TPB = X::TPB;
+  __shared__ cpx<T> lds_all[X::LDS > 0 ? TPB * X::LDS : 1];
   const int team = threadIdx.x / TL, j = threadIdx.x % TL;
   const int p = blockIdx.x * TPB + team;
   if (p >= a.npairs) return;
-  XBuf<T> lds{lds_all + team * FC::LDS_TEAM, 0};
   int b0, b1;
   const int tab = pair_rows(a, p, b0, b1);
-  cpx<T> tw[FC::NTW];
-  load_reg_twiddles<T, N, R>(a.twiddle, j, tw);
-  Coef<T, N, R> co;
-  load_coef<T, N, R>(a.table + size_t(tab) * (N / 2 + 1) * kTableStride, j, co);
+  X x(lds_all + team * X::LDS, j, a);
+  const int N = x.n();
+  Coef<T, R> co;
+  load_coef(x, a.table + size_t(tab) * (N / 2 + 1) * kTableStride, co);
   const T invn = T(1) / T(N);
   const int ns = stages_per_sub(a.scheme);
   const int nst = ns * a.substeps;
@@ -758,15 +966,15 @@ __global__ void __launch_bounds__(256) ks_adj_kernel(KArgs<T> a) {
   if constexpr (LOSS) lsc = *a.loss_scale;
   auto load_cot = [&](int step, cpx<T>* dst) __attribute__((always_inline)) {
     if (!LOSS || a.g_traj) {
-      load_pair<T, N, R>(a.g_traj + size_t(step) * N, gstride, b0, b1, j, dst);
+      load_pair(x, a.g_traj + size_t(step) * N, gstride, b0, b1, dst);
     } else {
 #pragma unroll
       for (int m = 0; m < R; ++m) dst[m] = cpx<T>{T(0), T(0)};
     }
     if constexpr (LOSS) {
       cpx<T> tr[R], tg[R];
-      load_pair<T, N, R>(a.traj + size_t(step) * N, gstride, b0, b1, j, tr);
-      load_pair<T, N, R>(a.target + size_t(step) * N, gstride, b0, b1, j, tg);
+      load_pair(x, a.traj + size_t(step) * N, gstride, b0, b1, tr);
+      load_pair(x, a.target + size_t(step) * N, gstride, b0, b1, tg);
 #pragma unroll
       for (int m = 0; m < R; ++m) dst[m] = dst[m] + lsc * (tr[m] - tg[m]);
     }
@@ -791,12 +999,12 @@ __global__ void __launch_bounds__(256) ks_adj_kernel(KArgs<T> a) {
     const int step = idx / a.substeps, sub = idx - step * a.substeps;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      if (k < ns) load_pair<T, N, R>(a.stages + (size_t(step) * nst + sub * ns + k) * N, sstride, b0, b1, j, dst[k]);
+      if (k < ns) load_pair(x, a.stages + (size_t(step) * nst + sub * ns + k) * N, sstride, b0, b1, dst[k]);
     if (sub == 0 && step > 0) load_cot(step - 1, ldst);
   };
   const int total = a.steps * a.substeps;
   fetch(total - 1, cur, lcur);
-  fft<T, N, R, -1>(Vb, tw, lds, j);
+  x.template fft<-1>(Vb);
   // Complete the loads above before the loop.  Loads still in flight into
   // `cur` on entry would make the compiler wait for every outstanding load
   // at the loop's first use of `cur`, i.e. for the prefetch issued at the
@@ -804,15 +1012,15 @@ __global__ void __launch_bounds__(256) ks_adj_kernel(KArgs<T> a) {
   __builtin_amdgcn_s_waitcnt(0);
 
   // z-bar = 2 z * Re F^H(conj(G~) X) (+ loss cotangent), then F(z-bar)
-  auto back_nl = [&](const cpx<T>* X, const cpx<T>* zs, bool with_loss, cpx<T>* out) {
-    adj_g_ifft<T, N, R>(X, y, co, tw, lds, j);
+  auto back_nl = [&](const cpx<T>* Xs, const cpx<T>* zs, bool with_loss, cpx<T>* out) {
+    adj_g_ifft(x, Xs, y, co);
 #pragma unroll
     for (int m = 0; m < R; ++m) out[m] = cpx<T>{T(2) * zs[m].x * y[m].x, T(2) * zs[m].y * y[m].y};
     if (with_loss) {
 #pragma unroll
       for (int m = 0; m < R; ++m) out[m] = out[m] + lcur[m];
     }
-    fft<T, N, R, -1>(out, tw, lds, j);
+    x.template fft<-1>(out);
   };
 
   for (int idx = total - 1; idx >= 0; --idx) {
@@ -904,15 +1112,15 @@ __global__ void __launch_bounds__(256) ks_adj_kernel(KArgs<T> a) {
     }
   }
   // u^0 = F^H-side of V^0 = F(u^0)/N:  u-bar = Re F^H(V-bar) / N
-  fft<T, N, R, +1>(Vb, tw, lds, j);
+  x.template fft<+1>(Vb);
 #pragma unroll
   for (int m = 0; m < R; ++m) Vb[m] = invn * Vb[m];
-  store_pair<T, N, R>(a.out, N, b0, b1, j, Vb);
+  store_pair(x, a.out, N, b0, b1, Vb);
   if (a.g_s) {
-    fft<T, N, R, +1>(Sb, tw, lds, j);
+    x.template fft<+1>(Sb);
 #pragma unroll
     for (int m = 0; m < R; ++m) Sb[m] = invn * Sb[m];
-    store_pair<T, N, R>(a.g_s, N, b0, b1, j, Sb);
+    store_pair(x, a.g_s, N, b0, b1, Sb);
   }
 }
 
@@ -995,13 +1203,13 @@ __global__ void ks_table_kernel(int scheme, int n, double dt, int dealias, const
 }
 
 // ---- host dispatch -----------------------------------------------------------------
-template <typename T, int N, int R>
+template <typename T, class X>
 void* pick_kernel(bool adj, bool loss) {
   if (loss)
-    return adj ? reinterpret_cast<void*>(&ks_adj_kernel<T, N, R, true>)
-               : reinterpret_cast<void*>(&ks_fwd_kernel<T, N, R, true>);
-  return adj ? reinterpret_cast<void*>(&ks_adj_kernel<T, N, R, false>)
-             : reinterpret_cast<void*>(&ks_fwd_kernel<T, N, R, false>);
+    return adj ? reinterpret_cast<void*>(&ks_adj_kernel<T, X, true>)
+               : reinterpret_cast<void*>(&ks_fwd_kernel<T, X, true>);
+  return adj ? reinterpret_cast<void*>(&ks_adj_kernel<T, X, false>)
+             : reinterpret_cast<void*>(&ks_fwd_kernel<T, X, false>);
 }
 
 // Radix choices: R points per lane, TL = N / R lanes per trajectory pair
@@ -1029,11 +1237,11 @@ int default_radix(int n, int npairs) {
 template <typename T, int N>
 void* pick_r(int r, bool adj, bool loss) {
   if constexpr (N / 2 <= 256)
-    if (r == 2) return pick_kernel<T, N, 2>(adj, loss);
+    if (r == 2) return pick_kernel<T, RegX<T, N, 2>>(adj, loss);
   if constexpr (N / 4 <= 256)
-    if (r == 4) return pick_kernel<T, N, 4>(adj, loss);
+    if (r == 4) return pick_kernel<T, RegX<T, N, 4>>(adj, loss);
   if constexpr (!std::is_same<T, double>::value && N / 8 <= 256)
-    if (r == 8) return pick_kernel<T, N, 8>(adj, loss);
+    if (r == 8) return pick_kernel<T, RegX<T, N, 8>>(adj, loss);
   return nullptr;
 }
 
@@ -1051,6 +1259,43 @@ void* pick_n(int n, int r, bool adj, bool loss) {
   }
 }
 
+// Resolutions of the mixed-radix transform: any n in [kMinN, kMaxN] that is
+// not a power of two in [16, 1024] (those run the register transform).
+constexpr int kMinN = 4, kMaxN = 1024;
+bool is_reg_n(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; }
+
+// Passes of the mixed-radix transform as 4-bit plan_radix codes, first pass
+// in the low bits: radix 4 while 4 divides n, then 2, the primes 3 .. 31,
+// and last the one prime factor above 31 a number <= 1024 can have (code
+// 15).  At most 10 passes.
+uint64_t mix_plan(int n) {
+  // (radix, plan_radix code) of the odd primes up to 31
+  static const int odd[10][2] = {{3, 2},  {5, 4},  {7, 5},  {11, 6},  {13, 7},
+                                 {17, 8}, {19, 9}, {23, 10}, {29, 11}, {31, 12}};
+  uint64_t code = 0;
+  int s = 0, m = n;
+  auto push = [&](int c) { code |= uint64_t(c) << (4 * s++); };
+  while (m % 4 == 0) { push(3); m /= 4; }
+  while (m % 2 == 0) { push(1); m /= 2; }
+  for (int i = 0; i < 10; ++i)
+    while (m % odd[i][0] == 0) { push(odd[i][1]); m /= odd[i][0]; }
+  if (m > 1) push(15);
+  return code;
+}
+
+// Team of the mixed-radix transform: two points per lane on one, two or
+// four wavefronts up to 512 points, four per lane on four wavefronts up to
+// 1024.
+int mix_team(int n) { return n <= 128 ? 64 : (n <= 256 ? 128 : 256); }
+
+template <typename T>
+void* pick_mix(int n, bool adj, bool loss) {
+  if (n <= 128) return pick_kernel<T, MixX<T, 64, 2>>(adj, loss);
+  if (n <= 256) return pick_kernel<T, MixX<T, 128, 2>>(adj, loss);
+  if (n <= 512) return pick_kernel<T, MixX<T, 256, 2>>(adj, loss);
+  return pick_kernel<T, MixX<T, 256, 4>>(adj, loss);
+}
+
 // Fused MSE arguments (see KArgs): target trajectory, forward partial sums,
 // adjoint scale.
 struct Mse {
@@ -1062,8 +1307,8 @@ struct Mse {
 template <typename T>
 int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int steps, void* traj,
               void* stages, const void* g_traj, void* g_s, bool adj, void* stream, const Mse& mse) {
-  INC_REQUIRE(d->n >= 16 && d->n <= 1024 && (d->n & (d->n - 1)) == 0, INC_EUNSUPPORTED,
-              "KS resolution must be a power of two in [16, 1024] (got %d)", d->n);
+  INC_REQUIRE(d->n >= kMinN && d->n <= kMaxN, INC_EUNSUPPORTED, "KS resolution must be in [%d, %d] (got %d)",
+              kMinN, kMaxN, d->n);
   INC_REQUIRE(d->scheme >= INC_ETD1 && d->scheme <= INC_ETDRK4, INC_EINVAL, "unknown KS time scheme %d",
               d->scheme);
   INC_REQUIRE(d->substeps >= 1, INC_EINVAL, "substeps must be >= 1");
@@ -1079,9 +1324,19 @@ int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int
               "the MSE forward needs loss_parts and traj");
   const int packed = d->table_idx == nullptr ? 1 : 0;
   const int npairs = packed ? (d->batch + 1) / 2 : d->batch;
-  int r = d->radix > 0 ? d->radix : default_radix(d->n, npairs);
-  if (std::is_same<T, double>::value && r != 2 && r != 4) r = default_radix(d->n, npairs);
-  void* fn = pick_n<T>(d->n, r, adj, mse.target != nullptr);
+  const bool reg = is_reg_n(d->n);
+  int r = 0, tl = 0;
+  void* fn = nullptr;
+  if (reg) {
+    r = d->radix > 0 ? d->radix : default_radix(d->n, npairs);
+    if (std::is_same<T, double>::value && r != 2 && r != 4) r = default_radix(d->n, npairs);
+    fn = pick_n<T>(d->n, r, adj, mse.target != nullptr);
+    tl = d->n / r;
+  } else {
+    // the mixed-radix teams take no radix choice
+    fn = pick_mix<T>(d->n, adj, mse.target != nullptr);
+    tl = mix_team(d->n);
+  }
   INC_REQUIRE(fn != nullptr, INC_EUNSUPPORTED, "no KS kernel for n=%d radix=%d", d->n, r);
   KArgs<T> a{};
   a.batch = d->batch;
@@ -1104,7 +1359,8 @@ int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int
   a.loss_scale = static_cast<const T*>(mse.scale);
   a.twiddle = static_cast<const T*>(d->twiddle);
   INC_REQUIRE(a.twiddle != nullptr, INC_EINVAL, "twiddle table is NULL");
-  const int tl = d->n / r;
+  a.n = d->n;
+  a.plan = reg ? 0 : mix_plan(d->n);
   const int tpb = tl >= 64 ? 1 : 64 / tl;
   dim3 grid((a.npairs + tpb - 1) / tpb), block(tl >= 64 ? tl : 64);
   void* args[] = {&a};
@@ -1132,8 +1388,8 @@ extern "C" {
 int inc_ks_tables(int dtype, int scheme, int n, double dt, int dealias, const double* L, int count,
                   void* table_out, void* stream) {
   inc::clear_error();
-  INC_REQUIRE(n >= 16 && n <= 1024 && (n & (n - 1)) == 0, INC_EUNSUPPORTED,
-              "KS resolution must be a power of two in [16, 1024] (got %d)", n);
+  INC_REQUIRE(n >= inc::kMinN && n <= inc::kMaxN, INC_EUNSUPPORTED, "KS resolution must be in [%d, %d] (got %d)",
+              inc::kMinN, inc::kMaxN, n);
   INC_REQUIRE(scheme >= INC_ETD1 && scheme <= INC_ETDRK4, INC_EINVAL, "unknown KS time scheme %d", scheme);
   INC_REQUIRE(count >= 1 && L && table_out, INC_EINVAL, "need at least one domain size and an output buffer");
   int total = count * (n / 2 + 1);

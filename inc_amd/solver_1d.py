@@ -1087,8 +1087,9 @@ class KSSolverTorch:
     ``time_scheme`` 'ETD1' / 'ETRK2' reproduce the reference (including the
     double step, ``doublestep``); 'ETDRK4' adds Cox-Matthews ETDRK4 with
     contour-integral coefficients and, when ``sim_params.dealias`` (default
-    True), the 2/3-rule mask on the nonlinear term.  ``resolution`` must be a
-    power of two in [16, 1024].
+    True), the 2/3-rule mask on the nonlinear term.  ``resolution`` is any
+    size in [4, 1024]: powers of two from 16 run the register-transpose FFT,
+    the others a mixed-radix transform through LDS.
     """
 
     def __init__(self, sim_params):

@@ -325,7 +325,9 @@ int inc_burgers_ragged_adj(const inc_ragged_desc* d, const void* stages, const v
 
 /*
  * Description of a KS problem (KSSolverTorch, solver_1d.py:198-212).
- * n must be a power of two, 16 <= n <= 1024.
+ * 4 <= n <= 1024: powers of two from 16 run the register-transpose FFT
+ * (radix chosen by `radix`), every other n a mixed-radix transform through
+ * LDS (`radix` ignored).
  */
 typedef struct inc_ks_desc {
   int32_t dtype;      /* INC_F32 / INC_F64 */

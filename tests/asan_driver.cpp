@@ -83,11 +83,14 @@ int main() {
   k.dtype = INC_F32;
   k.scheme = INC_ETDRK4;
   k.batch = 1;
-  k.n = 48;
+  k.n = 2048;
   k.substeps = 1;
   expect(inc_ks_fwd(&k, nullptr, nullptr, nullptr, 1, nullptr, nullptr, nullptr) == INC_EUNSUPPORTED &&
-             has("power of two"),
+             has("resolution"),
          "ks resolution");
+  k.n = 3;
+  expect(inc_ks_fwd(&k, nullptr, nullptr, nullptr, 1, nullptr, nullptr, nullptr) == INC_EUNSUPPORTED,
+         "ks resolution below the smallest transform");
   k.n = 64;
   k.scheme = INC_ETD1;
   k.table = one;

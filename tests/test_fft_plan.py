@@ -11,6 +11,8 @@ mode held by (register m, lane j) afterwards is the spectral index map that
 the kernels' coefficient load follows.  This is the algorithm of
 inc_amd/csrc/ks.hip (RPlan, spec_bits, load_reg_twiddles, reg_fft_fwd/inv);
 the kernels themselves are checked against the oracle by the GPU tests.
+The mixed-radix transform of the other resolutions (MixX, mix_plan,
+plan_radix, mix_pass, mix_pass_any) is restated at the end of this file.
 """
 import numpy as np
 import pytest
@@ -140,3 +142,74 @@ def test_plan_shapes():
     assert plan(256, 4)[2:] == ([2, 2, 2, 2], [6, 4, 2, 0])
     assert plan(128, 4)[2:] == ([2, 2, 2, 1], [5, 3, 1, 0])
     assert plan(1024, 4)[2:] == ([2, 2, 2, 2, 2], [8, 6, 4, 2, 0])
+
+
+# ---- the mixed-radix transform (MixX in ks.hip) for every other n ----------------
+# (radix, 4-bit pass code) as mix_plan encodes them and plan_radix decodes them
+_ODD = [(3, 2), (5, 4), (7, 5), (11, 6), (13, 7), (17, 8), (19, 9), (23, 10), (29, 11), (31, 12)]
+_DECODE = {1: 2, 2: 3, 3: 4, 4: 5, 5: 7, 6: 11, 7: 13, 8: 17, 9: 19, 10: 23, 11: 29, 12: 31}
+
+
+def mix_plan(n):
+    codes, m = [], n
+    while m % 4 == 0:
+        codes.append(3)
+        m //= 4
+    while m % 2 == 0:
+        codes.append(1)
+        m //= 2
+    for p, c in _ODD:
+        while m % p == 0:
+            codes.append(c)
+            m //= p
+    if m > 1:
+        codes.append(15)
+    return codes
+
+
+def mix_radices(n):
+    out, L = [], 1
+    for c in mix_plan(n):
+        p = _DECODE[c] if c != 15 else n // L
+        out.append(p)
+        L *= p
+    return out
+
+
+def mix_fft(x, sign):
+    """Stockham decimation in time: (L, M) -> (L p, M / p) per pass with
+    X'[(k + L t) M' + r] = sum_q w_p^{q t} w_n^{q k M'} X[(k p + q) M' + r]."""
+    n = len(x)
+    w = np.exp(-2j * np.pi * np.arange(n) / n)
+    if sign > 0:
+        w = np.conj(w)
+    a, L = x.astype(complex), 1
+    for p in mix_radices(n):
+        mp = n // (L * p)
+        out = np.empty(n, complex)
+        for k in range(L):
+            for r in range(mp):
+                v = np.array([a[(k * p + q) * mp + r] * w[q * k * mp] for q in range(p)])
+                for t in range(p):
+                    out[(k + L * t) * mp + r] = (v * w[(np.arange(p) * t * (n // p)) % n]).sum()
+        a, L = out, L * p
+    return a
+
+
+def test_mixed_radix_plan_covers_every_size():
+    # every n in [4, 1024] factors into at most 16 passes whose product is n,
+    # with code 15 only for the one prime factor above 31
+    for n in range(4, 1025):
+        codes = mix_plan(n)
+        assert len(codes) <= 16
+        assert int(np.prod(mix_radices(n))) == n
+        if 15 in codes:
+            assert codes.index(15) == len(codes) - 1 and mix_radices(n)[-1] > 31
+
+
+@pytest.mark.parametrize("n", [4, 6, 8, 12, 75, 77, 96, 100, 127, 243, 250, 384])
+def test_mixed_radix_transform_matches_numpy(n):
+    g = np.random.default_rng(n)
+    x = g.standard_normal(n) + 1j * g.standard_normal(n)
+    np.testing.assert_allclose(mix_fft(x, -1), np.fft.fft(x), rtol=0, atol=1e-12 * n)
+    np.testing.assert_allclose(mix_fft(x, +1), np.fft.ifft(x) * n, rtol=0, atol=1e-12 * n)

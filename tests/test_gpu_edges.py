@@ -46,9 +46,9 @@ def test_smallest_and_largest_sizes(cuda):
 
 
 def test_unsupported_sizes_raise_value_error(cuda):
-    ks = KSSolverTorch(KSSimParams([22.0], resolution=48, dt=0.25, time_scheme="ETD1", device="cuda"))
-    with pytest.raises(ValueError, match="power of two"):
-        ks.step(torch.zeros(2, 48, device=cuda), torch.tensor([22.0], device=cuda))
+    ks = KSSolverTorch(KSSimParams([22.0], resolution=2048, dt=0.25, time_scheme="ETD1", device="cuda"))
+    with pytest.raises(ValueError, match="resolution"):
+        ks.step(torch.zeros(2, 2048, device=cuda), torch.tensor([22.0], device=cuda))
     sv = BurgersSolverTorch(BGSimParams(dt=0.01, resolution=4, batch_size=1, device="cuda"))
     with pytest.raises(ValueError, match=">= 8"):
         sv.step(torch.zeros(1, 4, device=cuda))

@@ -53,10 +53,11 @@ def test_argument_errors_map_to_codes_without_launch():
     # unknown equation type
     d = L.BurgersDesc(dtype=L.INC_F32, batch=1, n=64, dx=0.1, source=9)
     assert lib.inc_burgers_fwd(ctypes.byref(d), None, None, None, 1, None, None, None) == L.INC_EINVAL
-    # KS resolution not a power of two
-    k = L.KSDesc(dtype=L.INC_F32, scheme=L.INC_ETDRK4, batch=1, n=48, substeps=1)
-    rc = lib.inc_ks_fwd(ctypes.byref(k), None, None, None, 1, None, None, None)
-    assert rc == L.INC_EUNSUPPORTED and b"power of two" in lib.inc_last_error()
+    # KS resolution outside [4, 1024] (every size inside has a transform)
+    for n in (3, 1025):
+        k = L.KSDesc(dtype=L.INC_F32, scheme=L.INC_ETDRK4, batch=1, n=n, substeps=1)
+        rc = lib.inc_ks_fwd(ctypes.byref(k), None, None, None, 1, None, None, None)
+        assert rc == L.INC_EUNSUPPORTED and b"resolution" in lib.inc_last_error()
     # adjoint without saved stages
     k = L.KSDesc(dtype=L.INC_F32, scheme=L.INC_ETD1, batch=1, n=64, substeps=1, table=1)
     assert lib.inc_ks_adj(ctypes.byref(k), None, None, None, None, 1, None) == L.INC_EINVAL

@@ -53,11 +53,12 @@ def test_ks_gradient_matches_reference_autograd(golden, key):
     np.testing.assert_allclose(ss.grad, g[f"{key}_gs"], rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("n", [64, 75, 96])
 @pytest.mark.parametrize("scheme", ["ETD1", "ETRK2", "ETDRK4"])
-def test_single_mode_linear_decay(scheme):
+def test_single_mode_linear_decay(scheme, n):
     # u = eps*cos(k x): the nonlinear term is O(eps^2); the mode evolves by
-    # e^{(k^2-k^4) dt} per sub-step.
-    n, L, dt = 64, 22.0, 0.25
+    # e^{(k^2-k^4) dt} per sub-step (odd and non-power-of-two n included).
+    L, dt = 22.0, 0.25
     so = KSOracle(ks_params([L], resolution=n, dt=dt, time_scheme=scheme))
     so.doublestep = False
     x = torch.arange(n, dtype=torch.float64) * (L / n)
@@ -241,3 +242,41 @@ def test_initial_conditions_noise_has_unit_frobenius_norm_over_the_batch():
     assert float((noise.norm(dim=1) - 1).abs().max()) > 1e-2
     with pytest.raises(ValueError):
         KSOracle(ks_params([22.0, 30.0], resolution=n, alpha=alpha)).initial_conditions()
+
+
+def _resample(u, m):
+    """Spectral interpolation of a periodic row onto m points (the modes
+    |k| < min(n, m) / 2 kept)."""
+    n = u.shape[-1]
+    U = torch.fft.fft(u)
+    h = (min(n, m) - 1) // 2
+    out = torch.zeros(m, dtype=torch.complex128)
+    out[:h + 1] = U[:h + 1]
+    out[m - h:] = U[n - h:]
+    return torch.fft.ifft(out).real * (m / n)
+
+
+@pytest.mark.parametrize("n", [75, 96, 127, 150])
+@pytest.mark.parametrize("scheme", ["ETD1", "ETRK2", "ETDRK4"])
+def test_resolution_independence_of_a_resolved_state(scheme, n):
+    # A smooth state (modes 1 and 2, amplitude 0.5, L = 22) stays resolved
+    # to rounding at every n here over 10 steps: its spectrum falls below
+    # 1e-16 well inside n/3, so neither the grid nor the 2/3 cut matters,
+    # and the solution at odd, prime and mixed-radix n, interpolated to 256
+    # points, is the n = 256 solution.  A wrong wavenumber of any mode
+    # (the negative half of an odd-n spectrum, the Nyquist handling of
+    # an even one) breaks the agreement.
+    L, dt = 22.0, 0.1
+
+    def run(m):
+        x = torch.arange(m, dtype=torch.float64) * (L / m)
+        u = (0.5 * torch.cos(2 * math.pi * x / L) + 0.25 * torch.sin(4 * math.pi * x / L + 0.3))[None]
+        so = KSOracle(ks_params([L], resolution=m, dt=dt, time_scheme=scheme))
+        so.doublestep = False
+        for _ in range(10):
+            u = so.step(u, so.domain_size)
+        return u[0]
+
+    ref = run(256)
+    got = _resample(run(n), 256)
+    assert float((got - ref).norm() / ref.norm()) < 1e-13
