@@ -738,6 +738,109 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
     for (int u = 0; u < U; ++u)
       if (tid + 256 * u < items) *reinterpret_cast<v4f*>(xs + jr[u] * CS + 4 * cq[u]) = v[u];
   };
+  // Epilogue straight from the accumulators: lane (lr, q) of tile (m, t)
+  // holds D[16m + 4q + r][16 (wave PW + t) + lr], so a store writes 16
+  // consecutive positions of four channels; mul is read the same way, all
+  // of a lane's values in flight at once (clamped addresses, unused past
+  // the edges).  Staging the tile through LDS for whole-row stores cost
+  // two barriers per job and measured 4-7% slower.
+  // element (m, t, r) of this lane in job `job` (row length N, first
+  // position i0, output base ybase): its address, or false past the edges
+  auto addr = [&](int job, int N, int i0, size_t ybase, int m, int t, int r,
+                  size_t& at) __attribute__((always_inline)) {
+    const int o = m * 16 + 4 * q + r, i = i0 + p0 + 16 * t + lr;
+    if constexpr (PACK) {
+      const int b = job * cb.rpj + (p0 + lr) / Nu;
+      at = (size_t(b) * cb.yC + cb.o0 + o) * Nu + (p0 + lr) % Nu;
+      return o < Co && b < cb.rows;
+    } else {
+      at = ybase + size_t(o) * N + i;
+      return o < Co && i < N;
+    }
+  };
+  // Activation and stores of a job's values, the activation chosen once
+  // (a branch per element would leave the compiler unable to count the
+  // loads still in flight at each element, and it would wait for all).
+  auto store_all = [&](float (*v)[PW][4], int job, int N, int i0, size_t ybase) __attribute__((always_inline)) {
+    auto each = [&](auto&& f) __attribute__((always_inline)) {
+#pragma unroll
+      for (int m = 0; m < CT; ++m)
+#pragma unroll
+        for (int t = 0; t < PW; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            size_t at;
+            if (addr(job, N, i0, ybase, m, t, r, at)) f(at, v[m][t][r]);
+          }
+    };
+    if (act == INC_ACT_GELU && dy) {
+      each([&](size_t at, float x) __attribute__((always_inline)) {
+        float gv;
+        dy[at] = gelu_and_grad(x, gv);
+        y[at] = gv;
+      });
+    } else if (act == INC_ACT_GELU) {
+      each([&](size_t at, float x) __attribute__((always_inline)) { y[at] = gelu_value(x); });
+    } else if (act == INC_ACT_RELU && dy) {
+      each([&](size_t at, float x) __attribute__((always_inline)) {
+        dy[at] = relu_grad(x);
+        y[at] = relu(x);
+      });
+    } else if (act == INC_ACT_RELU) {
+      each([&](size_t at, float x) __attribute__((always_inline)) { y[at] = relu(x); });
+    } else {
+      each([&](size_t at, float x) __attribute__((always_inline)) { y[at] = x; });
+    }
+  };
+  // the same address clamped into the output (for loads whose values past
+  // the edges go unused)
+  auto clamped = [&](int job, int N, int i0, size_t ybase, int m, int t, int r) __attribute__((always_inline)) {
+    const int o = min(m * 16 + 4 * q + r, Co - 1), i = min(i0 + p0 + 16 * t + lr, N - 1);
+    if constexpr (PACK) {
+      const int b = min(job * cb.rpj + (p0 + lr) / Nu, cb.rows - 1);
+      return (size_t(b) * cb.yC + cb.o0 + o) * Nu + (p0 + lr) % Nu;
+    }
+    return ybase + size_t(o) * N + i;
+  };
+  // the whole reduction in this launch (phase 0): bias, mul, activation.
+  // Its loads come before the job's matrix work and reach here long done
+  // (a load issued after a store is waited for together with it: one
+  // in-order counter for both): this lane's bias values once per block,
+  // and the job's mul values (1 without mul) in pmv.
+  float bv[CT][4];
+#pragma unroll
+  for (int m = 0; m < CT; ++m)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) bv[m][r] = bias ? bias[min(m * 16 + 4 * q + r, Co - 1)] : 0.f;
+  float pmv[CT][PW][4];
+  auto load_mul = [&](int job, int N, int i0, size_t ybase) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < CT; ++m)
+#pragma unroll
+      for (int t = 0; t < PW; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pmv[m][t][r] = mul ? mul[clamped(job, N, i0, ybase, m, t, r)] : 1.f;
+  };
+  auto epilogue = [&](const v4f (*ac)[PW], const float (*mv)[PW][4], int job, int N, int i0,
+                      size_t ybase) __attribute__((always_inline)) {
+    float v[CT][PW][4];
+#pragma unroll
+    for (int m = 0; m < CT; ++m)
+#pragma unroll
+      for (int t = 0; t < PW; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[m][t][r] = (ac[m][t][r] + bv[m][r]) * mv[m][t][r];
+    store_all(v, job, N, i0, ybase);
+  };
+  // A job's phase-0 epilogue runs in the next job, after that job's tile is
+  // staged: its stores (a lane stores only inside the edges, a count the
+  // compiler cannot know) then drain during the next job's matrix work,
+  // where the wait for every outstanding access at the top of each job
+  // would otherwise wait for them.  pacc and the p* fields hold the pending
+  // job (pjob < 0: none).
+  v4f pacc[CT][PW];
+  int pjob = -1, pN = 0, pi0 = 0;
+  size_t pybase = 0;
   v4f nx[U];
   if (int(blockIdx.x) < njobs) fetch(blockIdx.x, nx);
   for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
@@ -747,9 +850,11 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
     __syncthreads();  // the weights are staged and the previous tile consumed
     stage(nx);
     __syncthreads();
+    if (pjob >= 0) epilogue(pacc, pmv, pjob, pN, pi0, pybase);
     // unconditional for the same reason as the loads (the last job fetches
     // its own tile again, unused)
     fetch(min(job + int(gridDim.x), njobs - 1), nx);
+    if (cb.phase == 0) load_mul(job, N, i0, ybase);
     v4f acc[CT][PW];
 #pragma unroll
     for (int m = 0; m < CT; ++m)
@@ -803,88 +908,52 @@ __global__ void __launch_bounds__(256) conv_fwd_mfma_kernel(int njobs, int Ci, i
       mma(a1, b1);
       __builtin_amdgcn_sched_barrier(0);
     }
-    // Epilogue straight from the accumulators: lane (lr, q) of tile (m, t)
-    // holds D[16m + 4q + r][16 (wave PW + t) + lr], so a store writes 16
-    // consecutive positions of four channels; mul is read the same way, all
-    // of a lane's values in flight at once (clamped addresses, unused past
-    // the edges).  Staging the tile through LDS for whole-row stores cost
-    // two barriers per job and measured 4-7% slower.
-    auto store_act = [&](size_t at, float v) __attribute__((always_inline)) {
-      if (act == INC_ACT_GELU) {
-        if (dy) {
-          float gv;
-          dy[at] = gelu_and_grad(v, gv);
-          v = gv;
-        } else {
-          v = gelu_value(v);
-        }
-      } else if (act == INC_ACT_RELU) {
-        if (dy) dy[at] = relu_grad(v);
-        v = relu(v);
-      }
-      y[at] = v;
-    };
-    // element (m, t, r) of this lane: its address, or false past the edges
-    auto addr = [&](int m, int t, int r, size_t& at) __attribute__((always_inline)) {
-      const int o = m * 16 + 4 * q + r, i = i0 + p0 + 16 * t + lr;
-      if constexpr (PACK) {
-        const int b = job * cb.rpj + (p0 + lr) / Nu;
-        at = (size_t(b) * cb.yC + cb.o0 + o) * Nu + (p0 + lr) % Nu;
-        return o < Co && b < cb.rows;
-      } else {
-        at = ybase + size_t(o) * N + i;
-        return o < Co && i < N;
-      }
-    };
-    if (cb.phase == 0) {  // the whole reduction in this launch
-      float mv[CT][PW][4];
+    if (cb.phase == 0) {
 #pragma unroll
       for (int m = 0; m < CT; ++m)
 #pragma unroll
-        for (int t = 0; t < PW; ++t)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int o = min(m * 16 + 4 * q + r, Co - 1), i = min(i0 + p0 + 16 * t + lr, N - 1);
-            size_t at = ybase + size_t(o) * N + i;
-            if constexpr (PACK) {
-              const int b = min(job * cb.rpj + (p0 + lr) / Nu, cb.rows - 1);
-              at = (size_t(b) * cb.yC + cb.o0 + o) * Nu + (p0 + lr) % Nu;
-            }
-            mv[m][t][r] = mul ? mul[at] : 1.f;
-          }
-#pragma unroll
-      for (int m = 0; m < CT; ++m)
-#pragma unroll
-        for (int t = 0; t < PW; ++t)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            size_t at;
-            if (!addr(m, t, r, at)) continue;
-            const int o = m * 16 + 4 * q + r;
-            store_act(at, (acc[m][t][r] + (bias ? bias[o] : 0.f)) * mv[m][t][r]);
-          }
+        for (int t = 0; t < PW; ++t) pacc[m][t] = acc[m][t];
+      pjob = job;
+      pN = N;
+      pi0 = i0;
+      pybase = ybase;
     } else {  // one input-channel block of several, accumulated into y in stream order
+      // the loads first (the running sum in y or the bias, and mul; clamped
+      // addresses past the edges, values unused), then the stores
+      float pv[CT][PW][4], mv[CT][PW][4];
 #pragma unroll
       for (int m = 0; m < CT; ++m)
 #pragma unroll
         for (int t = 0; t < PW; ++t)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            size_t at;
-            if (!addr(m, t, r, at)) continue;
-            const int o = m * 16 + 4 * q + r;
-            if (cb.phase == 1) {  // the first block: its partial sum and the bias
-              y[at] = acc[m][t][r] + (bias ? bias[o] : 0.f);
-            } else if (cb.phase == 2) {  // a middle block
-              y[at] += acc[m][t][r];
-            } else {  // the last block: the sum, the multiplier and the activation
-              float v = y[at] + acc[m][t][r];
-              if (mul) v *= mul[at];
-              store_act(at, v);
-            }
+            const size_t at = clamped(job, N, i0, ybase, m, t, r);
+            pv[m][t][r] = cb.phase == 1 ? (bias ? bias[min(m * 16 + 4 * q + r, Co - 1)] : 0.f) : y[at];
+            mv[m][t][r] = (cb.phase == 3 && mul) ? mul[at] : 1.f;
           }
+      float v[CT][PW][4];
+#pragma unroll
+      for (int m = 0; m < CT; ++m)
+#pragma unroll
+        for (int t = 0; t < PW; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) v[m][t][r] = (acc[m][t][r] + pv[m][t][r]) * mv[m][t][r];
+      if (cb.phase == 3) {  // the last block: multiplier and activation
+        store_all(v, job, N, i0, ybase);
+      } else {  // the first block (with the bias) or a middle one
+#pragma unroll
+        for (int m = 0; m < CT; ++m)
+#pragma unroll
+          for (int t = 0; t < PW; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              size_t at;
+              if (addr(job, N, i0, ybase, m, t, r, at)) y[at] = v[m][t][r];
+            }
+      }
     }
   }
+  if (pjob >= 0) epilogue(pacc, pmv, pjob, pN, pi0, pybase);
 }
 
 int cu_count() {
