@@ -699,6 +699,40 @@ def extra_ks_saturation(b=8192, h=200):
             "fwd_ms": tf * 1e3, "adj_ms": ta * 1e3}
 
 
+def extra_ks_mixed_radix(b=256, h=200, sizes=(250, 256, 384, 512)):
+    """KS ETDRK4 fwd+adj at resolutions that are not powers of two (the
+    mixed-radix transform through LDS) beside the power-of-two ones next to
+    them (the register-transpose transform), L scaled with N."""
+    import torch
+    import inc_amd
+    dev = torch.device("cuda")
+    out = {"workload": f"KS ETDRK4 fwd+adj B={b}, {h} steps, fp32, L = 64 N / 256"}
+    for n in sizes:
+        Ln = 64.0 * n / 256
+        sv = inc_amd.KSSolverTorch(inc_amd.KSSimParams([Ln], resolution=n, dt=DT, time_scheme="ETDRK4", device="cuda"))
+        sv.doublestep = False
+        Ld = torch.tensor([Ln], device=dev)
+        u0 = initial_state(b, n, Ln, 1007).float().to(dev)
+        cot = torch.randn(b, h, n, device=dev) * 1e-3
+
+        def run():
+            u = u0.clone().requires_grad_(True)
+            torch.autograd.backward(sv.rollout(u, h, Ld), cot)
+        run()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(3):
+            a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            run()
+            e.record()
+            torch.cuda.synchronize()
+            best = min(best, a.elapsed_time(e) * 1e-3)
+        kind = "register" if n & (n - 1) == 0 else "mixed-radix"
+        out[f"N{n}"] = {"transform": kind, "ms": best * 1e3, "trajectory_steps_per_s": b * h / best}
+    return out
+
+
 def extra_ks_checkpoint(b=B, h=H, ks=(10, 50, 100)):
     """The headline workload (KS ETDRK4 fwd+adj, B=256, N=256, 1000 steps)
     with checkpoint-and-recompute every k steps in the fused rollout: time
@@ -895,7 +929,7 @@ def main():
     if not args.no_extras:
         out["batched"] = batched_ragged_inc()
         out["other_workloads"] = {"ks_saturated": extra_ks_saturation(), "ks_mse_fused": extra_ks_mse(),
-                                  "ks_checkpoint": extra_ks_checkpoint(),
+                                  "ks_checkpoint": extra_ks_checkpoint(), "ks_mixed_radix": extra_ks_mixed_radix(),
                                   "burgers_saturated": extra_burgers_saturation(), "burgers_inc": extra_burgers_inc(),
                                   "burgers_inc_eval": extra_burgers_inc_eval(),
                                   "ks_inc": extra_ks_inc(), "resnet1d": extra_reference_net("resnet1d"),
