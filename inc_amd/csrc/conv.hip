@@ -552,6 +552,135 @@ __global__ void __launch_bounds__(256) conv_wgrad_mfma_kernel(int njobs, int Ci,
 
 // MFMA weight gradient when the shape fits (Co <= 64, Ci*K + 1 <= 512, LDS);
 // returns INC_EUNSUPPORTED (without touching the error text) otherwise.
+// The same weight gradient with the job tiles copied into LDS
+// asynchronously (global_load_lds: each lane's value lands at the copy's LDS
+// base + 4 * lane, with no register in between), double-buffered: the next
+// job's tiles stream in while this job's MFMAs read the other buffer, so no
+// staging sits between the jobs' matrix work.  Rows of whole 128-position
+// tiles or ragged row tiles (not packed short rows).  Fragments, wavefront
+// roles and partial-sum slots are those of conv_wgrad_mfma_kernel; the input
+// tile is stored flat, xs[c * xw + j] (xw = 128 + K - 1: one copy per 64
+// values), followed by the ones and zeros rows.  Gradient values past the
+// row's end or past Co are written as zeros by the lanes that copy nothing.
+// NW = 8 wavefronts (WT * WP = 8) where the two buffers leave room for one
+// block per CU, so that every SIMD still holds two wavefronts.
+template <int MT, int NT, int NW>
+__global__ void __launch_bounds__(64 * NW) conv_wgrad_async_kernel(int njobs, int Ci, int Co, int Nu, int K, int WT,
+                                                              int WP, const float* __restrict__ g,
+                                                              const float* __restrict__ x, float* __restrict__ part,
+                                                              int accumulate, RowTiles rt, ChanBlock cb) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int P = K / 2;
+  constexpr int GS = kWgTP + 1;  // odd row stride: the 16 rows of a fragment read hit distinct banks
+  const int xw = kWgTP + K - 1;
+  const int gsz = MT * 16 * GS, xsz = (Ci + 2) * xw, bsz = gsz + xsz;
+  float* bufs = reinterpret_cast<float*>(smem_raw);  // two of [gs: MT*16][GS] [xs: Ci+2][xw]
+  const int tid = threadIdx.x, lane = tid & 63;
+  for (int e = tid; e < 2 * xw; e += 64 * NW) {
+    const float v = e < xw ? 1.f : 0.f;
+    bufs[gsz + Ci * xw + e] = v;
+    bufs[bsz + gsz + Ci * xw + e] = v;
+  }
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int tphase = wave % WT, pphase = wave / WT;
+  const bool active = pphase < WP;
+  const int ncw = Ci * K, ntile = (ncw + 1 + 15) / 16;
+  const int lr = lane & 15, lk = lane >> 4;
+  int off[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int tile = tphase + t * WT, col = tile * 16 + lr;
+    off[t] = tile >= ntile || col > ncw ? Ci * xw + xw : (col == ncw ? Ci * xw : (col / K) * xw + col % K);
+  }
+  v4f acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  const int tiles = (Nu + kWgTP - 1) / kWgTP;
+  const int nxc = (Ci * xw + 63) / 64;
+  const float rxw = 1.f / float(xw);
+  // copies of job `job` into buffer `b`, shared round-robin by the wavefronts
+  auto issue = [&](int job, int b) __attribute__((always_inline)) {
+    const JobPos jp = job_pos(rt, job, tiles, kWgTP, Nu);
+    const int N = jp.n, i0 = jp.i0;
+    const float* gb = g + jp.base * cb.gC + size_t(cb.o0) * N;
+    const float* xb = x + jp.base * cb.xC + size_t(cb.c0) * N;
+    float* gsb = bufs + b * bsz;
+    float* xsb = gsb + gsz;
+    for (int ii = wave; ii < MT * 32; ii += NW) {  // gradient: MT*16 rows of two 64-position halves
+      const int o = ii >> 1, j = ((ii & 1) << 6) + lane;
+      float* dst = gsb + o * GS + ((ii & 1) << 6);
+      if (o < Co && i0 + j < N) __builtin_amdgcn_global_load_lds(gb + size_t(o) * N + i0 + j, dst, 4, 0, 0);
+      else dst[lane] = 0.f;
+    }
+    for (int ii = wave; ii < nxc; ii += NW) {  // input: Ci rows of xw points with their periodic halo
+      const int e = ii * 64 + lane;
+      if (e < Ci * xw) {
+        // c = e / xw from a float reciprocal (e < 2^14): (e + 0.5) rxw is within
+        // 2^-21 of (e + 0.5) / xw, whose fraction stays 0.5 / xw from the integers
+        const int c = int((float(e) + 0.5f) * rxw), j = e - c * xw;
+        int pos = i0 - P + j;  // in [-P, N + kWgTP + K): one wrap unless N is small
+        pos = pos < 0 ? pos + N : (pos >= N ? pos - N : pos);
+        if (pos >= N) pos %= N;
+        __builtin_amdgcn_global_load_lds(xb + size_t(c) * N + pos, xsb + ii * 64, 4, 0, 0);
+      }
+    }
+  };
+  int b = 0;
+  if (int(blockIdx.x) < njobs) issue(blockIdx.x, 0);
+  for (int job = blockIdx.x; job < njobs; job += gridDim.x) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wavefront's copies of the job have landed
+    __syncthreads();                 // and every wavefront's; the other buffer's last readers are done
+    if (job + int(gridDim.x) < njobs) issue(job + int(gridDim.x), b ^ 1);
+    if (active) {
+      const float* gs = bufs + b * bsz;
+      const float* xs = gs + gsz;
+      float a[MT], bv[NT], an[MT], bn[NT];
+      auto load = [&](int s, float* fa, float* fb) __attribute__((always_inline)) {
+        const int p = 4 * s + lk;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) fa[m] = gs[(m * 16 + lr) * GS + p];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) fb[t] = xs[off[t] + p];
+      };
+      load(pphase, a, bv);
+      for (int s = pphase; s < kWgTP / 4; s += WP) {
+        load(min(s + WP, kWgTP / 4 - 1), an, bn);  // unconditional: the last one is unused
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int t = 0; t < NT; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], bv[t], acc[m][t], 0, 0, 0);
+#pragma unroll
+        for (int m = 0; m < MT; ++m) a[m] = an[m];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bv[t] = bn[t];
+      }
+    }
+    b ^= 1;
+  }
+  if (!active) return;
+  const int width = Co * ncw + Co;
+  float* out = part + size_t(blockIdx.x * WP + pphase) * width;
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int tile = tphase + t * WT, col = tile * 16 + lr;
+      if (tile >= ntile || col > ncw) continue;  // padding
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int o = m * 16 + 4 * lk + r;
+        if (o >= Co) continue;
+        float* dst = col < ncw ? out + size_t(o) * ncw + col : out + size_t(Co) * ncw + o;
+        *dst = accumulate ? *dst + acc[m][t][r] : acc[m][t][r];
+      }
+    }
+}
+
+int cu_count();
+int blocks_per_cu(const void* fn, size_t lds, int threads = 256);
+
 int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const float* x, float* dw, float* db,
                     float* part, int64_t work_elems, hipStream_t st, const WgradMode& md, ChanBlock cb) {
   const int MT = (Co + 15) / 16;
@@ -572,10 +701,56 @@ int conv_wgrad_mfma(int B, int Ci, int Co, int N, int K, const float* g, const f
   const int WP = int(std::min<int64_t>(4 / WT, slots));
   const int jobs = md.rt.tile ? md.njobs : (pack ? (B + cb.rpj - 1) / cb.rpj : B * ((N + kWgTP - 1) / kWgTP));
   const int nblk = int(std::min<int64_t>(jobs, slots / WP));
-  if (md.nparts) *md.nparts = std::max(*md.nparts, nblk * WP);
-  if (nblk < 1) return INC_OK;  // no active rows: the partial sums stay as they are
-  dim3 grid(nblk), block(256);
+  if (nblk < 1) {  // no active rows: the partial sums stay as they are
+    if (md.nparts) *md.nparts = std::max(*md.nparts, 0);
+    return INC_OK;
+  }
   const int accumulate = md.accumulate ? 1 : 0;
+  // unpacked rows: the asynchronous, double-buffered kernel when two
+  // buffers fit (grid: what the CUs hold at once)
+  const size_t lds_async = 2 * (size_t(MT) * 16 * (kWgTP + 1) + size_t(Ci + 2) * (kWgTP + K - 1)) * sizeof(float);
+  if (!pack && lds_async <= 150 * 1024) {
+    const void* fn = nullptr;
+    // one block per CU at this LDS size: eight wavefronts, positions split in two more phases
+    const bool eight = lds_async > 75 * 1024 && 8 / WT >= 2;
+    const int NWv = eight ? 8 : 4, WPa = eight ? int(std::min<int64_t>(8 / WT, slots)) : WP;
+#define INC_WGA(M, T)                                                                          \
+  fn = eight ? reinterpret_cast<const void*>(&conv_wgrad_async_kernel<M, T, 8>)              \
+             : reinterpret_cast<const void*>(&conv_wgrad_async_kernel<M, T, 4>)
+#define INC_WGA_T(M)                \
+  switch (NT) {                     \
+    case 1: INC_WGA(M, 1); break;   \
+    case 2: INC_WGA(M, 2); break;   \
+    case 3: INC_WGA(M, 3); break;   \
+    case 4: INC_WGA(M, 4); break;   \
+    case 6: INC_WGA(M, 6); break;   \
+    default: INC_WGA(M, 8); break;  \
+  }
+    switch (MT) {
+      case 1: INC_WGA_T(1); break;
+      case 2: INC_WGA_T(2); break;
+      case 3: INC_WGA_T(3); break;
+      default: INC_WGA_T(4); break;
+    }
+#undef INC_WGA_T
+#undef INC_WGA
+    const int per_cu = blocks_per_cu(fn, lds_async, 64 * NWv);
+    if (per_cu >= 1) {
+      const int ablk = int(std::min<int64_t>({int64_t(jobs), slots / WPa, int64_t(cu_count()) * per_cu}));
+      if (md.nparts) *md.nparts = std::max(*md.nparts, ablk * WPa);
+      int nj = jobs, ci = Ci, co = Co, nu = N, kk = K, wt = WT, wp = WPa, accu = accumulate;
+      RowTiles rt = md.rt;
+      void* args[] = {&nj, &ci, &co, &nu, &kk, &wt, &wp, &g, &x, &part, &accu, &rt, &cb};
+      INC_HIP(hipLaunchKernel(fn, dim3(ablk), dim3(64 * NWv), args, lds_async, st));
+      int rc = check_launch("conv1d weight gradient (matrix cores, asynchronous tiles)");
+      if (rc || !md.reduce) return rc;
+      hipLaunchKernelGGL(reduce_parts_kernel<float>, dim3((width + 15) / 16), dim3(256), 0, st, ablk * WPa, width,
+                         part, dw, db, Co * ncw, ncw, cb.xC * K);
+      return check_launch("conv1d weight gradient reduction");
+    }
+  }
+  if (md.nparts) *md.nparts = std::max(*md.nparts, nblk * WP);
+  dim3 grid(nblk), block(256);
 #define INC_WGM(M, T)                                                                                         \
   if (pack)                                                                                                   \
     hipLaunchKernelGGL((conv_wgrad_mfma_kernel<M, T, true>), grid, block, lds, st, jobs, Ci, Co, N, K, WT, WP, \
@@ -976,7 +1151,7 @@ int cu_count() {
 // Resident blocks per CU of a kernel at a dynamic LDS size, queried once per
 // (kernel, LDS size) and cached: the query is a runtime call, and a call
 // that is being captured into a hipGraph then only enqueues its kernel.
-int blocks_per_cu(const void* fn, size_t lds) {
+int blocks_per_cu(const void* fn, size_t lds, int threads) {
   struct Entry {
     const void* fn;
     size_t lds;
@@ -989,7 +1164,7 @@ int blocks_per_cu(const void* fn, size_t lds) {
   for (int i = 0; i < used; ++i)
     if (cache[i].fn == fn && cache[i].lds == lds) return cache[i].per_cu;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess) return 0;
   if (used < 64) cache[used++] = Entry{fn, lds, per_cu};
   return per_cu;
 }
