@@ -618,7 +618,13 @@ class BurgersSolverTorch:
         return J, arrs
 
     def _desc(self, u, dt, t, x=None):
+        if u.dim() != 2:
+            raise ValueError(f"state must be [batch, resolution] (got {tuple(u.shape)})")
         b, n = u.shape
+        if n != self.resolution:
+            # dx and the diffusion stencil belong to the solver's grid (the
+            # reference's dense D2 product fails on a mismatched state)
+            raise ValueError(f"state has {n} points, solver resolution is {self.resolution}")
         dtype, dev = u.dtype, u.device
         d = L.BurgersDesc()
         d.dtype = L.dtype_code(dtype)

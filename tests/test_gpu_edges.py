@@ -52,6 +52,13 @@ def test_unsupported_sizes_raise_value_error(cuda):
     sv = BurgersSolverTorch(BGSimParams(dt=0.01, resolution=4, batch_size=1, device="cuda"))
     with pytest.raises(ValueError, match=">= 8"):
         sv.step(torch.zeros(1, 4, device=cuda))
+    # a state whose length is not the solver's resolution (dx and the
+    # stencil belong to the solver's grid)
+    sv = BurgersSolverTorch(BGSimParams(dt=0.01, resolution=64, batch_size=2, device="cuda"))
+    with pytest.raises(ValueError, match="resolution"):
+        sv.step(torch.zeros(2, 32, device=cuda))
+    with pytest.raises(ValueError, match="resolution"):
+        sv.stability_numbers(torch.zeros(2, 128, device=cuda))
     with pytest.raises(KeyError):
         KSSolverTorch(KSSimParams([22.0], resolution=64, time_scheme="RK4", device="cuda"))
     with pytest.raises(ValueError):
