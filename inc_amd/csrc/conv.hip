@@ -200,9 +200,21 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int Nu, c
   const int lane = tid % kWave;
   const int o0 = (blockIdx.y * kWaves + wave) * CPT;
   const T* xb = x + jp.base * Ci;
-  for (int e = tid; e < Ci * XW; e += blockDim.x) {
-    int c = e / XW, j = e - c * XW;
-    xs[e] = xb[size_t(c) * N + wrapn(i0 - P + j, N)];
+  // four loads of a thread in flight before their LDS writes
+  constexpr int SU = 4;
+  for (int e0 = tid; e0 < Ci * XW; e0 += 256 * SU) {
+    T v[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const int e = e0 + u * 256;
+      if (e < Ci * XW) {
+        const int c = e / XW, j = e - c * XW;
+        v[u] = xb[size_t(c) * N + wrapn(i0 - P + j, N)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SU; ++u)
+      if (e0 + u * 256 < Ci * XW) xs[e0 + u * 256] = v[u];
   }
   __syncthreads();
   if (o0 >= Co) return;
@@ -211,19 +223,42 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int Nu, c
   for (int a = 0; a < CPT; ++a)
 #pragma unroll
     for (int p = 0; p < kPPT; ++p) acc[a][p] = T(0);
-  const int p0 = lane * kPPT;
+  // Position p of a lane: with many output channels per wavefront the
+  // lanes take positions lane, lane + 64, ... (every output store is 64
+  // consecutive values); with few, consecutive pairs (one input window of
+  // kPPT + K - 1 values serves both, fewer LDS reads per input channel).
+  constexpr bool kStrided = CPT >= 4;
+  auto pos = [&](int p) { return kStrided ? p * kWave + lane : lane * kPPT + p; };
   for (int c = 0; c < Ci; ++c) {
-    T xv[kPPT + K - 1];
+    if constexpr (kStrided) {
+      T xv[kPPT][K];
 #pragma unroll
-    for (int j = 0; j < kPPT + K - 1; ++j) xv[j] = xs[c * XW + p0 + j];
+      for (int p = 0; p < kPPT; ++p)
 #pragma unroll
-    for (int a = 0; a < CPT; ++a) {
-      const int o = o0 + a < Co ? o0 + a : Co - 1;  // clamp: extra channels are computed and dropped
+        for (int k = 0; k < K; ++k) xv[p][k] = xs[c * XW + pos(p) + k];
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const T wv = FLIP ? w[(size_t(c) * Co + o) * K + (K - 1 - k)] : w[(size_t(o) * Ci + c) * K + k];
+      for (int a = 0; a < CPT; ++a) {
+        const int o = o0 + a < Co ? o0 + a : Co - 1;  // clamp: extra channels are computed and dropped
 #pragma unroll
-        for (int p = 0; p < kPPT; ++p) acc[a][p] += wv * xv[p + k];
+        for (int k = 0; k < K; ++k) {
+          const T wv = FLIP ? w[(size_t(c) * Co + o) * K + (K - 1 - k)] : w[(size_t(o) * Ci + c) * K + k];
+#pragma unroll
+          for (int p = 0; p < kPPT; ++p) acc[a][p] += wv * xv[p][k];
+        }
+      }
+    } else {
+      T xv[kPPT + K - 1];
+#pragma unroll
+      for (int j = 0; j < kPPT + K - 1; ++j) xv[j] = xs[c * XW + pos(0) + j];
+#pragma unroll
+      for (int a = 0; a < CPT; ++a) {
+        const int o = o0 + a < Co ? o0 + a : Co - 1;  // clamp: extra channels are computed and dropped
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const T wv = FLIP ? w[(size_t(c) * Co + o) * K + (K - 1 - k)] : w[(size_t(o) * Ci + c) * K + k];
+#pragma unroll
+          for (int p = 0; p < kPPT; ++p) acc[a][p] += wv * xv[p + k];
+        }
       }
     }
   }
@@ -235,7 +270,7 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int Nu, c
     const size_t row = size_t(jp.base) * Co + size_t(o) * N;
 #pragma unroll
     for (int p = 0; p < kPPT; ++p) {
-      const int i = i0 + p0 + p;
+      const int i = i0 + pos(p);
       if (i < N) {
         T v = acc[a][p] + bo;
         if (mul) v *= mul[row + i];
