@@ -728,7 +728,9 @@ def extra_ks_mixed_radix(b=256, h=200, sizes=(250, 256, 384, 512)):
             e.record()
             torch.cuda.synchronize()
             best = min(best, a.elapsed_time(e) * 1e-3)
-        kind = "register" if n & (n - 1) == 0 else "mixed-radix"
+        pm = [q for q in (3, 5) if n % q == 0 and 32 <= n // q <= 256 and (n // q) & (n // q - 1) == 0]
+        kind = ("register" if n & (n - 1) == 0 else
+                (f"radix-{pm[0]} step + register transforms" if pm else "mixed-radix through LDS"))
         out[f"N{n}"] = {"transform": kind, "ms": best * 1e3, "trajectory_steps_per_s": b * h / best}
     return out
 

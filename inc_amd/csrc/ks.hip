@@ -24,8 +24,10 @@
 //    the lanes of R = 4 (half the instructions per lane), which wins while
 //    the batch leaves SIMDs idle; R = 4 wins once every SIMD has work.
 //  * Resolutions that are not powers of two in [16, 1024] run the same
-//    scheme kernels over a mixed-radix transform through LDS (MixX below);
-//    the kernels are written once over the transform policy.
+//    scheme kernels over a mixed-radix transform through LDS (MixX below),
+//    or, in float32 at N = 3 * 2^k and 5 * 2^k, over a radix-3/5 register
+//    step before power-of-two register transforms (MixRegX); the kernels are
+//    written once over the transform policy.
 //  * Spectral quantities are kept scaled by 1/N, so the unnormalised inverse
 //    transform returns physical values directly.
 //  * The state stays in registers across all steps of a launch; HBM sees
@@ -402,8 +404,9 @@ __device__ __forceinline__ void reg_fft_inv(cpx<T>* v, const cpx<T>* tw, XBuf<T>
   }
 }
 
-// per-lane twiddles of the register transform
-template <typename T, int N, int R>
+// per-lane twiddles of the register transform; tab holds W_{S N}^i, read at
+// every S-th entry (S > 1: the N-point transform inside a longer one)
+template <typename T, int N, int R, int S = 1>
 __device__ __forceinline__ void load_reg_twiddles(const T* tab, int j, cpx<T>* tw) {
   using PL = RPlan<N, R>;
 #pragma unroll
@@ -414,7 +417,7 @@ __device__ __forceinline__ void load_reg_twiddles(const T* tab, int j, cpx<T>* t
 #pragma unroll
     for (int r = 1; r < P; ++r) {
       const int k = (nrest * r * scale) & (N - 1);
-      tw[PL::tw_off(s) + r - 1] = cpx<T>{tab[2 * k], tab[2 * k + 1]};
+      tw[PL::tw_off(s) + r - 1] = cpx<T>{tab[2 * S * k], tab[2 * S * k + 1]};
     }
   }
 }
@@ -683,6 +686,69 @@ struct MixX {
 #pragma unroll
     for (int m = 0; m < R; ++m) v[m] = j + m * TL < nn ? src[j + m * TL] : cpx<T>{T(0), T(0)};
     __syncthreads();  // the next transform writes the buffers again
+  }
+};
+
+// N = P * M (P = 3 or 5, M a power of two): one radix-P step over the
+// registers, then P independent M-point register transforms.  Lane j holds
+// the points j + m*TL, m < P*RS (TL = M / RS), so register mm + RS*n2 holds
+// x[n1 + M*n2] with n1 = j + mm*TL.  Forward (decimation in frequency):
+//   X[P k1 + r] = sum_n1 W_M^{n1 k1} (W_N^{n1 r} sum_n2 x[n1 + M n2] W_P^{n2 r}),
+// the radix-P DFT and the twiddles W_N^{n1 r} in registers, then the
+// M-point transform of registers {mm + RS*r} for each r; the inverse runs
+// the steps backwards.  The P sub-transforms are independent, so their
+// instruction chains interleave.  Mode of register mm + RS*r, lane j:
+// P * spec_index<M, RS>(mm, j) + r.
+template <typename T, int P, int M, int RS>
+struct MixRegX {
+  static constexpr int R = P * RS, TL = M / RS, TPB = TL >= 64 ? 1 : 64 / TL;
+  static constexpr int LDS = FftCfg<T, M, RS>::LDS_TEAM;
+  static constexpr bool kMasked = false;
+  cpx<T> tw[FftCfg<T, M, RS>::NTW];  // the M-point transforms' twiddles
+  cpx<T> pt[RS][P - 1];              // W_N^{n1 r}, r = 1 .. P-1
+  XBuf<T> xb;
+  int j;
+  __device__ __forceinline__ MixRegX(cpx<T>* lds, int lane, const KArgs<T>& a) : xb{lds, 0}, j(lane) {
+    load_reg_twiddles<T, M, RS, P>(a.twiddle, j, tw);
+#pragma unroll
+    for (int mm = 0; mm < RS; ++mm)
+#pragma unroll
+      for (int r = 1; r < P; ++r) {
+        const int k = ((j + mm * TL) * r) % (P * M);
+        pt[mm][r - 1] = cpx<T>{a.twiddle[2 * k], a.twiddle[2 * k + 1]};
+      }
+  }
+  __device__ __forceinline__ static constexpr int n() { return P * M; }
+  __device__ __forceinline__ int spec(int m) const { return P * spec_index<M, RS>(m % RS, j) + m / RS; }
+  template <int SIGN>
+  __device__ __forceinline__ void fft(cpx<T>* v) {
+    if constexpr (SIGN < 0) {
+#pragma unroll
+      for (int mm = 0; mm < RS; ++mm) {
+        cpx<T> x[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) x[q] = v[mm + RS * q];
+        dft_mix<P, -1>(x);
+        v[mm] = x[0];
+#pragma unroll
+        for (int r = 1; r < P; ++r) v[mm + RS * r] = cmul(x[r], pt[mm][r - 1]);
+      }
+#pragma unroll
+      for (int r = 0; r < P; ++r) reg_transform<T, M, RS, -1>(v + RS * r, tw, xb, j);
+    } else {
+#pragma unroll
+      for (int r = 0; r < P; ++r) reg_transform<T, M, RS, +1>(v + RS * r, tw, xb, j);
+#pragma unroll
+      for (int mm = 0; mm < RS; ++mm) {
+        cpx<T> x[P];
+        x[0] = v[mm];
+#pragma unroll
+        for (int r = 1; r < P; ++r) x[r] = cmulc(v[mm + RS * r], pt[mm][r - 1]);
+        dft_mix<P, +1>(x);
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[mm + RS * q] = x[q];
+      }
+    }
   }
 };
 
@@ -1283,6 +1349,30 @@ uint64_t mix_plan(int n) {
   return code;
 }
 
+// float32 resolutions P * M (P = 3 or 5, M a power of two in [32, 256]):
+// the radix-P step in registers before P register transforms of M points,
+// two points per lane each (TL = M / 2 lanes per pair).
+template <int P>
+void* pick_mixreg_p(int m, bool adj, bool loss) {
+  switch (m) {
+    case 32: return pick_kernel<float, MixRegX<float, P, 32, 2>>(adj, loss);
+    case 64: return pick_kernel<float, MixRegX<float, P, 64, 2>>(adj, loss);
+    case 128: return pick_kernel<float, MixRegX<float, P, 128, 2>>(adj, loss);
+    case 256: return pick_kernel<float, MixRegX<float, P, 256, 2>>(adj, loss);
+    default: return nullptr;
+  }
+}
+int mixreg_m(int n, int& p) {  // M of n = p * M, or 0
+  for (int q : {3, 5}) {
+    const int m = n / q;
+    if (n % q == 0 && m >= 32 && m <= 256 && (m & (m - 1)) == 0 && q * m <= kMaxN) {
+      p = q;
+      return m;
+    }
+  }
+  return 0;
+}
+
 // Team of the mixed-radix transform: two points per lane on one, two or
 // four wavefronts up to 512 points, four per lane on four wavefronts up to
 // 1024.
@@ -1334,8 +1424,15 @@ int ks_launch(const inc_ks_desc* d, const void* u, const void* s, void* out, int
     tl = d->n / r;
   } else {
     // the mixed-radix teams take no radix choice
-    fn = pick_mix<T>(d->n, adj, mse.target != nullptr);
-    tl = mix_team(d->n);
+    int p = 0;
+    const int m = std::is_same<T, float>::value ? mixreg_m(d->n, p) : 0;
+    if (m) {
+      fn = p == 3 ? pick_mixreg_p<3>(m, adj, mse.target != nullptr) : pick_mixreg_p<5>(m, adj, mse.target != nullptr);
+      tl = m / 2;
+    } else {
+      fn = pick_mix<T>(d->n, adj, mse.target != nullptr);
+      tl = mix_team(d->n);
+    }
   }
   INC_REQUIRE(fn != nullptr, INC_EUNSUPPORTED, "no KS kernel for n=%d radix=%d", d->n, r);
   KArgs<T> a{};

@@ -1,6 +1,8 @@
 """KS at resolutions that are not powers of two in [16, 1024] (the reference
 takes any N through torch.fft, solvers/solver_1d.py:208-210): the
-mixed-radix transform through LDS against the FP64 oracle, per scheme, for
+mixed-radix transform through LDS (and, in float32 at P * 2^k, the radix-P
+register step before power-of-two register transforms) against the FP64
+oracle, per scheme, for
 odd, prime, mixed-radix and small n, packed pairs and unpaired or mixed-L
 batches, the fused rollout + MSE, the INC rollout with a correction
 network, and resolution independence against the power-of-two kernels.
@@ -75,6 +77,70 @@ def test_mixed_radix_fp32(cuda, n):
     assert rel_l2(*st) < 1e-5
     assert rel_l2(*gu) < 1e-4
     assert rel_l2(*gs) < 1e-4
+
+
+# float32 sizes P * 2^k (P = 3, 5; 2^k in [32, 256]) run a radix-P step in
+# registers before P power-of-two register transforms
+REG_MIXED = [96, 192, 384, 768, 160, 320, 640]
+
+
+@pytest.mark.parametrize("n", REG_MIXED)
+@pytest.mark.parametrize("scheme", ["ETD1", "ETRK2", "ETDRK4"])
+def test_register_mixed_radix_fp32(cuda, n, scheme):
+    # packed pairs with an unpaired last trajectory, and per-trajectory L
+    for Ls in ([22.0 * n / 64] * 5, [22.0 * n / 64, 30.0 * n / 64, 40.0 * n / 64]):
+        st, gu, gs = _case(n, scheme, Ls, torch.float32, seed=3 * n, H=3)
+        assert rel_l2(*st) < 1e-5, Ls
+        assert rel_l2(*gu) < 1e-4, Ls
+        assert rel_l2(*gs) < 1e-4, Ls
+
+
+def test_register_mixed_radix_fused_mse_and_resolution_independence(cuda):
+    # the loss-fused kernels on the register path, and a resolved state at
+    # n = 96, 192, 384 against the power-of-two kernel at 512 (float32)
+    b, n, L, H = 5, 192, 40.0, 6
+    gen = torch.Generator().manual_seed(23)
+    u = modes(b, n, L, gen)
+    target = 0.3 * torch.randn(b, H, n, generator=gen, dtype=torch.float64)
+    so = KSOracle(ks_params([L] * b, resolution=n, dt=0.1, time_scheme="ETDRK4"))
+    so.doublestep = False
+    uo = u.clone().requires_grad_(True)
+    cur, outs = uo, []
+    for _ in range(H):
+        cur = so.step(cur, so.domain_size)
+        outs.append(cur)
+    ref = torch.stack(outs, 1)
+    ref_loss = ((ref - target) ** 2).mean()
+    ref_loss.backward()
+    sv = gpu_solver([L], n, 0.1, "ETDRK4", dtype=torch.float32)
+    sv.doublestep = False
+    ug = u.float().to(cuda).requires_grad_(True)
+    loss, traj = sv.rollout_mse(ug, target.float().to(cuda), torch.tensor([L], device=cuda))
+    loss.backward()
+    assert abs(float(loss) - float(ref_loss)) < 1e-5 * float(ref_loss)
+    assert rel_l2(traj, ref.detach()) < 1e-5
+    assert rel_l2(ug.grad, uo.grad) < 1e-4
+    Lr, dt, steps = 22.0, 0.1, 10
+
+    def run(m):
+        x = torch.arange(m, dtype=torch.float64) * (Lr / m)
+        u0 = (0.5 * torch.cos(2 * math.pi * x / Lr) + 0.25 * torch.sin(4 * math.pi * x / Lr + 0.3))[None]
+        svm = gpu_solver([Lr], m, dt, "ETDRK4", dtype=torch.float32)
+        svm.doublestep = False
+        return svm.rollout(u0.float().to(cuda), steps, torch.tensor([Lr], device=cuda))[0, -1].double().cpu()
+
+    def resample(v, m):
+        k = v.shape[-1]
+        V = torch.fft.fft(v)
+        h = (min(k, m) - 1) // 2
+        out = torch.zeros(m, dtype=torch.complex128)
+        out[:h + 1] = V[:h + 1]
+        out[m - h:] = V[k - h:]
+        return torch.fft.ifft(out).real * (m / k)
+
+    ref512 = run(512)
+    for m in (96, 192, 384):
+        assert rel_l2(resample(run(m), 512), ref512) < 1e-6, m
 
 
 def test_mixed_radix_double_step_and_correction_u(cuda):
