@@ -12,7 +12,9 @@ the kernels' coefficient load follows.  This is the algorithm of
 inc_amd/csrc/ks.hip (RPlan, spec_bits, load_reg_twiddles, reg_fft_fwd/inv);
 the kernels themselves are checked against the oracle by the GPU tests.
 The mixed-radix transform of the other resolutions (MixX, mix_plan,
-plan_radix, mix_pass, mix_pass_any) is restated at the end of this file.
+plan_radix, mix_pass, mix_pass_any) and the radix-P register step before
+power-of-two register transforms (MixRegX) are restated at the end of this
+file.
 """
 import numpy as np
 import pytest
@@ -213,3 +215,34 @@ def test_mixed_radix_transform_matches_numpy(n):
     x = g.standard_normal(n) + 1j * g.standard_normal(n)
     np.testing.assert_allclose(mix_fft(x, -1), np.fft.fft(x), rtol=0, atol=1e-12 * n)
     np.testing.assert_allclose(mix_fft(x, +1), np.fft.ifft(x) * n, rtol=0, atol=1e-12 * n)
+
+
+# ---- radix-P register step before power-of-two register transforms (MixRegX) ----
+@pytest.mark.parametrize("p,m", [(3, 32), (3, 128), (3, 256), (5, 32), (5, 64), (5, 128)])
+def test_radix_p_register_step_matches_numpy(p, m):
+    # N = p*m, rs = 2 registers per sub-transform, tl = m / 2 lanes: register
+    # mm + rs*n2 of lane j holds x[j + mm*tl + m*n2].  Forward: the radix-p DFT
+    # over n2 and the twiddles W_N^{n1 r} (n1 = j + mm*tl) in registers, then
+    # the m-point register transform of registers {mm + rs*r}; the mode of
+    # register mm + rs*r, lane j is p * spec_index(mm, j) + r.  The inverse
+    # runs the steps backwards.
+    n, rs = p * m, 2
+    tl = m // rs
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    v = x.reshape(p * rs, tl)                  # register mm + rs*n2, lane j
+    n1 = np.arange(tl)[None, :] + np.arange(rs)[:, None] * tl     # [rs, tl]
+    Wp = np.exp(-2j * np.pi * np.outer(np.arange(p), np.arange(p)) / p)
+    y = np.einsum("rq,qaj->raj", Wp, v.reshape(p, rs, tl))          # radix-p over n2
+    y = y * np.exp(-2j * np.pi * np.arange(p)[:, None, None] * n1[None] / n)
+    X = np.stack([forward(y[r], m, rs) for r in range(p)])          # [p, rs, tl]
+    k1 = spec_index(m, rs)
+    modes = p * k1[None] + np.arange(p)[:, None, None]
+    assert sorted(modes.ravel()) == list(range(n))
+    ref = np.fft.fft(x)
+    np.testing.assert_allclose(X, ref[modes], rtol=0, atol=1e-12 * np.abs(ref).max())
+    # inverse: m-point inverses, conjugate twiddles, inverse radix-p
+    z = np.stack([inverse(X[r], m, rs) for r in range(p)])
+    z = z * np.exp(2j * np.pi * np.arange(p)[:, None, None] * n1[None] / n)
+    back = np.einsum("qr,raj->qaj", np.conj(Wp), z).reshape(p * rs, tl)
+    np.testing.assert_allclose(back, n * v, rtol=0, atol=1e-11 * n * np.abs(v).max())
