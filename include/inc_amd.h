@@ -326,8 +326,9 @@ int inc_burgers_ragged_adj(const inc_ragged_desc* d, const void* stages, const v
 /*
  * Description of a KS problem (KSSolverTorch, solver_1d.py:198-212).
  * 4 <= n <= 1024: powers of two from 16 run the register-transpose FFT
- * (radix chosen by `radix`), every other n a mixed-radix transform through
- * LDS (`radix` ignored).
+ * (radix chosen by `radix`); float32 n = 3 * 2^k or 5 * 2^k (96 .. 768) a
+ * radix-3/5 register step before power-of-two register transforms; every
+ * other n a mixed-radix transform through LDS (`radix` ignored for both).
  */
 typedef struct inc_ks_desc {
   int32_t dtype;      /* INC_F32 / INC_F64 */
