@@ -1375,7 +1375,7 @@ int wgrad_launch_k(int njobs, int Ci, int Co, int N, int TI, int nblk, size_t ld
 // kernels do.
 constexpr int kThinWaves = 8;
 
-template <typename T, int CPW, bool ONE_IN>
+template <typename T, int CPW, bool ONE_IN, int KM>
 __global__ void __launch_bounds__(64 * kThinWaves) conv_wgrad_thin_kernel(int njobs, int C, int Nu, int K,
                                                                        const T* __restrict__ g,
                                                                        const T* __restrict__ x,
@@ -1384,12 +1384,12 @@ __global__ void __launch_bounds__(64 * kThinWaves) conv_wgrad_thin_kernel(int nj
   const int lane = threadIdx.x & 63;
   const int c0 = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)) * CPW;
   const int P = K / 2;
-  T acc[CPW][9], bacc[CPW];
+  T acc[CPW][KM], bacc[CPW];
 #pragma unroll
   for (int cc = 0; cc < CPW; ++cc) {
     bacc[cc] = T(0);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) acc[cc][k] = T(0);
+    for (int k = 0; k < KM; ++k) acc[cc][k] = T(0);
   }
   constexpr int H = kRowTile / 64;  // positions per lane and job
   const int tiles = (Nu + kRowTile - 1) / kRowTile;
@@ -1413,11 +1413,11 @@ __global__ void __launch_bounds__(64 * kThinWaves) conv_wgrad_thin_kernel(int nj
       ok[h] = i < N;
       ic[h] = ok[h] ? i : N - 1;
     }
-    T wv[H][9], cv[H][CPW];
+    T wv[H][KM], cv[H][CPW];
 #pragma unroll
     for (int h = 0; h < H; ++h) {
 #pragma unroll
-      for (int k = 0; k < 9; ++k) {
+      for (int k = 0; k < KM; ++k) {
         int q = ONE_IN ? ic[h] + k - P : ic[h] - k + P;  // in [-P, N + P): one wrap (K <= N)
         q = q < 0 ? q + N : (q >= N ? q - N : q);
         wv[h][k] = k < K ? (ONE_IN ? xb : gb)[q] : T(0);
@@ -1430,7 +1430,7 @@ __global__ void __launch_bounds__(64 * kThinWaves) conv_wgrad_thin_kernel(int nj
     for (int h = 0; h < H; ++h) {
       if constexpr (!ONE_IN) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) wv[h][k] = ok[h] ? wv[h][k] : T(0);
+        for (int k = 0; k < KM; ++k) wv[h][k] = ok[h] ? wv[h][k] : T(0);
       }
       if (ONE_IN) {
 #pragma unroll
@@ -1438,17 +1438,17 @@ __global__ void __launch_bounds__(64 * kThinWaves) conv_wgrad_thin_kernel(int nj
           const T v = ok[h] ? cv[h][cc] : T(0);
           bacc[cc] += v;
 #pragma unroll
-          for (int k = 0; k < 9; ++k)
+          for (int k = 0; k < KM; ++k)
             if (k < K) acc[cc][k] += v * wv[h][k];
         }
       } else {
 #pragma unroll
-        for (int k = 0; k < 9; ++k)
+        for (int k = 0; k < KM; ++k)
           if (k == P) bacc[0] += wv[h][k];  // g[i]
 #pragma unroll
         for (int cc = 0; cc < CPW; ++cc)
 #pragma unroll
-          for (int k = 0; k < 9; ++k)
+          for (int k = 0; k < KM; ++k)
             if (k < K) acc[cc][k] += cv[h][cc] * wv[h][k];
       }
     }
@@ -1458,7 +1458,7 @@ __global__ void __launch_bounds__(64 * kThinWaves) conv_wgrad_thin_kernel(int nj
 #pragma unroll
   for (int cc = 0; cc < CPW; ++cc) {
 #pragma unroll
-    for (int k = 0; k < 9; ++k)
+    for (int k = 0; k < KM; ++k)
       if (k < K) acc[cc][k] = wave_sum(acc[cc][k], kWave);
     bacc[cc] = wave_sum(bacc[cc], kWave);
   }
@@ -1468,7 +1468,7 @@ __global__ void __launch_bounds__(64 * kThinWaves) conv_wgrad_thin_kernel(int nj
     const int c = c0 + cc;
     if (c >= C) break;
 #pragma unroll
-    for (int k = 0; k < 9; ++k)
+    for (int k = 0; k < KM; ++k)
       if (k < K) {
         T* dst = out + c * K + k;
         *dst = accumulate ? *dst + acc[cc][k] : acc[cc][k];
@@ -1503,13 +1503,22 @@ int conv_wgrad_thin(int B, int Ci, int Co, int N, int K, const T* g, const T* x,
   const int cpw = (C + kThinWaves - 1) / kThinWaves;
   const int acc = md.accumulate ? 1 : 0;
   dim3 grid(nblk), block(64 * kThinWaves);
-#define INC_WT(CPW)                                                                                              \
-  if (one_in)                                                                                                    \
-    hipLaunchKernelGGL((conv_wgrad_thin_kernel<T, CPW, true>), grid, block, 0, st, jobs, C, N, K, g, x, part, acc, \
-                       md.rt);                                                                                   \
-  else                                                                                                           \
-    hipLaunchKernelGGL((conv_wgrad_thin_kernel<T, CPW, false>), grid, block, 0, st, jobs, C, N, K, g, x, part,    \
+  // taps held per lane: 7 covers K <= 7 (fewer registers, more resident blocks), 9 the rest
+#define INC_WT_K(CPW, KM)                                                                                           \
+  if (one_in)                                                                                                       \
+    hipLaunchKernelGGL((conv_wgrad_thin_kernel<T, CPW, true, KM>), grid, block, 0, st, jobs, C, N, K, g, x, part,    \
+                       acc, md.rt);                                                                                 \
+  else                                                                                                              \
+    hipLaunchKernelGGL((conv_wgrad_thin_kernel<T, CPW, false, KM>), grid, block, 0, st, jobs, C, N, K, g, x, part,   \
                        acc, md.rt)
+#define INC_WT(CPW)          \
+  do {                       \
+    if (K <= 7) {            \
+      INC_WT_K(CPW, 7);      \
+    } else {                 \
+      INC_WT_K(CPW, 9);      \
+    }                        \
+  } while (0)
   if (cpw <= 1) INC_WT(1);
   else if (cpw <= 2) INC_WT(2);
   else if (cpw <= 3) INC_WT(3);
@@ -1517,6 +1526,7 @@ int conv_wgrad_thin(int B, int Ci, int Co, int N, int K, const T* g, const T* x,
   else if (cpw <= 6) INC_WT(6);
   else INC_WT(8);
 #undef INC_WT
+#undef INC_WT_K
   int rc = check_launch("conv1d weight gradient (one-channel side)");
   if (rc || !md.reduce) return rc;
   hipLaunchKernelGGL(reduce_parts_kernel<T>, dim3((width + 15) / 16), dim3(256), 0, st, nblk, width, part, dw, db,
