@@ -585,10 +585,11 @@ def _cpu_time(fn, threads, reps=1, warmup=False):
 
 def cpu_baselines():
     """The FP64 oracle (a PyTorch-CPU restatement of the reference's op
-    sequence) on a bounded sample of every configuration, forward + adjoint
+    sequence) on a bounded sample of every workload, forward + adjoint
     (autograd through the oracle), on 1 host thread and on all of them.
-    cfg0 is the reference's own algorithm exactly (Euler, WENO5 with LF
-    splitting, dense-matrix diffusion) at its full size."""
+    burgers_reference_B1_N256 is the reference's own algorithm exactly
+    (Euler, WENO5 with LF splitting, dense-matrix diffusion) at its full
+    size."""
     import torch
     from oracle import BurgersOracle, KSOracle, burgers_params, ks_params
     from oracle import simulate_rollout as o_rollout
@@ -596,7 +597,7 @@ def cpu_baselines():
     cores = len(os.sched_getaffinity(0))
     out = {"cpu_model": cpu_model(), "cores": cores}
 
-    def cfg0():
+    def burgers_reference():
         so = BurgersOracle(burgers_params(256, 0.005, nu=0.01, batch_size=1))
         u = sine_state(256, 1000)[None].clone().requires_grad_(True)
         cur, outs = u, []
@@ -605,7 +606,7 @@ def cpu_baselines():
             outs.append(cur)
         torch.stack(outs, 1).square().mean().backward()
 
-    def cfg0_fwd():
+    def burgers_reference_fwd():
         so = BurgersOracle(burgers_params(256, 0.005, nu=0.01, batch_size=1))
         with torch.no_grad():
             cur = sine_state(256, 1000)[None]
@@ -634,7 +635,7 @@ def cpu_baselines():
     ns, nus, dts, hz = ragged_workload()
     sample = list(range(0, len(ns), len(ns) // 4))[:4]
 
-    def cfg4():
+    def uneven_inc():
         for i in sample:
             so = BurgersOracle(burgers_params(ns[i], dts[i], nu=nus[i], batch_size=1, time_scheme="SSPRK3"))
             u = sine_state(ns[i], 3000 + i)[None].clone().requires_grad_(True)
@@ -642,14 +643,16 @@ def cpu_baselines():
             res.square().mean().backward()
 
     cases = {
-        "cfg0": (cfg0, 200, "reference algorithm: Burgers Euler/WENO5/dense D2, B=1, N=256, 200 steps, fp64"),
-        "cfg0_forward_only": (cfg0_fwd, 200, "the same, forward only"),
-        "cfg1": (lambda: ks_case(256, 256, 64.0, 2), 256 * 2, "KS ETDRK4, B=256, N=256, 2 of 1000 steps"),
-        "cfg2": (lambda: burgers_inc(16, 128, 16, cnn2), 16 * 16,
-                 "Burgers INC + CNN(4x32,k5), 16 of 512 trajectories, N=128, 16 steps"),
-        "cfg3": (lambda: ks_case(2, 512, 100.0, 8, cnn=cnn3), 2 * 8,
-                 "KS INC + CNN(6x48,k7), 2 of 1024 trajectories, N=512, 8 of 64 steps"),
-        "cfg4": (cfg4, sum(hz[i] for i in sample),
+        "burgers_reference_B1_N256": (burgers_reference, 200,
+                                      "reference algorithm: Burgers Euler/WENO5/dense D2, B=1, N=256, 200 steps, fp64"),
+        "burgers_reference_B1_N256_forward_only": (burgers_reference_fwd, 200, "the same, forward only"),
+        "ks_etdrk4_B256_N256": (lambda: ks_case(256, 256, 64.0, 2), 256 * 2,
+                                "KS ETDRK4, B=256, N=256, 2 of 1000 steps"),
+        "burgers_inc_B512_N128": (lambda: burgers_inc(16, 128, 16, cnn2), 16 * 16,
+                                  "Burgers INC + CNN(4x32,k5), 16 of 512 trajectories, N=128, 16 steps"),
+        "ks_inc_B1024_N512": (lambda: ks_case(2, 512, 100.0, 8, cnn=cnn3), 2 * 8,
+                              "KS INC + CNN(6x48,k7), 2 of 1024 trajectories, N=512, 8 of 64 steps"),
+        "uneven_inc_2048": (uneven_inc, sum(hz[i] for i in sample),
                  "uneven Burgers INC + CNN(4x32,k5), 4 of 2048 trajectories with their own N, nu, dt, horizon"),
     }
     first = True
@@ -940,7 +943,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_baselines"] = cpu_baselines()
         if "batched" in out:
-            c4 = out["cpu_baselines"]["cfg4"]
+            c4 = out["cpu_baselines"]["uneven_inc_2048"]
             out["batched"]["cpu_baseline"] = {
                 "value": c4[f"value_{out['cpu_baselines']['cores']}_threads"], "unit": c4["unit"],
                 "cores": out["cpu_baselines"]["cores"], "kind": "port", "sample": c4["sample"]}
