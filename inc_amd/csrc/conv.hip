@@ -91,10 +91,21 @@ struct JobPos {
   int64_t base;  // first point of the row (element offset C * base)
 };
 
+// The tile lists are never written by the kernels that read them: read
+// through constant-address-space pointers at a wave-uniform index, they are
+// scalar loads (scalar cache, counted apart from the vector memory traffic)
+// instead of vector loads that wait behind every access in flight.  job is
+// wave-uniform in every caller.
+template <typename V>
+__device__ __forceinline__ V const_load(const V* p, int i) {
+  return ((const __attribute__((address_space(4))) V*)p)[i];
+}
+
 __device__ __forceinline__ JobPos job_pos(const RowTiles& rt, int job, int tiles, int TI, int N) {
   if (rt.tile) {
-    const int b = rt.tile[2 * job], i0 = rt.tile[2 * job + 1];
-    return JobPos{b, i0, rt.n[b], rt.off[b]};
+    job = __builtin_amdgcn_readfirstlane(job);
+    const int b = const_load(rt.tile, 2 * job), i0 = const_load(rt.tile, 2 * job + 1);
+    return JobPos{b, i0, const_load(rt.n, b), const_load(rt.off, b)};
   }
   const int b = job / tiles;
   return JobPos{b, (job - b * tiles) * TI, N, int64_t(b) * N};
