@@ -144,17 +144,17 @@ SIGNATURES = {
     "inc_ragged_inc_workspace": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), ctypes.POINTER(CnnDesc), ctypes.c_int,
                                                 ctypes.POINTER(ctypes.c_int64)]),
     "inc_ragged_inc_fwd": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), ctypes.POINTER(CnnDesc),
-                                          ctypes.POINTER(ConvRows), _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                          ctypes.POINTER(ConvRows), _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
                                           ctypes.c_int64, _vp]),
     "inc_ragged_inc_adj": (ctypes.c_int, [ctypes.POINTER(RaggedDesc), ctypes.POINTER(CnnDesc),
-                                          ctypes.POINTER(ConvRows), _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
-                                          _vp, ctypes.c_int64, _vp]),
+                                          ctypes.POINTER(ConvRows), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                          ctypes.c_int, _vp, ctypes.c_int64, _vp]),
     "inc_ks_inc_workspace": (ctypes.c_int, [ctypes.POINTER(KSDesc), ctypes.POINTER(CnnDesc), ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int64)]),
     "inc_ks_inc_fwd": (ctypes.c_int, [ctypes.POINTER(KSDesc), ctypes.POINTER(CnnDesc), _vp, ctypes.c_int, _vp, _vp,
-                                      _vp, _vp, ctypes.c_int64, _vp]),
+                                      _vp, _vp, _vp, ctypes.c_int64, _vp]),
     "inc_ks_inc_adj": (ctypes.c_int, [ctypes.POINTER(KSDesc), ctypes.POINTER(CnnDesc), _vp, _vp, _vp, _vp, _vp, _vp,
-                                      _vp, ctypes.c_int, _vp, ctypes.c_int64, _vp]),
+                                      _vp, _vp, ctypes.c_int, _vp, ctypes.c_int64, _vp]),
 }
 
 _lib = None

@@ -77,10 +77,12 @@ class _RaggedIncRollout(torch.autograd.Function):
     """The INC rollout of a ragged batch with a CorrectionCNN
     (inc_ragged_inc_fwd / _adj): per step one ragged convolution per network
     layer over the trajectories still inside their horizon and one ragged
-    solver step; the backward evaluates the network again at every step's
-    input state.  Returns the [steps, total] trajectory, differentiable in u0
-    and in the network's parameters (``params`` in the order
-    [w0, b0, w1, b1, ...])."""
+    solver step.  With a gradient wanted the forward keeps every step's
+    hidden activations for the backward when rb.keep_activations allows
+    (CorrectionCNN.activation_store); otherwise the backward evaluates the
+    network again at every step's input state.  Returns the [steps, total]
+    trajectory, differentiable in u0 and in the network's parameters
+    (``params`` in the order [w0, b0, w1, b1, ...])."""
 
     @staticmethod
     def forward(ctx, u0, rb, model, steps, need_grad, *params):
@@ -97,12 +99,13 @@ class _RaggedIncRollout(torch.autograd.Function):
         traj = u0.new_empty(steps, rb.total)
         ns = 3 if rb.scheme == _lib.INC_SSPRK3 else 1
         stages = u0.new_empty(steps, ns, rb.total) if need_grad else None
+        acts = model.activation_store(steps, rb.total, u0, rb.keep_activations) if need_grad else None
         rc = lib.inc_ragged_inc_fwd(ctypes.byref(d), ctypes.byref(cdesc), ctypes.byref(rows), active, _lib.ptr(u0),
-                                    steps, _lib.ptr(traj), None, _lib.ptr(stages), _lib.ptr(work), work.numel(),
-                                    _lib.stream())
+                                    steps, _lib.ptr(traj), None, _lib.ptr(stages), _lib.ptr(acts), _lib.ptr(work),
+                                    work.numel(), _lib.stream())
         _lib.check(rc, "RaggedBurgers INC rollout")
         if need_grad:
-            ctx.save_for_backward(u0, traj, stages)
+            ctx.save_for_backward(u0, traj, stages, acts)
         ctx.rb, ctx.model, ctx.steps = rb, model, steps
         ctx.param_shapes = [p.shape for p in params]
         ctx.keep = (keep_w, active)
@@ -110,7 +113,7 @@ class _RaggedIncRollout(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_traj):
-        u0, traj, stages = ctx.saved_tensors
+        u0, traj, stages, acts = ctx.saved_tensors
         rb, steps = ctx.rb, ctx.steps
         if g_traj is None:
             g_traj = traj.new_zeros(steps, rb.total)
@@ -120,15 +123,17 @@ class _RaggedIncRollout(torch.autograd.Function):
         rows, active = rb._rows(steps)
         lib = _lib.load()
         elems = ctypes.c_int64(0)
-        _lib.check(lib.inc_ragged_inc_workspace(ctypes.byref(d), ctypes.byref(cdesc), 1, ctypes.byref(elems)),
+        kind = 1 if acts is None else 2
+        _lib.check(lib.inc_ragged_inc_workspace(ctypes.byref(d), ctypes.byref(cdesc), kind, ctypes.byref(elems)),
                    "RaggedBurgers INC rollout workspace")
         work = u0.new_empty(elems.value)
         g_u0 = torch.empty_like(u0)
         sizes = [math.prod(sh) for sh in ctx.param_shapes]
         g_w = u0.new_empty(sum(sizes))
         rc = lib.inc_ragged_inc_adj(ctypes.byref(d), ctypes.byref(cdesc), ctypes.byref(rows), active, _lib.ptr(u0),
-                                    _lib.ptr(traj), _lib.ptr(stages), _lib.ptr(g_traj), _lib.ptr(g_u0),
-                                    _lib.ptr(g_w), steps, _lib.ptr(work), work.numel(), _lib.stream())
+                                    _lib.ptr(traj), _lib.ptr(stages), _lib.ptr(acts), _lib.ptr(g_traj),
+                                    _lib.ptr(g_u0), _lib.ptr(g_w), steps, _lib.ptr(work), work.numel(),
+                                    _lib.stream())
         _lib.check(rc, "RaggedBurgers INC rollout backward")
         grads = [g.view(sh) for g, sh in zip(g_w.split(sizes), ctx.param_shapes)]
         return (g_u0, None, None, None, None, *grads)
@@ -143,7 +148,7 @@ class RaggedBurgers:
     """
 
     def __init__(self, n, nu, dt, horizon=None, L=2 * math.pi, scheme="SSPRK3", convection="WENO",
-                 device="cuda", dtype=torch.float32, team=0):
+                 device="cuda", dtype=torch.float32, team=0, keep_activations="auto"):
         n = [int(v) for v in n]
         self.count = len(n)
         if not (len(nu) == len(dt) == self.count) or (horizon is not None and len(horizon) != self.count):
@@ -155,6 +160,9 @@ class RaggedBurgers:
         self.dtype, self.device, self.L = dtype, torch.device(device), float(L)
         self._team_auto = team == "auto" or (team == 0 and autotune.enabled())
         self.team = 0 if team == "auto" else int(team)
+        # the one-call INC rollout keeps every step's hidden activations for
+        # its adjoint (True, False or "auto", CorrectionCNN.activation_store)
+        self.keep_activations = keep_activations
         # internal order: grouped by n (stable), so each resolution is contiguous
         self.perm = sorted(range(self.count), key=lambda b: n[b])
         self.n = [n[b] for b in self.perm]

@@ -11,6 +11,8 @@ the matrix cores in float32, reduced deterministically; the GELU between
 layers applied while each convolution stages its input, so post-activations
 are never stored); ``backend='torch'`` uses torch.nn.Conv1d and nn.GELU.
 """
+import math
+
 import torch
 from torch import nn
 from torch.nn import functional as F
@@ -219,6 +221,26 @@ class CorrectionCNN(nn.Module):
             d.weight[i] = w.data_ptr()
             d.bias[i] = b.data_ptr()
         return d, keep
+
+    def activation_store(self, steps, total, like, keep="auto"):
+        """The buffer in which a one-call INC rollout over ``total`` state
+        values keeps every step's hidden activations and GELU derivatives
+        for its adjoint ([steps, layers - 1, 2, channels * total], ``like``'s
+        dtype and device; inc_ragged_inc_* / inc_ks_inc_* ``acts``), or None:
+        the adjoint then evaluates the network's hidden layers again at every
+        step.  ``keep``: True, False, or "auto" (keep when the buffer takes at
+        most half of the device's free memory)."""
+        if not (keep is True or keep is False or keep == "auto"):
+            raise ValueError(f"keep_activations must be True, False or 'auto' (got {keep!r})")
+        if keep is False:
+            return None
+        convs = list(self.convs)
+        shape = (steps, len(convs) - 1, 2, convs[0].weight.shape[0] * total)
+        if keep == "auto":
+            free, _ = torch.cuda.mem_get_info(like.device)
+            if math.prod(shape) * like.element_size() > free // 2:
+                return None
+        return like.new_empty(shape)
 
     def forward(self, u, L=None):
         if self.convs[0].backend == "hip":

@@ -1679,13 +1679,16 @@ constexpr int kIncSlots = 512;
 // batch, or batch x n), in elements of the state's type.  forward: two
 // activation buffers [C][tot], the correction [tot] and the scaled last
 // layer (C K + 1); adjoint: the recomputed activations and their GELU
-// derivatives ((L - 1) each), two cotangent buffers [C][tot], four [tot]
-// vectors, the scaled last layer and every layer's partial sums.
+// derivatives ((L - 1) each, not with stored activations), two cotangent
+// buffers [C][tot], four [tot] vectors, the scaled last layer and every
+// layer's partial sums.
+enum IncMode { kIncForward = 0, kIncAdjoint = 1, kIncAdjointStored = 2 };
+
 struct IncLayout {
   int64_t act[2], s, wl, y[8], dy[8], g[2], lam_s, lam, lam_sol, gx, part[8], total;
 };
 
-IncLayout inc_layout(int64_t tot, const inc_cnn_desc* cnn, bool adj) {
+IncLayout inc_layout(int64_t tot, const inc_cnn_desc* cnn, int mode) {
   IncLayout L{};
   const int64_t C = cnn->channels, K = cnn->kernel;
   int64_t o = 0;
@@ -1695,15 +1698,16 @@ IncLayout inc_layout(int64_t tot, const inc_cnn_desc* cnn, bool adj) {
     return at;
   };
   L.wl = take(C * K + 1);
-  if (!adj) {
+  if (mode == kIncForward) {
     L.act[0] = take(C * tot);
     L.act[1] = take(C * tot);
     L.s = take(tot);
   } else {
-    for (int l = 0; l < cnn->layers - 1; ++l) {
-      L.y[l] = take(C * tot);
-      L.dy[l] = take(C * tot);
-    }
+    if (mode == kIncAdjoint)
+      for (int l = 0; l < cnn->layers - 1; ++l) {
+        L.y[l] = take(C * tot);
+        L.dy[l] = take(C * tot);
+      }
     L.g[0] = take(C * tot);
     L.g[1] = take(C * tot);
     L.lam_s = take(tot);
@@ -1717,6 +1721,14 @@ IncLayout inc_layout(int64_t tot, const inc_cnn_desc* cnn, bool adj) {
   }
   L.total = o;
   return L;
+}
+
+// The workspace kind of the *_inc_workspace calls: 0 forward, 1 adjoint
+// evaluating the network again, 2 adjoint reading stored activations.
+int inc_mode_check(int kind) {
+  INC_REQUIRE(kind >= kIncForward && kind <= kIncAdjointStored, INC_EINVAL,
+              "workspace kind must be 0, 1 or 2 (got %d)", kind);
+  return INC_OK;
 }
 
 int cnn_check(const inc_cnn_desc* cnn) {
@@ -1812,11 +1824,24 @@ struct KsStep {
   }
 };
 
+// Stored activations of an INC rollout: per step and hidden layer l the
+// activation and its GELU derivative, C x tot values each, at
+// acts + ((k (L - 1) + l) 2 + {0, 1}) C tot.
+template <typename T>
+T* stored_act(T* acts, int k, int l, int which, const inc_cnn_desc* cnn, int64_t tot) {
+  const int64_t ct = int64_t(cnn->channels) * tot;
+  return acts + ((int64_t(k) * (cnn->layers - 1) + l) * 2 + which) * ct;
+}
+
+// acts (optional): the forward also writes every step's hidden activations
+// and GELU derivatives there (the per-step slots replace the two reused
+// activation buffers, so the only extra traffic is the derivatives), and
+// the adjoint reads them instead of evaluating the network again.
 template <typename T, typename Step>
 int inc_fwd(Step& sv, const inc_cnn_desc* cnn, const NetRows& nr, const T* u0, int steps, T* traj, T* corr,
-            T* stages, T* work, hipStream_t st) {
+            T* stages, T* acts, T* work, hipStream_t st) {
   const int64_t tot = sv.tot;
-  const IncLayout L = inc_layout(tot, cnn, false);
+  const IncLayout L = inc_layout(tot, cnn, kIncForward);
   const int C = cnn->channels, K = cnn->kernel, nl = cnn->layers;
   const int ns = sv.ns;
   T* wl = work + L.wl;
@@ -1827,12 +1852,18 @@ int inc_fwd(Step& sv, const inc_cnn_desc* cnn, const NetRows& nr, const T* u0, i
   if (!rc && corr) rc = fill<T>(int64_t(steps) * tot, T(0), corr, st);
   if (rc) return rc;
   T* bufs[8];
+  T* dbufs[8];
   for (int l = 0; l < nl - 1; ++l) bufs[l] = work + L.act[l & 1];
   const T* in = u0;
   for (int k = 0; k < steps; ++k) {
     const int jobs = nr.jobs(k);
     T* s = corr ? corr + int64_t(k) * tot : work + L.s;
-    rc = inc_network<T>(cnn, wl, in, bufs, nullptr, s, nr, jobs, st);
+    if (acts)
+      for (int l = 0; l < nl - 1; ++l) {
+        bufs[l] = stored_act(acts, k, l, 0, cnn, tot);
+        dbufs[l] = stored_act(acts, k, l, 1, cnn, tot);
+      }
+    rc = inc_network<T>(cnn, wl, in, bufs, acts ? dbufs : nullptr, s, nr, jobs, st);
     if (rc) return rc;
     T* out = traj + int64_t(k) * tot;
     rc = sv.fwd(k, in, s, out, stages ? stages + int64_t(k) * ns * tot : nullptr, st);
@@ -1843,12 +1874,14 @@ int inc_fwd(Step& sv, const inc_cnn_desc* cnn, const NetRows& nr, const T* u0, i
 }
 
 // g_corr (optional, [steps][tot]): cotangent of the corrections, added to
-// the solver's cotangent of each step's correction.
+// the solver's cotangent of each step's correction.  acts (optional): what
+// inc_fwd stored there; without it the network is evaluated again at every
+// step's input state.
 template <typename T, typename Step>
 int inc_adj(Step& sv, const inc_cnn_desc* cnn, const NetRows& nr, const T* u0, const T* traj, const T* stages,
-            const T* g_traj, const T* g_corr, T* g_u0, T* g_w, int steps, T* work, hipStream_t st) {
+            const T* acts, const T* g_traj, const T* g_corr, T* g_u0, T* g_w, int steps, T* work, hipStream_t st) {
   const int64_t tot = sv.tot;
-  const IncLayout L = inc_layout(tot, cnn, true);
+  const IncLayout L = inc_layout(tot, cnn, acts ? kIncAdjointStored : kIncAdjoint);
   const int C = cnn->channels, K = cnn->kernel, nl = cnn->layers;
   const int ns = sv.ns;
   T* wl = work + L.wl;
@@ -1861,12 +1894,13 @@ int inc_adj(Step& sv, const inc_cnn_desc* cnn, const NetRows& nr, const T* u0, c
   if (!rc) rc = fill<T>(L.total - L.part[0], T(0), work + L.part[0], st);
   if (!rc) rc = fill<T>(tot, T(0), work + L.gx, st);
   if (rc) return rc;
-  T* y[8];
-  T* dy[8];
-  for (int l = 0; l < nl - 1; ++l) {
-    y[l] = work + L.y[l];
-    dy[l] = work + L.dy[l];
-  }
+  T* y[8] = {};
+  T* dy[8] = {};
+  if (!acts)
+    for (int l = 0; l < nl - 1; ++l) {
+      y[l] = work + L.y[l];
+      dy[l] = work + L.dy[l];
+    }
   T* lam = work + L.lam;
   T* lam_s = work + L.lam_s;
   T* lam_sol = work + L.lam_sol;
@@ -1888,9 +1922,17 @@ int inc_adj(Step& sv, const inc_cnn_desc* cnn, const NetRows& nr, const T* u0, c
     rc = sv.adj(k, stages + int64_t(k) * ns * tot, lam_in, lam_sol, lam_s, st);
     if (!rc && g_corr) rc = lincomb<T>(tot, T(1), lam_s, g_corr + int64_t(k) * tot, nullptr, lam_s, st);
     if (rc) return rc;
-    // the network again at the step's input state, with its GELU derivatives
-    rc = inc_network<T>(cnn, wl, in, y, dy, nullptr, nr, jobs, st);
-    if (rc) return rc;
+    // the step's hidden activations with their GELU derivatives: stored by
+    // the forward, or the network again at the step's input state
+    if (acts) {
+      for (int l = 0; l < nl - 1; ++l) {
+        y[l] = const_cast<T*>(stored_act(acts, k, l, 0, cnn, tot));
+        dy[l] = const_cast<T*>(stored_act(acts, k, l, 1, cnn, tot));
+      }
+    } else {
+      rc = inc_network<T>(cnn, wl, in, y, dy, nullptr, nr, jobs, st);
+      if (rc) return rc;
+    }
     if (jobs > 0) {
       // last layer (C -> 1): weight gradient of lam_s x y[L-2] (scaled at the end), input cotangent
       // scale W^T lam_s times the GELU derivative
@@ -2093,54 +2135,59 @@ int inc_ragged_inc_workspace(const inc_ragged_desc* d, const inc_cnn_desc* cnn, 
   INC_REQUIRE(d && cnn && elems, INC_EINVAL, "null argument");
   INC_REQUIRE(cnn->layers >= 2 && cnn->layers <= 8 && cnn->channels >= 1 && cnn->kernel >= 1, INC_EINVAL,
               "bad network description");
-  *elems = inc::inc_layout(d->total, cnn, adjoint != 0).total;
+  const int rc = inc::inc_mode_check(adjoint);
+  if (rc) return rc;
+  *elems = inc::inc_layout(d->total, cnn, adjoint).total;
   return INC_OK;
 }
 
 int inc_ragged_inc_fwd(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* rows,
                        const int32_t* active_tiles, const void* u0, int steps, void* traj, void* corr, void* stages,
-                       void* work, int64_t work_elems, void* stream) {
+                       void* acts, void* work, int64_t work_elems, void* stream) {
   inc::clear_error();
   int rc = inc::inc_check(d, cnn, rows);
   if (rc) return rc;
   INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
   INC_REQUIRE(u0 && traj && work, INC_EINVAL, "u0, traj and work are required");
-  INC_REQUIRE(work_elems >= inc::inc_layout(d->total, cnn, false).total, INC_EINVAL, "workspace too small");
+  INC_REQUIRE(work_elems >= inc::inc_layout(d->total, cnn, inc::kIncForward).total, INC_EINVAL,
+              "workspace too small");
   if (d->count == 0) return INC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   inc::RaggedStep sv{*d, d->total, d->scheme == INC_SSPRK3 ? 3 : 1};
   const inc::NetRows nr{inc::row_tiles(rows), 0, 0, active_tiles, rows->njobs};
   if (d->dtype == INC_F32)
     return inc::inc_fwd<float>(sv, cnn, nr, static_cast<const float*>(u0), steps, static_cast<float*>(traj),
-                               static_cast<float*>(corr), static_cast<float*>(stages), static_cast<float*>(work), st);
+                               static_cast<float*>(corr), static_cast<float*>(stages), static_cast<float*>(acts),
+                               static_cast<float*>(work), st);
   return inc::inc_fwd<double>(sv, cnn, nr, static_cast<const double*>(u0), steps, static_cast<double*>(traj),
-                              static_cast<double*>(corr), static_cast<double*>(stages), static_cast<double*>(work),
-                              st);
+                              static_cast<double*>(corr), static_cast<double*>(stages), static_cast<double*>(acts),
+                              static_cast<double*>(work), st);
 }
 
 int inc_ragged_inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* rows,
                        const int32_t* active_tiles, const void* u0, const void* traj, const void* stages,
-                       const void* g_traj, void* g_u0, void* g_w, int steps, void* work, int64_t work_elems,
-                       void* stream) {
+                       const void* acts, const void* g_traj, void* g_u0, void* g_w, int steps, void* work,
+                       int64_t work_elems, void* stream) {
   inc::clear_error();
   int rc = inc::inc_check(d, cnn, rows);
   if (rc) return rc;
   INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
   INC_REQUIRE(u0 && traj && stages && g_traj && g_u0 && g_w && work, INC_EINVAL, "missing buffer");
-  INC_REQUIRE(work_elems >= inc::inc_layout(d->total, cnn, true).total, INC_EINVAL, "workspace too small");
+  INC_REQUIRE(work_elems >= inc::inc_layout(d->total, cnn, acts ? inc::kIncAdjointStored : inc::kIncAdjoint).total,
+              INC_EINVAL, "workspace too small");
   if (d->count == 0) return INC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   inc::RaggedStep sv{*d, d->total, d->scheme == INC_SSPRK3 ? 3 : 1};
   const inc::NetRows nr{inc::row_tiles(rows), 0, 0, active_tiles, rows->njobs};
   if (d->dtype == INC_F32)
     return inc::inc_adj<float>(sv, cnn, nr, static_cast<const float*>(u0), static_cast<const float*>(traj),
-                               static_cast<const float*>(stages), static_cast<const float*>(g_traj), nullptr,
-                               static_cast<float*>(g_u0), static_cast<float*>(g_w), steps, static_cast<float*>(work),
-                               st);
+                               static_cast<const float*>(stages), static_cast<const float*>(acts),
+                               static_cast<const float*>(g_traj), nullptr, static_cast<float*>(g_u0),
+                               static_cast<float*>(g_w), steps, static_cast<float*>(work), st);
   return inc::inc_adj<double>(sv, cnn, nr, static_cast<const double*>(u0), static_cast<const double*>(traj),
-                              static_cast<const double*>(stages), static_cast<const double*>(g_traj), nullptr,
-                              static_cast<double*>(g_u0), static_cast<double*>(g_w), steps,
-                              static_cast<double*>(work), st);
+                              static_cast<const double*>(stages), static_cast<const double*>(acts),
+                              static_cast<const double*>(g_traj), nullptr, static_cast<double*>(g_u0),
+                              static_cast<double*>(g_w), steps, static_cast<double*>(work), st);
 }
 
 int inc_ks_inc_workspace(const inc_ks_desc* d, const inc_cnn_desc* cnn, int adjoint, int64_t* elems) {
@@ -2148,54 +2195,60 @@ int inc_ks_inc_workspace(const inc_ks_desc* d, const inc_cnn_desc* cnn, int adjo
   INC_REQUIRE(d && cnn && elems, INC_EINVAL, "null argument");
   INC_REQUIRE(cnn->layers >= 2 && cnn->layers <= 8 && cnn->channels >= 1 && cnn->kernel >= 1, INC_EINVAL,
               "bad network description");
-  *elems = inc::inc_layout(int64_t(d->batch) * d->n, cnn, adjoint != 0).total;
+  const int rc = inc::inc_mode_check(adjoint);
+  if (rc) return rc;
+  *elems = inc::inc_layout(int64_t(d->batch) * d->n, cnn, adjoint).total;
   return INC_OK;
 }
 
 int inc_ks_inc_fwd(const inc_ks_desc* d, const inc_cnn_desc* cnn, const void* u0, int steps, void* traj, void* corr,
-                   void* stages, void* work, int64_t work_elems, void* stream) {
+                   void* stages, void* acts, void* work, int64_t work_elems, void* stream) {
   inc::clear_error();
   int rc = inc::ks_inc_check(d, cnn);
   if (rc) return rc;
   INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
   INC_REQUIRE(u0 && traj && work, INC_EINVAL, "u0, traj and work are required");
   const int64_t tot = int64_t(d->batch) * d->n;
-  INC_REQUIRE(work_elems >= inc::inc_layout(tot, cnn, false).total, INC_EINVAL, "workspace too small");
+  INC_REQUIRE(work_elems >= inc::inc_layout(tot, cnn, inc::kIncForward).total, INC_EINVAL, "workspace too small");
   if (d->batch == 0) return INC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   inc::KsStep sv{d, tot, inc::ks_stages(d)};
   const inc::NetRows nr{inc::RowTiles{}, d->batch, d->n, nullptr, 0};
   if (d->dtype == INC_F32)
     return inc::inc_fwd<float>(sv, cnn, nr, static_cast<const float*>(u0), steps, static_cast<float*>(traj),
-                               static_cast<float*>(corr), static_cast<float*>(stages), static_cast<float*>(work), st);
+                               static_cast<float*>(corr), static_cast<float*>(stages), static_cast<float*>(acts),
+                               static_cast<float*>(work), st);
   return inc::inc_fwd<double>(sv, cnn, nr, static_cast<const double*>(u0), steps, static_cast<double*>(traj),
-                              static_cast<double*>(corr), static_cast<double*>(stages), static_cast<double*>(work),
-                              st);
+                              static_cast<double*>(corr), static_cast<double*>(stages), static_cast<double*>(acts),
+                              static_cast<double*>(work), st);
 }
 
 int inc_ks_inc_adj(const inc_ks_desc* d, const inc_cnn_desc* cnn, const void* u0, const void* traj,
-                   const void* stages, const void* g_traj, const void* g_corr, void* g_u0, void* g_w, int steps,
-                   void* work, int64_t work_elems, void* stream) {
+                   const void* stages, const void* acts, const void* g_traj, const void* g_corr, void* g_u0,
+                   void* g_w, int steps, void* work, int64_t work_elems, void* stream) {
   inc::clear_error();
   int rc = inc::ks_inc_check(d, cnn);
   if (rc) return rc;
   INC_REQUIRE(steps >= 1, INC_EINVAL, "steps must be >= 1 (got %d)", steps);
   INC_REQUIRE(u0 && traj && stages && g_traj && g_u0 && g_w && work, INC_EINVAL, "missing buffer");
   const int64_t tot = int64_t(d->batch) * d->n;
-  INC_REQUIRE(work_elems >= inc::inc_layout(tot, cnn, true).total, INC_EINVAL, "workspace too small");
+  INC_REQUIRE(work_elems >= inc::inc_layout(tot, cnn, acts ? inc::kIncAdjointStored : inc::kIncAdjoint).total,
+              INC_EINVAL, "workspace too small");
   if (d->batch == 0) return INC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   inc::KsStep sv{d, tot, inc::ks_stages(d)};
   const inc::NetRows nr{inc::RowTiles{}, d->batch, d->n, nullptr, 0};
   if (d->dtype == INC_F32)
     return inc::inc_adj<float>(sv, cnn, nr, static_cast<const float*>(u0), static_cast<const float*>(traj),
-                               static_cast<const float*>(stages), static_cast<const float*>(g_traj),
-                               static_cast<const float*>(g_corr), static_cast<float*>(g_u0), static_cast<float*>(g_w),
-                               steps, static_cast<float*>(work), st);
+                               static_cast<const float*>(stages), static_cast<const float*>(acts),
+                               static_cast<const float*>(g_traj), static_cast<const float*>(g_corr),
+                               static_cast<float*>(g_u0), static_cast<float*>(g_w), steps, static_cast<float*>(work),
+                               st);
   return inc::inc_adj<double>(sv, cnn, nr, static_cast<const double*>(u0), static_cast<const double*>(traj),
-                              static_cast<const double*>(stages), static_cast<const double*>(g_traj),
-                              static_cast<const double*>(g_corr), static_cast<double*>(g_u0),
-                              static_cast<double*>(g_w), steps, static_cast<double*>(work), st);
+                              static_cast<const double*>(stages), static_cast<const double*>(acts),
+                              static_cast<const double*>(g_traj), static_cast<const double*>(g_corr),
+                              static_cast<double*>(g_u0), static_cast<double*>(g_w), steps,
+                              static_cast<double*>(work), st);
 }
 
 }  // extern "C"

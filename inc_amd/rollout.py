@@ -82,11 +82,12 @@ def simulate_rollout(model, solver, u_initial, mstep, correction_term, L=None, c
         return empty, (empty.clone() if correction_term else None)
     if fused and correction_term == "INC" and _fusable_inc(model, solver, u_initial):
         return solver.rollout_inc(model, u_initial, mstep)
-    # KS INC with a CorrectionCNN: one call per direction; its adjoint keeps
-    # no activations (the network is evaluated again per step), so it needs
-    # no checkpoint segments
+    # KS INC with a CorrectionCNN: one call per direction.  Its adjoint reads
+    # every step's stored hidden activations (solver.keep_activations), or,
+    # with checkpoint_every given, keeps none and evaluates the network
+    # again per step
     if fused and correction_term == "INC" and _fusable_ks_inc(model, solver, u_initial):
-        return solver.rollout_inc(model, u_initial, mstep, L)
+        return solver.rollout_inc(model, u_initial, mstep, L, keep_activations=False if checkpoint_every else None)
     # checkpoint_every=k on a fused rollout: its adjoint recomputes the stage
     # states k steps at a time instead of keeping them for the whole rollout
     # (fixed time step; the adaptive rollout keeps the step loop for it)

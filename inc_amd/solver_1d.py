@@ -993,13 +993,15 @@ class _KSIncRollout(torch.autograd.Function):
     for a KS solver and a CorrectionCNN in one library call per direction
     (inc_ks_inc_fwd / _adj): per step the network as one convolution launch
     per layer over the batch and one ETD step with its output as the
-    correction; the backward evaluates the network again at every step's
-    input, so no activation is kept between the calls.  Returns (results,
-    corrections), both [B, steps, N], differentiable in u and in the
-    network's parameters (``params`` in the order [w0, b0, w1, b1, ...])."""
+    correction.  With a gradient wanted the forward keeps every step's
+    hidden activations for the backward as ``keep_acts`` says
+    (CorrectionCNN.activation_store); otherwise the backward evaluates the
+    network again at every step's input.  Returns (results, corrections),
+    both [B, steps, N], differentiable in u and in the network's parameters
+    (``params`` in the order [w0, b0, w1, b1, ...])."""
 
     @staticmethod
-    def forward(ctx, u, solver, model, scheme, substeps, steps, domain_size, need_grad, *params):
+    def forward(ctx, u, solver, model, scheme, substeps, steps, domain_size, need_grad, keep_acts, *params):
         ctx.set_materialize_grads(False)
         u = u.contiguous()
         b, n = u.shape
@@ -1014,35 +1016,37 @@ class _KSIncRollout(torch.autograd.Function):
         corr = u.new_empty(steps, b, n)
         nst = _KS_NS[scheme] * substeps
         stages = u.new_empty(steps, b, nst, n) if need_grad else None
+        acts = model.activation_store(steps, b * n, u, keep_acts) if need_grad else None
         rc = lib.inc_ks_inc_fwd(ctypes.byref(desc), ctypes.byref(cdesc), L.ptr(u), steps, L.ptr(traj), L.ptr(corr),
-                                L.ptr(stages), L.ptr(work), work.numel(), L.stream())
+                                L.ptr(stages), L.ptr(acts), L.ptr(work), work.numel(), L.stream())
         L.check(rc, "KSSolverTorch.rollout_inc")
         if need_grad:
             # the parameters too: an in-place update before the backward is then an autograd error
-            ctx.save_for_backward(u, traj, stages, *params)
+            ctx.save_for_backward(u, traj, stages, acts, *params)
         ctx.desc, ctx.keep, ctx.cdesc, ctx.keep_w = desc, keep, cdesc, keep_w
         ctx.steps, ctx.param_shapes = steps, [p.shape for p in params]
         return traj.transpose(0, 1), corr.transpose(0, 1)
 
     @staticmethod
     def backward(ctx, g_traj, g_corr):
-        u0, traj, stages = ctx.saved_tensors[:3]
+        u0, traj, stages, acts = ctx.saved_tensors[:4]
         g_traj = g_traj.transpose(0, 1).contiguous() if g_traj is not None else torch.zeros_like(traj)
         g_corr = g_corr.transpose(0, 1).contiguous() if g_corr is not None else None
         lib = L.load()
         elems = ctypes.c_int64(0)
-        L.check(lib.inc_ks_inc_workspace(ctypes.byref(ctx.desc), ctypes.byref(ctx.cdesc), 1, ctypes.byref(elems)),
+        kind = 1 if acts is None else 2
+        L.check(lib.inc_ks_inc_workspace(ctypes.byref(ctx.desc), ctypes.byref(ctx.cdesc), kind, ctypes.byref(elems)),
                 "KSSolverTorch.rollout_inc workspace")
         work = u0.new_empty(elems.value)
         g_u0 = torch.empty_like(u0)
         sizes = [math.prod(sh) for sh in ctx.param_shapes]
         g_w = u0.new_empty(sum(sizes))
         rc = lib.inc_ks_inc_adj(ctypes.byref(ctx.desc), ctypes.byref(ctx.cdesc), L.ptr(u0), L.ptr(traj),
-                                L.ptr(stages), L.ptr(g_traj), L.ptr(g_corr), L.ptr(g_u0), L.ptr(g_w), ctx.steps,
+                                L.ptr(stages), L.ptr(acts), L.ptr(g_traj), L.ptr(g_corr), L.ptr(g_u0), L.ptr(g_w), ctx.steps,
                                 L.ptr(work), work.numel(), L.stream())
         L.check(rc, "KSSolverTorch.rollout_inc backward")
         grads = [g.view(sh) for g, sh in zip(g_w.split(sizes), ctx.param_shapes)]
-        return (g_u0, None, None, None, None, None, None, None, *grads)
+        return (g_u0, None, None, None, None, None, None, None, None, *grads)
 
 
 class _KSRolloutMSE(torch.autograd.Function):
@@ -1111,6 +1115,9 @@ class KSSolverTorch:
         # two real trajectories per complex FFT (default) or one per FFT; the
         # unpacked form keeps a trajectory finite when its FFT partner is not
         self.pack_pairs = bool(getattr(sim_params, "pack_pairs", True))
+        # rollout_inc: keep every step's hidden activations for the adjoint
+        # (True, False or "auto", CorrectionCNN.activation_store)
+        self.keep_activations = getattr(sim_params, "keep_activations", "auto")
         radix = getattr(sim_params, "radix", 0) or 0
         self._radix_auto = radix == "auto" or (radix == 0 and autotune.enabled())
         self.radix = 0 if radix == "auto" else int(radix)
@@ -1253,14 +1260,18 @@ class KSSolverTorch:
         return self._advance(u, domain_size, scheme, 2 if self.doublestep else 1, steps=mstep, want_traj=True,
                              checkpoint_every=checkpoint_every)
 
-    def rollout_inc(self, model, u, mstep, domain_size=None):
+    def rollout_inc(self, model, u, mstep, domain_size=None, keep_activations=None):
         """simulate_rollout(model, self, u, mstep, 'INC', L) (trainer_1d.py:154-183)
         for a CorrectionCNN (any width up to 64, odd kernel size up to 9,
         2..8 layers, parameters on the GPU in the state's dtype) in one
         library call (inc_ks_inc_fwd), and its gradient in one more
         (inc_ks_inc_adj): returns (results, corrections), both [B, mstep,
-        N].  The network runs on this library's convolution kernels; the
-        adjoint keeps the step states and stages, not the activations."""
+        N].  The network runs on this library's convolution kernels.  For
+        the adjoint the forward keeps the step states and stages, and every
+        step's hidden activations as ``keep_activations`` says (True, False,
+        or "auto": when they take at most half of the free device memory;
+        without them the adjoint evaluates the network again per step);
+        ``keep_activations`` given here overrides it for this call."""
         if self.correction_u is not None:
             raise ValueError("rollout_inc does not apply correction_u; use step()")
         L.require_like(u, *model.parameters(), what="correction network parameter")
@@ -1275,7 +1286,8 @@ class KSSolverTorch:
             domain_size = self.domain_size
         params = [t for cv in model.convs for t in (cv.weight, cv.bias)]
         need = torch.is_grad_enabled() and (u.requires_grad or any(p.requires_grad for p in params))
-        traj, corr = _KSIncRollout.apply(u, self, model, scheme, substeps, mstep, domain_size, need, *params)
+        keep = self.keep_activations if keep_activations is None else keep_activations
+        traj, corr = _KSIncRollout.apply(u, self, model, scheme, substeps, mstep, domain_size, need, keep, *params)
         self.correction_s = corr[:, -1] if mstep > 0 else None
         return traj, corr
 

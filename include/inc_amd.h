@@ -510,20 +510,27 @@ int inc_upsample2_cat_bwd(int dtype, int B, int C, int n, int Cs, const void* g,
  * trajectory cotangent g_traj [steps, total] it writes g_u0 [total] and the
  * network's parameter gradient summed over the batch, g_w [P] =
  * [w0, b0, w1, b1, ...] in the weight and bias tensors' layouts.  The
- * network is evaluated again at every step's input state; no activation is
- * kept between the two calls.  work: workspace of work_elems elements of
- * the state's type, at least what inc_ragged_inc_workspace reports for the
- * forward (adjoint = 0) or the adjoint (adjoint = 1).  The weight gradients
- * are summed over steps in a fixed order (deterministic).
+ * weight gradients are summed over steps in a fixed order (deterministic).
+ * The hidden activations at every step either come from acts or are
+ * evaluated again (acts NULL in both calls):
+ *   acts [steps, L - 1, 2, C * total]  per step and hidden layer the GELU
+ *                                      output and its derivative, in the
+ *                                      convolutions' layout; written by the
+ *                                      forward, read by the adjoint
+ * Stored activations replace the adjoint's repeat of the network's hidden
+ * layers (one convolution per layer and step) at the cost of that buffer.
+ * work: workspace of work_elems elements of the state's type, at least what
+ * inc_ragged_inc_workspace reports for kind 0 (forward), 1 (adjoint
+ * evaluating the network again) or 2 (adjoint reading acts).
  */
 int inc_ragged_inc_workspace(const inc_ragged_desc* d, const inc_cnn_desc* cnn, int adjoint, int64_t* elems);
 int inc_ragged_inc_fwd(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* rows,
                        const int32_t* active_tiles, const void* u0, int steps, void* traj, void* corr, void* stages,
-                       void* work, int64_t work_elems, void* stream);
+                       void* acts, void* work, int64_t work_elems, void* stream);
 int inc_ragged_inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const inc_conv_rows* rows,
                        const int32_t* active_tiles, const void* u0, const void* traj, const void* stages,
-                       const void* g_traj, void* g_u0, void* g_w, int steps, void* work, int64_t work_elems,
-                       void* stream);
+                       const void* acts, const void* g_traj, void* g_u0, void* g_w, int steps, void* work,
+                       int64_t work_elems, void* stream);
 
 /*
  * INC training rollout of a KS batch in one call: simulate_rollout(model,
@@ -541,16 +548,16 @@ int inc_ragged_inc_adj(const inc_ragged_desc* d, const inc_cnn_desc* cnn, const 
  * stages, the trajectory cotangent g_traj [steps, batch, n] and the
  * corrections' cotangent g_corr [steps, batch, n] (or NULL) it writes g_u0
  * [batch, n] and the parameter gradient g_w [P] = [w0, b0, w1, b1, ...]
- * summed over the batch and the steps in a fixed order.  The network is
- * evaluated again at every step's input state.  work: workspace of at least
- * what inc_ks_inc_workspace reports (adjoint = 0 forward, 1 adjoint).
+ * summed over the batch and the steps in a fixed order.  acts (or NULL in
+ * both calls: the network evaluated again at every step's input state) and
+ * the workspace kinds as for inc_ragged_inc_*, with total = batch * n.
  */
 int inc_ks_inc_workspace(const inc_ks_desc* d, const inc_cnn_desc* cnn, int adjoint, int64_t* elems);
 int inc_ks_inc_fwd(const inc_ks_desc* d, const inc_cnn_desc* cnn, const void* u0, int steps, void* traj, void* corr,
-                   void* stages, void* work, int64_t work_elems, void* stream);
+                   void* stages, void* acts, void* work, int64_t work_elems, void* stream);
 int inc_ks_inc_adj(const inc_ks_desc* d, const inc_cnn_desc* cnn, const void* u0, const void* traj,
-                   const void* stages, const void* g_traj, const void* g_corr, void* g_u0, void* g_w, int steps,
-                   void* work, int64_t work_elems, void* stream);
+                   const void* stages, const void* acts, const void* g_traj, const void* g_corr, void* g_u0,
+                   void* g_w, int steps, void* work, int64_t work_elems, void* stream);
 
 #ifdef __cplusplus
 }

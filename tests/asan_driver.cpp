@@ -120,6 +120,10 @@ int main() {
   int64_t elems = 0;
   expect(inc_ragged_inc_workspace(&r, &cn, 1, &elems) == INC_OK && elems > 4 * 32 * int64_t(r.total),
          "ragged INC workspace");
+  int64_t stored = 0;
+  expect(inc_ragged_inc_workspace(&r, &cn, 2, &stored) == INC_OK && stored < elems,
+         "adjoint workspace reading stored activations is smaller");
+  expect(inc_ragged_inc_workspace(&r, &cn, 3, &stored) == INC_EINVAL && has("kind"), "workspace kind");
   cn.layers = 9;
   expect(inc_ragged_inc_workspace(&r, &cn, 0, &elems) == INC_EINVAL, "network depth");
 

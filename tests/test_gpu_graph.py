@@ -50,7 +50,10 @@ def _capture(fn):
     return graph, out
 
 
-def test_ragged_inc_training_step_graph_replays_match_eager(cuda):
+@pytest.mark.parametrize("keep", [False, True, "auto"])
+def test_ragged_inc_training_step_graph_replays_match_eager(cuda, keep):
+    # keep: the adjoint reads the forward's stored activations (True; "auto"
+    # decides inside the capture) or evaluates the network again (False)
     count, steps = 40, 12
     gen = torch.Generator().manual_seed(21)
     choices = [64, 128, 256, 512, 1024]
@@ -59,7 +62,7 @@ def test_ragged_inc_training_step_graph_replays_match_eager(cuda):
                                                                       generator=gen)).tolist()
     dts = [stable_dt(n, nu) for n, nu in zip(ns, nus)]
     hz = torch.randint(3, steps + 1, (count,), generator=gen).tolist()
-    rb = RaggedBurgers(ns, nus, dts, horizon=hz, team=128)
+    rb = RaggedBurgers(ns, nus, dts, horizon=hz, team=128, keep_activations=keep)
     u0 = rb.pack([_field(n, 500 + b) for b, n in enumerate(ns)])
     cnn = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(3), scale=0.01).cuda()
     params = list(cnn.parameters())

@@ -37,12 +37,14 @@ def _u0(b, n, Ls, seed, kmax=10):
     return torch.stack(out)
 
 
-def _run(dtype, fused, scheme="ETDRK4", doublestep=False, Ls=(64.0,), b=6, n=128, H=12, use_corr=True):
+def _run(dtype, fused, scheme="ETDRK4", doublestep=False, Ls=(64.0,), b=6, n=128, H=12, use_corr=True,
+         keep="auto", depth=3, width=16, k=5):
     Ld = torch.tensor([Ls[i % len(Ls)] for i in range(b)] if len(Ls) > 1 else list(Ls), dtype=torch.float64,
                       device="cuda")
     sv = KSSolverTorch(KSSimParams(list(Ls), resolution=n, dt=0.25, time_scheme=scheme, device="cuda"))
     sv.doublestep = doublestep
-    net = CorrectionCNN(3, 16, 5, generator=torch.Generator().manual_seed(1), scale=0.01).to("cuda", dtype)
+    sv.keep_activations = keep
+    net = CorrectionCNN(depth, width, k, generator=torch.Generator().manual_seed(1), scale=0.01).to("cuda", dtype)
     u = _u0(b, n, Ls, 2).to("cuda", dtype).requires_grad_(True)
     w = torch.randn(b, H, n, generator=torch.Generator().manual_seed(3), dtype=torch.float64).to("cuda", dtype)
     wc = torch.randn(b, H, n, generator=torch.Generator().manual_seed(4), dtype=torch.float64).to("cuda", dtype)
@@ -74,6 +76,33 @@ def test_ks_inc_one_call_two_domain_sizes_and_fp32(cuda):
     assert rel_l2(t2, t3) < 1e-5
     for a, b in zip(g2, g3):
         assert rel_l2(a, b) < 1e-4
+
+
+@pytest.mark.parametrize("dtype,n,shape", [(torch.float64, 128, (3, 16, 5)), (torch.float32, 256, (6, 48, 7)),
+                                           (torch.float32, 96, (4, 32, 3))])
+def test_ks_inc_stored_activations_match_recompute(cuda, dtype, n, shape):
+    # the forward's stored hidden activations and GELU derivatives are the
+    # very values the adjoint's repeat of the network computes (same kernels,
+    # same inputs), so both adjoints agree bit for bit
+    depth, width, k = shape
+    t0, c0, g0, _ = _run(dtype, True, n=n, b=5, H=7, keep=False, depth=depth, width=width, k=k)
+    t1, c1, g1, _ = _run(dtype, True, n=n, b=5, H=7, keep=True, depth=depth, width=width, k=k)
+    assert torch.equal(t0, t1) and torch.equal(c0, c1)
+    for i, (a, b) in enumerate(zip(g1, g0)):
+        assert torch.equal(a, b), i
+
+
+def test_activation_store_policy(cuda):
+    net = CorrectionCNN(5, 24, 5).cuda()
+    like = torch.zeros(3, device="cuda")
+    assert net.activation_store(4, 100, like, False) is None
+    a = net.activation_store(4, 100, like, True)
+    assert a.shape == (4, 4, 2, 2400) and a.dtype == like.dtype and a.is_cuda
+    assert net.activation_store(4, 100, like, "auto").shape == a.shape     # small: kept
+    free, _ = torch.cuda.mem_get_info()
+    assert net.activation_store(1, free, like, "auto") is None                # over half the free memory
+    with pytest.raises(ValueError, match="keep_activations"):
+        net.activation_store(4, 100, like, "yes")
 
 
 def test_ks_inc_one_call_without_gradient(cuda):

@@ -195,6 +195,23 @@ def test_ragged_inc_rollout_one_call_matches_per_group_path(cuda):
     assert e_one < 2e-3 and e_one < 1.2 * e_group, (e_one, e_group)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_ragged_inc_stored_activations_match_recompute(cuda, dtype):
+    # uneven batch with horizons: the forward's stored activations (rows past
+    # a horizon never written, never read) give the adjoint of the network's
+    # repeat bit for bit
+    steps = 9
+    rb, u0, cnn, target, mask = _inc_case(30, steps, dtype, seed=17, width=48, k=7, depth=5)
+    out = {}
+    for keep in (False, True):
+        rb.keep_activations = keep
+        out[keep] = _loss_and_grads(rb, u0, cnn, target, mask, steps, True)
+    a, b = out[True], out[False]
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]) and torch.equal(a[2], b[2])
+    for i, (ga, gb) in enumerate(zip(a[3], b[3])):
+        assert torch.equal(ga, gb), i
+
+
 def test_ragged_inc_rollout_gradient_central_differences(cuda):
     steps = 6
     rb, u0, cnn, target, mask = _inc_case(12, steps, torch.float64, seed=4, width=16, k=3, depth=3)
