@@ -293,7 +293,8 @@ def extra_ks_inc(step_path=True):
     """KS INC fwd+adj: L=100, N=512, dt=0.25, B=1024, 64 steps, correction
     from a 6-layer 48-channel k=7 circular CNN (output x 0.01).  The
     CorrectionCNN goes to simulate_rollout, which runs the rollout in one
-    library call per direction (inc_ks_inc_fwd / _adj); with step_path the
+    library call per direction (inc_ks_inc_fwd / _adj, every step's hidden
+    activations kept for the adjoint: 32 GB); with step_path the
     step-by-step path (the network called from Python every step,
     checkpointed every 8 steps) is timed beside it."""
     import torch
