@@ -237,7 +237,11 @@ class CorrectionCNN(nn.Module):
         convs = list(self.convs)
         shape = (steps, len(convs) - 1, 2, convs[0].weight.shape[0] * total)
         if keep == "auto":
+            # free device memory plus what torch's caching allocator holds
+            # unused (the previous step's buffer, so that every training step
+            # decides alike)
             free, _ = torch.cuda.mem_get_info(like.device)
+            free += torch.cuda.memory_reserved(like.device) - torch.cuda.memory_allocated(like.device)
             if math.prod(shape) * like.element_size() > free // 2:
                 return None
         return like.new_empty(shape)
