@@ -228,7 +228,6 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int Nu, c
       if (e0 + u * 256 < Ci * XW) xs[e0 + u * 256] = v[u];
   }
   __syncthreads();
-  if (o0 >= Co) return;
   T acc[CPT][kPPT];
 #pragma unroll
   for (int a = 0; a < CPT; ++a)
@@ -240,7 +239,39 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int Ci, int Co, int Nu, c
   // kPPT + K - 1 values serves both, fewer LDS reads per input channel).
   constexpr bool kStrided = CPT >= 4;
   auto pos = [&](int p) { return kStrided ? p * kWave + lane : lane * kPPT + p; };
-  for (int c = 0; c < Ci; ++c) {
+  if constexpr (CPT == 1) {
+    if (Co == 1 && Ci >= kWaves) {
+      // One output channel (a network's last layer, or the input gradient of
+      // its first): wavefront w sums input channels w, w + kWaves, ... for
+      // every position of the tile, and the kWaves partial sums meet in LDS
+      // (over the staged tile, Ci * XW >= kWaves * TI) in wavefront order.
+      for (int c = wave; c < Ci; c += kWaves) {
+        T xv[kPPT + K - 1];
+#pragma unroll
+        for (int j = 0; j < kPPT + K - 1; ++j) xv[j] = xs[c * XW + pos(0) + j];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const T wv = FLIP ? w[size_t(c) * K + (K - 1 - k)] : w[size_t(c) * K + k];
+#pragma unroll
+          for (int p = 0; p < kPPT; ++p) acc[0][p] += wv * xv[p + k];
+        }
+      }
+      __syncthreads();  // every wavefront is done with the staged tile
+#pragma unroll
+      for (int p = 0; p < kPPT; ++p) xs[wave * TI + pos(p)] = acc[0][p];
+      __syncthreads();
+      if (wave != 0) return;
+#pragma unroll
+      for (int p = 0; p < kPPT; ++p) {
+        T v = xs[pos(p)];
+#pragma unroll
+        for (int q = 1; q < kWaves; ++q) v += xs[q * TI + pos(p)];
+        acc[0][p] = v;
+      }
+    }
+  }
+  if (o0 >= Co) return;
+  for (int c = (CPT == 1 && Co == 1 && Ci >= kWaves) ? Ci : 0; c < Ci; ++c) {
     if constexpr (kStrided) {
       T xv[kPPT][K];
 #pragma unroll
