@@ -516,7 +516,10 @@ int inc_upsample2_cat_bwd(int dtype, int B, int C, int n, int Cs, const void* g,
  *   acts [steps, L - 1, 2, C * total]  per step and hidden layer the GELU
  *                                      output and its derivative, in the
  *                                      convolutions' layout; written by the
- *                                      forward, read by the adjoint
+ *                                      forward, read by the adjoint, which
+ *                                      must get the buffer the forward
+ *                                      filled with the same descriptor,
+ *                                      network, rows, u0 and steps
  * Stored activations replace the adjoint's repeat of the network's hidden
  * layers (one convolution per layer and step) at the cost of that buffer.
  * work: workspace of work_elems elements of the state's type, at least what
