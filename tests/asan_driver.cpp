@@ -124,6 +124,23 @@ int main() {
   expect(inc_ragged_inc_workspace(&r, &cn, 2, &stored) == INC_OK && stored < elems,
          "adjoint workspace reading stored activations is smaller");
   expect(inc_ragged_inc_workspace(&r, &cn, 3, &stored) == INC_EINVAL && has("kind"), "workspace kind");
+  // the KS INC adjoint: the smaller workspace serves only the adjoint that
+  // reads stored activations (rejected before any device call otherwise)
+  inc_ks_desc kd{};
+  kd.dtype = INC_F32;
+  kd.scheme = INC_ETDRK4;
+  kd.batch = 4;
+  kd.n = 64;
+  kd.substeps = 1;
+  kd.table = one;
+  int64_t full = 0, small = 0;
+  expect(inc_ks_inc_workspace(&kd, &cn, 1, &full) == INC_OK && inc_ks_inc_workspace(&kd, &cn, 2, &small) == INC_OK &&
+             small < full,
+         "KS INC adjoint workspaces");
+  expect(inc_ks_inc_adj(&kd, &cn, one, one, one, nullptr, one, nullptr, wone, wone, 3, wone, small, nullptr) ==
+                 INC_EINVAL &&
+             has("workspace"),
+         "adjoint without stored activations needs the larger workspace");
   cn.layers = 9;
   expect(inc_ragged_inc_workspace(&r, &cn, 0, &elems) == INC_EINVAL, "network depth");
 
