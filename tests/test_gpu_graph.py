@@ -1,7 +1,8 @@
 """hipGraph capture of rollouts (torch.cuda.graph): the ragged INC training
 step on the one-call path (inc_ragged_inc_fwd / _adj: the network as ragged
 convolutions, the ragged solver step, the elementwise cotangent of the masked
-MSE and the adjoint) and the KS ETDRK4 rollout with its adjoint.
+MSE and the adjoint), the KS ETDRK4 rollout with its adjoint, and the KS
+INC training step on its one-call path (inc_ks_inc_fwd / _adj).
 
 Inside a captured region the library makes no stream operation other than
 kernel launches: its occupancy and device-attribute queries are answered
@@ -124,3 +125,45 @@ def test_ks_rollout_and_adjoint_graph_replays_match_eager(cuda):
         assert torch.equal(u.grad, g_e), rep
         with torch.no_grad():
             _ = sv.rollout(u0 * 0.5, 5, Ld).sum().item()
+
+
+@pytest.mark.parametrize("keep", [False, True])
+def test_ks_inc_training_step_graph_replays_match_eager(cuda, keep):
+    # the KS INC one-call path (inc_ks_inc_fwd / _adj) with the network's
+    # parameter gradients; keep: the forward's stored activations in the
+    # graph's pool (True) or the network evaluated again in the adjoint
+    b, n, h, Lk = 16, 128, 8, 64.0
+    sv = KSSolverTorch(KSSimParams([Lk], resolution=n, dt=0.25, time_scheme="ETDRK4", device="cuda"))
+    sv.doublestep = False
+    sv.keep_activations = keep
+    Ld = torch.tensor([Lk], dtype=torch.float64, device=cuda)
+    gen = torch.Generator().manual_seed(8)
+    x = torch.arange(n, dtype=torch.float64) * (Lk / n)
+    u0 = sum(0.5 * torch.randn(b, 1, generator=gen, dtype=torch.float64) * torch.cos(2 * math.pi * k * x / Lk)
+             for k in range(1, 9)).float().to(cuda)
+    target = (0.9 * u0[:, None, :]).expand(b, h, n).contiguous()
+    net = CorrectionCNN(4, 32, 5, generator=torch.Generator().manual_seed(9), scale=0.01).cuda()
+    params = list(net.parameters())
+
+    def train_step():
+        traj, _ = inc_amd.simulate_rollout(net, sv, u0, h, "INC", L=Ld)
+        torch.autograd.backward(traj, (2.0 / traj.numel()) * (traj - target))
+        return traj
+
+    for p in params:
+        p.grad = None
+    traj_e = train_step().detach().clone()
+    g_e = [p.grad.clone() for p in params]
+    for p in params:
+        p.grad = None
+    graph, traj_g = _capture(train_step)
+    for rep in range(3):
+        for p in params:
+            p.grad.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(traj_g, traj_e), rep
+        for a, g in zip(params, g_e):
+            assert torch.equal(a.grad, g), rep
+        with torch.no_grad():
+            _ = sv.rollout(u0 * 0.5, 3, Ld).sum().item()
